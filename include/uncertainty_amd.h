@@ -1,0 +1,155 @@
+/*
+ * uncertainty_amd.h -- C ABI of the MI355X (gfx950) camera + point covariance
+ * pipeline.  Plain pointers and sizes; no C++, HIP or torch types cross it.
+ *
+ * It replaces the reference's computation entry point
+ *     void computeCovariances(cov::Options&, cov::Statistic&, ceres::CRSMatrix&,
+ *                             double* camUnc, double* ptUnc)
+ *     (src/uncertaintyTE/compute.h:92-97, implemented at compute.cpp:458-540)
+ * which the reference's public getCovariances (include/uncertainty.h:48-61) and
+ * its CLI (src/software/main_uncertainty.cpp:86) call.  The C++ wrapper with the
+ * reference's own signature is declared in include/uncertainty.h.
+ *
+ * Semantics follow the reference TAYLOR_EXPANSION path:
+ *   - the Jacobian is a CRS matrix with 2 rows per observation and cp+3
+ *     nonzeros per row (cp camera columns, then 3 point columns);
+ *   - three fixed points remove the point columns and every observation of
+ *     those points (compute.cpp:283-368);
+ *   - output: cp(cp+1)/2 upper-triangle values per camera, row-major per block
+ *     (compute.cpp:168-183, including its (X[j,k] + X[k,j]/2) form), and 6
+ *     values (xx, xy, xz, yy, yz, zz) per point (compute.cpp:443-455), fixed
+ *     points left at zero.
+ * Differences from the reference: errors are returned as codes instead of
+ * exit(1)/segfaults, stage times are true seconds (the reference prints ten
+ * times the seconds, compute.cpp:33), and the call is reentrant per context.
+ */
+#ifndef UNCERTAINTY_AMD_H
+#define UNCERTAINTY_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UNC_ABI_VERSION 1
+
+/* error codes */
+#define UNC_OK 0
+#define UNC_ERR_BAD_INPUT 1        /* layout/size/ids inconsistent with the options      */
+#define UNC_ERR_NO_FIXED_POINTS 2  /* reference crashes here (compute.cpp:374-378, 446)  */
+#define UNC_ERR_SINGULAR 3         /* zero Jacobian column or singular 3x3 point block   */
+#define UNC_ERR_OOM 4              /* device memory (or admission) refused               */
+#define UNC_ERR_DEVICE 5           /* HIP runtime error                                  */
+#define UNC_ERR_UNSUPPORTED 6      /* option combination not implemented on the device   */
+#define UNC_ERR_NUMERIC 7          /* NaN/Inf in the result                              */
+
+/* cov::Options::_algorithm values (include/uncertainty.h:11-13) */
+#define UNC_ALG_SVD_QR_ITERATION 0
+#define UNC_ALG_SVD_DEVIDE_AND_CONQUER 1
+#define UNC_ALG_TAYLOR_EXPANSION 2
+
+/* arithmetic of the Z inverse (the only stage whose precision is selectable) */
+#define UNC_PREC_FP64 0   /* plain FP64 TE path: reference output, <=1e-9 per block     */
+#define UNC_PREC_MIXED 1  /* gauge-fixed Z^+ with FP32 MFMA + FP64 refinement, <=1e-5 */
+
+typedef struct unc_options {
+  int algorithm;      /* UNC_ALG_*; the device implements TAYLOR_EXPANSION           */
+  double epsilon;     /* cov::Options::_epsilon (SVD modes only)                      */
+  double lambda;      /* -1: 10^(-1.2653 log10(N) - 2.9415), compute.cpp:123-125    */
+  int num_cams;
+  int cam_params;     /* cp, 1..9                                                     */
+  int num_points;
+  int num_obs;        /* informational (header of the .jacob)                         */
+  int svd_remove_n;   /* SVD modes only                                               */
+  int max_iter_te;    /* unused by the reference; kept for layout parity              */
+  int pts2fix[3];     /* fixed point ids, -1 -1 -1 = none                             */
+  int precision;      /* UNC_PREC_*                                                   */
+  int refine;         /* FP64 path: one Newton step with a compensated residual on
+                         (Z + lambda I)^-1 (default 1)                                */
+  int device;         /* HIP device ordinal                                           */
+  int flags;          /* reserved, 0                                                  */
+} unc_options;
+
+typedef struct unc_stats {
+  /* cov::Statistic times (include/uncertainty.h:36), in true seconds */
+  double time_create_j, time_fix_j, time_norm_j, time_multiply_jj, time_split_jj,
+      time_inv_v, time_compose_z, time_inv_z, time_te, time_pts_unc, time_all;
+  double lambda;
+  int fixed_pts[3];        /* after the reference's partial sort (compute.cpp:285) */
+  int num_terms;           /* accepted Taylor terms = cycle_change size            */
+  double cycle_change[20];
+  /* device-side measurements */
+  double gemm_ms;          /* summed duration of the dense GEMM launches (HIP events) */
+  double gemm_flops;       /* 2 M N K summed over those launches                      */
+  int gemm_count;
+  int num_obs_kept;        /* observations left after fixPts                          */
+  double stage_ms[12];     /* per device stage, see unc_stage_name()                  */
+} unc_stats;
+
+/* Optional host copies of intermediates (any pointer may be NULL).
+ * N = num_cams*cam_params, npc = num_points-3, dense matrices column-major. */
+typedef struct unc_debug {
+  double *U_s;     /* num_cams*cp*cp scaled camera blocks (row-major per block) */
+  double *Vinv_s;  /* npc*9 inverses of the scaled point blocks                  */
+  double *d;       /* N + 3*npc column scales                                     */
+  double *Z;       /* N*N scaled Schur complement, before + lambda I              */
+  double *X0;      /* N*N (Z + lambda I)^-1 as used by the chain                  */
+  double *X;       /* N*N Taylor result                                           */
+} unc_debug;
+
+typedef struct unc_context unc_context;
+
+/* Fill defaults: TAYLOR_EXPANSION, lambda -1, FP64, refine 1, device 0. */
+void unc_default_options(unc_options *opts);
+
+/* Per-device context: stream, reusable workspace, timing events.  Reentrant
+ * across contexts; one context must not be used by two threads at once. */
+int unc_context_create(int device, unc_context **ctx);
+void unc_context_destroy(unc_context *ctx);
+
+/* Host-memory drop-in for computeCovariances (compute.cpp:458).  rows has
+ * num_rows+1 entries, cols/values rows[num_rows] entries.  cam_unc holds
+ * num_cams*cp(cp+1)/2 doubles, pt_unc num_points*6.  ctx may be NULL (a
+ * per-device default context is used, serialised by a lock). */
+int unc_compute(unc_context *ctx, const unc_options *opts, int num_rows,
+                int num_cols, const int *rows, const int *cols,
+                const double *values, double *cam_unc, double *pt_unc,
+                unc_stats *stats);
+
+/* Same with every array already resident in device memory (HBM) on ctx's
+ * device; outputs are device pointers.  stream: hipStream_t or NULL for the
+ * context's stream.  Returns after the results are complete. */
+int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows,
+                       int num_cols, const int *d_rows, const int *d_cols,
+                       const double *d_values, double *d_cam_unc,
+                       double *d_pt_unc, unc_stats *stats, void *stream);
+
+/* unc_compute plus host copies of intermediates (parity tests). */
+int unc_compute_debug(unc_context *ctx, const unc_options *opts, int num_rows,
+                      int num_cols, const int *rows, const int *cols,
+                      const double *values, double *cam_unc, double *pt_unc,
+                      unc_stats *stats, unc_debug *dbg);
+
+/* Device bytes a solve of this size needs (admission control). */
+unsigned long long unc_workspace_bytes(const unc_options *opts, int num_rows);
+
+/* Kernel-level diagnostics on device 0 (host pointers, column-major); they
+ * pin single kernels in the parity tests and have no reference counterpart.
+ *   unc_diag_gemm_f64     C = alpha A B + beta C with the FP64 MFMA GEMM;
+ *                         *absmax_out = max|C| from its fused epilogue
+ *   unc_diag_inverse      X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels,
+ *                         plus the compensated Newton step when refine != 0
+ *   unc_diag_residual_dd  R = I - A X with the compensated (Dot2) kernel */
+int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
+                      const double *B, int ldb, double beta, double *C, int ldc,
+                      double *absmax_out);
+int unc_diag_inverse(int n, const double *A, double *X, int refine);
+int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
+
+const char *unc_error_string(int code);
+const char *unc_stage_name(int stage);
+int unc_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UNCERTAINTY_AMD_H */

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Contract (north star): the plain FP64 path matches the reference restatement
+to <= 1e-9 relative Frobenius error per 9x9 camera block and per 3x3 point
+block.  Stages before the dense inverse (fixPts, scales, blocks, 3x3
+inverses, Schur) are required to be bit-identical to the oracle, which runs
+the same fma sequence.
+"""
+import copy
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import uncertaintyte_amd as u
+from uncertaintyte_amd import jacob, scene
+
+pytestmark = pytest.mark.gpu
+TOL_BLOCK = 1e-9
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def block_rel(a, b):
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    return np.where(nb > 0, d / np.where(nb > 0, nb, 1), d)
+
+
+# ----------------------------------------------------------------- kernels
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (130, 77, 45), (300, 257, 129), (576, 576, 576)])
+def test_gemm_f64(M, N, K):
+    say(f"gemm {M}x{N}x{K}")
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    C, m = u.diag_gemm(A, B, C0, alpha=-0.75, beta=0.5)
+    ref = -0.75 * (A @ B) + 0.5 * C0
+    assert np.abs(C - ref).max() <= 1e-13 * np.abs(A).max() * np.abs(B).max() * K
+    assert m == np.abs(C).max()
+
+
+def test_residual_dd_kernel():
+    say("residual_dd")
+    rng = np.random.default_rng(7)
+    n = 150
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Q * np.logspace(0, -8, n)) @ Q.T
+    A = 0.5 * (A + A.T)
+    X = np.linalg.inv(A)
+    R = u.diag_residual_dd(A, X)
+    Ro = oracle.residual_dd(A, X)
+    assert np.abs(R - Ro).max() <= 1e-13 * np.abs(Ro).max() + 1e-300
+
+
+def test_inverse_kernel():
+    say("gj inverse")
+    rng = np.random.default_rng(3)
+    n = 200
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Q * np.logspace(0, -6, n)) @ Q.T
+    A = 0.5 * (A + A.T)
+    X = u.diag_inverse(A, refine=True)
+    Xl = oracle.inv_lu(A)
+    Xl = Xl + oracle.gemm(Xl, oracle.residual_dd(A, Xl))
+    assert np.linalg.norm(X - Xl) / np.linalg.norm(Xl) < 1e-12
+
+
+# ----------------------------------------------------------------- pipeline
+
+def _cases():
+    return {
+        "cube": lambda: jacob.read_jacob(os.path.join(os.path.dirname(__file__), "golden", "01_Cube.jacob")),
+        "daliborka": lambda: jacob.read_jacob(os.path.join(os.path.dirname(__file__), "golden", "04_Daliborka.jacob")),
+        "toy10x1000": lambda: scene.scene_jacobian(scene.ring_scene(10, 1000, seed=1)),
+        "rings40x4000": lambda: scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11)),
+    }
+
+
+@pytest.fixture(scope="module", params=list(_cases().keys()))
+def case(request):
+    j = _cases()[request.param]()
+    o = oracle.pipeline(j, debug=True)
+    return request.param, j, o
+
+
+def test_pipeline_parity(case):
+    name, j, o = case
+    say(f"pipeline parity {name}: cams={j.num_cams} pts={j.num_points} obs={j.num_obs}")
+    r = u.compute_covariances(j, debug=True)
+    g = r.debug
+    # bit-exact stages
+    assert np.array_equal(g["d"], o["d"]), "column scales"
+    assert np.array_equal(g["U_s"], o["U_s"]), "camera blocks"
+    assert np.array_equal(g["Vinv_s"], o["Vinv_s"]), "3x3 inverses"
+    assert np.array_equal(g["Z"], o["Z"]), "Schur complement"
+    assert r.lambda_ == o["lambda_"]
+    assert r.fixed == o["fixed"]
+    # tolerance stages
+    ex0 = np.linalg.norm(g["X0"] - o["X0"]) / np.linalg.norm(o["X0"])
+    assert ex0 < 1e-11, ex0
+    assert r.n_terms == o["n_terms"]
+    np.testing.assert_allclose(r.cycle_change, o["cycle_change"], rtol=1e-9)
+    rc = block_rel(r.cam, o["cam"])
+    rp = block_rel(r.pt, o["pt"])
+    say(f"  {name}: X0 rel {ex0:.2e}  cam max {rc.max():.2e}  pt max {rp.max():.2e}")
+    assert rc.max() <= TOL_BLOCK, rc.max()
+    assert rp.max() <= TOL_BLOCK, rp.max()
+    assert np.array_equal(r.pt == 0, o["pt"] == 0)
+
+
+def test_config2_parity():
+    """200 cameras on 4 rings, 100k points (the bench workload), vs the oracle."""
+    say("config2 parity: generating")
+    j = scene.make_workload("rings200")
+    o = oracle.pipeline(j)
+    say("config2 parity: gpu")
+    r = u.compute_covariances(j)
+    rc = block_rel(r.cam, o["cam"])
+    rp = block_rel(r.pt, o["pt"])
+    say(f"  config2: cam max {rc.max():.2e} median {np.median(rc):.2e}  pt max {rp.max():.2e}")
+    assert r.n_terms == o["n_terms"]
+    assert rc.max() <= TOL_BLOCK
+    assert rp.max() <= TOL_BLOCK
+
+
+def test_daliborka_golden_on_gpu(daliborka, golden_dir):
+    say("daliborka vs output.cov")
+    hdr, cam, pt = jacob.read_cov(os.path.join(golden_dir, "04_Daliborka_output.cov"))
+    r = u.compute_covariances(daliborka)
+    assert block_rel(r.cam, cam).max() < 1e-5
+    mask = np.linalg.norm(pt, axis=1) > 0
+    assert block_rel(r.pt[mask], pt[mask]).max() < 1e-5
+    assert ["%g" % c for c in r.cycle_change] == hdr["TE cycle change"].split()
+
+
+def test_unrefined_path_within_noise_floor(daliborka):
+    say("unrefined inverse")
+    o = oracle.pipeline(daliborka)
+    r = u.compute_covariances(daliborka, refine=False)
+    # plain FP64 inverse: cond(Z + lambda I) * eps ~ 1e-9 noise, no compensated step
+    assert block_rel(r.cam, o["cam"]).max() < 1e-7
+
+
+# ----------------------------------------------------------------- errors
+
+def test_error_no_fixed_points(golden_dir):
+    say("error: no fixed points")
+    j = jacob.read_jacob(os.path.join(golden_dir, "03_Toy.jacob"))
+    with pytest.raises(u.UncError) as e:
+        u.compute_covariances(j)
+    assert e.value.code == u.UNC_ERR_NO_FIXED_POINTS
+
+
+def test_error_zero_column(cube):
+    say("error: zero column")
+    j = copy.deepcopy(cube)
+    j.vals = j.vals.copy()
+    j.vals[j.cols == 2 * 9 + 6] = 0.0
+    with pytest.raises(u.UncError) as e:
+        u.compute_covariances(j)
+    assert e.value.code == u.UNC_ERR_SINGULAR
+
+
+def test_error_bad_layout(cube):
+    say("error: bad layout")
+    j = copy.deepcopy(cube)
+    j.cols = j.cols.copy()
+    j.cols[1], j.cols[2] = j.cols[2], j.cols[1]
+    with pytest.raises(u.UncError) as e:
+        u.compute_covariances(j)
+    assert e.value.code == u.UNC_ERR_BAD_INPUT
+
+
+# ----------------------------------------------------------------- device API
+
+def test_device_api_matches_host_api(daliborka):
+    say("device API")
+    import torch
+    j = daliborka
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(j.rows).to(dev)
+    cols = torch.from_numpy(j.cols).to(dev)
+    vals = torch.from_numpy(j.vals).to(dev)
+    cam = torch.zeros(j.num_cams * 45, dtype=torch.float64, device=dev)
+    pt = torch.zeros(j.num_points * 6, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    o = u.make_options(j)
+    st = u.compute_device(o, j.num_rows, j.num_cols, rows.data_ptr(), cols.data_ptr(),
+                          vals.data_ptr(), cam.data_ptr(), pt.data_ptr())
+    r = u.compute_covariances(j)
+    assert np.array_equal(cam.cpu().numpy().reshape(r.cam.shape), r.cam)
+    assert np.array_equal(pt.cpu().numpy().reshape(r.pt.shape), r.pt)
+    assert st.num_terms == r.n_terms and st.gemm_count == 2 + r.n_terms

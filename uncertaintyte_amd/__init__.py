@@ -1,0 +1,304 @@
+"""MI355X (gfx950) camera + point covariance pipeline of uncertaintyTE.
+
+Python host mirror of the reference interface.  The work runs in the HIP
+library `_build/libuncertainty_amd.so` (C ABI: include/uncertainty_amd.h); this
+module binds it with ctypes and raises if the library is missing -- there is
+no CPU fallback.
+
+    from uncertaintyte_amd import jacob, compute_covariances
+    J = jacob.read_jacob("scene.jacob")
+    res = compute_covariances(J)          # res.cam (nC, 45), res.pt (nP, 6)
+
+Mirrors:
+  compute_covariances  <- computeCovariances / getCovariances
+                          (compute.cpp:458-540, include/uncertainty.h:48-61)
+  Uncertainty          <- cov::Uncertainty accessors (uncertainty.cpp:36-185)
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import jacob  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libuncertainty_amd.so")
+CLI_PATH = os.path.join(HERE, "_build", "uncertainty")
+
+UNC_OK = 0
+UNC_ERR_BAD_INPUT = 1
+UNC_ERR_NO_FIXED_POINTS = 2
+UNC_ERR_SINGULAR = 3
+UNC_ERR_OOM = 4
+UNC_ERR_DEVICE = 5
+UNC_ERR_UNSUPPORTED = 6
+UNC_ERR_NUMERIC = 7
+PREC_FP64, PREC_MIXED = 0, 1
+TAYLOR_EXPANSION = 2
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_int), ("epsilon", ctypes.c_double),
+                ("lambda_", ctypes.c_double), ("num_cams", ctypes.c_int),
+                ("cam_params", ctypes.c_int), ("num_points", ctypes.c_int),
+                ("num_obs", ctypes.c_int), ("svd_remove_n", ctypes.c_int),
+                ("max_iter_te", ctypes.c_int), ("pts2fix", ctypes.c_int * 3),
+                ("precision", ctypes.c_int), ("refine", ctypes.c_int),
+                ("device", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "time_create_j", "time_fix_j", "time_norm_j", "time_multiply_jj", "time_split_jj",
+        "time_inv_v", "time_compose_z", "time_inv_z", "time_te", "time_pts_unc", "time_all",
+        "lambda_")] + [("fixed_pts", ctypes.c_int * 3), ("num_terms", ctypes.c_int),
+                       ("cycle_change", ctypes.c_double * 20), ("gemm_ms", ctypes.c_double),
+                       ("gemm_flops", ctypes.c_double), ("gemm_count", ctypes.c_int),
+                       ("num_obs_kept", ctypes.c_int), ("stage_ms", ctypes.c_double * 12)]
+
+
+class Debug(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("U_s", "Vinv_s", "d", "Z", "X0", "X")]
+
+
+class UncError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m uncertaintyte_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    L.unc_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.unc_default_options.restype = None
+    L.unc_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.unc_context_destroy.argtypes = [vp]
+    L.unc_context_destroy.restype = None
+    L.unc_compute.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                              vp, vp, ctypes.POINTER(Stats)]
+    L.unc_compute_device.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int, ctypes.c_int, vp,
+                                     vp, vp, vp, vp, ctypes.POINTER(Stats), vp]
+    L.unc_compute_debug.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int, ctypes.c_int, vp, vp,
+                                    vp, vp, vp, ctypes.POINTER(Stats), ctypes.POINTER(Debug)]
+    L.unc_workspace_bytes.argtypes = [ctypes.POINTER(Options), ctypes.c_int]
+    L.unc_workspace_bytes.restype = ctypes.c_ulonglong
+    L.unc_error_string.argtypes = [ctypes.c_int]
+    L.unc_error_string.restype = ctypes.c_char_p
+    L.unc_stage_name.argtypes = [ctypes.c_int]
+    L.unc_stage_name.restype = ctypes.c_char_p
+    L.unc_abi_version.restype = ctypes.c_int
+    L.unc_diag_gemm_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp,
+                                    ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp,
+                                    ctypes.c_int, dp]
+    L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
+    L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def error_string(code: int) -> str:
+    return lib().unc_error_string(code).decode()
+
+
+def _check(code: int, what: str):
+    if code != UNC_OK:
+        raise UncError(code, f"{what}: {error_string(code)}")
+
+
+def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
+                 fix=None, lam: Optional[float] = None) -> Options:
+    o = Options()
+    lib().unc_default_options(ctypes.byref(o))
+    o.algorithm = TAYLOR_EXPANSION
+    o.lambda_ = float(jac.lam if lam is None else lam)
+    o.num_cams, o.cam_params, o.num_points, o.num_obs = (jac.num_cams, jac.cam_params,
+                                                         jac.num_points, jac.num_obs)
+    fx = fix if fix is not None else (jac.fix if jac.fix is not None else (-1, -1, -1))
+    for i in range(3):
+        o.pts2fix[i] = int(fx[i])
+    o.precision = precision
+    o.refine = 1 if refine else 0
+    o.device = device
+    return o
+
+
+@dataclasses.dataclass
+class Result:
+    cam: np.ndarray              # (num_cams, cp(cp+1)/2)
+    pt: np.ndarray               # (num_points, 6)
+    lambda_: float
+    n_terms: int
+    cycle_change: list
+    fixed: tuple
+    stats: Stats
+    debug: Optional[dict] = None
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def compute_covariances(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
+                        fix=None, lam: Optional[float] = None, debug: bool = False,
+                        ctx=None) -> Result:
+    """Run the pipeline on a host-resident Jacobian (jacob.Jacobian)."""
+    L = lib()
+    o = make_options(jac, precision, refine, device, fix, lam)
+    cp = jac.cam_params
+    per = cp * (cp + 1) // 2
+    cam = np.zeros(jac.num_cams * per, np.float64)
+    pt = np.zeros(jac.num_points * 6, np.float64)
+    rows = np.ascontiguousarray(jac.rows, np.int32)
+    cols = np.ascontiguousarray(jac.cols, np.int32)
+    vals = np.ascontiguousarray(jac.vals, np.float64)
+    st = Stats()
+    dbg_arrays = None
+    if debug:
+        N = jac.num_cams * cp
+        npc = jac.num_points - 3
+        dbg_arrays = dict(U_s=np.zeros(jac.num_cams * cp * cp), Vinv_s=np.zeros(npc * 9),
+                          d=np.zeros(N + 3 * npc), Z=np.zeros(N * N), X0=np.zeros(N * N),
+                          X=np.zeros(N * N))
+        dbg = Debug(**{k: v.ctypes.data for k, v in dbg_arrays.items()})
+        rc = L.unc_compute_debug(ctx, ctypes.byref(o), jac.num_rows, jac.num_cols, _ptr(rows),
+                                 _ptr(cols), _ptr(vals), _ptr(cam), _ptr(pt), ctypes.byref(st),
+                                 ctypes.byref(dbg))
+    else:
+        rc = L.unc_compute(ctx, ctypes.byref(o), jac.num_rows, jac.num_cols, _ptr(rows),
+                           _ptr(cols), _ptr(vals), _ptr(cam), _ptr(pt), ctypes.byref(st))
+    _check(rc, "unc_compute")
+    out = None
+    if debug:
+        N = jac.num_cams * cp
+        out = dict(U_s=dbg_arrays["U_s"].reshape(jac.num_cams, cp, cp),
+                   Vinv_s=dbg_arrays["Vinv_s"].reshape(-1, 3, 3), d=dbg_arrays["d"])
+        for k in ("Z", "X0", "X"):
+            out[k] = dbg_arrays[k].reshape(N, N).T.copy()
+    return Result(cam.reshape(jac.num_cams, per), pt.reshape(jac.num_points, 6), st.lambda_,
+                  st.num_terms, list(st.cycle_change[:st.num_terms]), tuple(st.fixed_pts), st, out)
+
+
+class Context:
+    """A device context (stream + reusable workspace), see unc_context_create."""
+
+    def __init__(self, device: int = 0):
+        self.handle = ctypes.c_void_p()
+        _check(lib().unc_context_create(device, ctypes.byref(self.handle)), "unc_context_create")
+
+    def close(self):
+        if self.handle:
+            lib().unc_context_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compute_device(opts: Options, num_rows: int, num_cols: int, d_rows: int, d_cols: int,
+                   d_vals: int, d_cam: int, d_pt: int, ctx=None, stream: int = 0) -> Stats:
+    """Device-resident call (integer device addresses, e.g. torch .data_ptr())."""
+    st = Stats()
+    h = ctx.handle if isinstance(ctx, Context) else ctx
+    rc = lib().unc_compute_device(h, ctypes.byref(opts), num_rows, num_cols, d_rows, d_cols,
+                                  d_vals, d_cam, d_pt, ctypes.byref(st), stream or None)
+    _check(rc, "unc_compute_device")
+    return st
+
+
+def workspace_bytes(opts: Options, num_rows: int) -> int:
+    return int(lib().unc_workspace_bytes(ctypes.byref(opts), num_rows))
+
+
+# --------------------------------------------------------------------------
+# kernel diagnostics (device 0, host arrays; numpy [row, col] convention)
+# --------------------------------------------------------------------------
+
+def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0):
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    Af = np.asfortranarray(A, dtype=np.float64)
+    Bf = np.asfortranarray(B, dtype=np.float64)
+    Cf = np.asfortranarray(np.zeros((M, N)) if C is None else C, dtype=np.float64).copy(order="F")
+    m = ctypes.c_double()
+    rc = lib().unc_diag_gemm_f64(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K, beta,
+                                 Cf.ctypes.data, M, ctypes.byref(m))
+    _check(rc, "unc_diag_gemm_f64")
+    return np.asarray(Cf), m.value
+
+
+def diag_inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:
+    n = A.shape[0]
+    Af = np.asfortranarray(A, dtype=np.float64)
+    X = np.zeros((n, n), order="F")
+    _check(lib().unc_diag_inverse(n, Af.ctypes.data, X.ctypes.data, 1 if refine else 0),
+           "unc_diag_inverse")
+    return np.asarray(X)
+
+
+def diag_residual_dd(A: np.ndarray, X: np.ndarray) -> np.ndarray:
+    n = A.shape[0]
+    Af = np.asfortranarray(A, dtype=np.float64)
+    Xf = np.asfortranarray(X, dtype=np.float64)
+    R = np.zeros((n, n), order="F")
+    _check(lib().unc_diag_residual_dd(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data),
+           "unc_diag_residual_dd")
+    return np.asarray(R)
+
+
+# --------------------------------------------------------------------------
+# cov::Uncertainty accessors (src/uncertaintyTE/uncertainty.cpp:36-185)
+# --------------------------------------------------------------------------
+
+class Uncertainty:
+    """Output container with the reference's unpack and eigen accessors."""
+
+    def __init__(self, cam_params: int, cam_unc: np.ndarray, pt_unc: np.ndarray):
+        self.cam_params = cam_params
+        self.per = cam_params * (cam_params + 1) // 2
+        self.cam = np.asarray(cam_unc, np.float64).reshape(-1, self.per)
+        self.pt = np.asarray(pt_unc, np.float64).reshape(-1, 6)
+
+    def getCameraUncMatrix(self, i: int) -> np.ndarray:
+        cp = self.cam_params
+        M = np.zeros((cp, cp))
+        iu = np.triu_indices(cp)
+        M[iu] = self.cam[i]
+        M.T[iu] = self.cam[i]
+        return M
+
+    def getCamerasUncMatrices(self) -> np.ndarray:
+        return np.stack([self.getCameraUncMatrix(i) for i in range(len(self.cam))])
+
+    def getPointUncMatrix(self, i: int) -> np.ndarray:
+        a = self.pt[i]
+        return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])
+
+    def getPointsUncMatrices(self) -> np.ndarray:
+        return np.stack([self.getPointUncMatrix(i) for i in range(len(self.pt))])
+
+    def getCamerasUncEigenValues(self) -> np.ndarray:
+        # uncertainty.cpp:74-89: real parts of the (general) eigenvalues, descending
+        ev = np.linalg.eigvals(self.getCamerasUncMatrices()).real
+        return -np.sort(-ev, axis=1)
+
+    def getPointsUncEigenValues(self) -> np.ndarray:
+        ev = np.linalg.eigvals(self.getPointsUncMatrices()).real
+        return -np.sort(-ev, axis=1)

@@ -1,0 +1,157 @@
+// cli.cpp -- `uncertainty` command line tool with the reference's flags
+// (src/software/main_uncertainty.cpp:33-93): read a .jacob, compute the
+// covariances on the MI355X, write a .cov.
+//
+//   uncertainty -in=scene.jacob [-in_form=JACOBIAN] -out=scene.cov
+//               [-alg=TAYLOR_EXPANSION] [-cam=RADIAL] [-debug] [-legacy_time_units]
+//
+// As in the reference, a .jacob input always runs TAYLOR_EXPANSION
+// (JacobianIO.cpp:25 ignores -alg).  -legacy_time_units prints stage times the
+// way the reference does (ten times the seconds, compute.cpp:33).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+#include "../../include/uncertainty.h"
+#include "../../include/uncertainty_amd.h"
+
+namespace {
+
+// JacobianIO::loadJacobian (JacobianIO.cpp:32-63), token for token, with the
+// stream state checked so a truncated file is an error instead of garbage.
+bool load_jacobian(const std::string &path, ceres::CRSMatrix &J, cov::Options &opt) {
+  std::ifstream f(path);
+  if (!f.good()) {
+    std::cerr << "\nThe input file doesn't exist.\n";
+    return false;
+  }
+  int numJ;
+  int fx[3];
+  opt._algorithm = TAYLOR_EXPANSION;
+  opt._epsilon = 1e-10;
+  opt._svdRemoveN = 7;
+  opt._maxIterTE = -1;
+  f >> opt._lambda >> opt._numCams >> opt._camParams >> opt._numPoints >> opt._numObs;
+  f >> fx[0] >> fx[1] >> fx[2];
+  if (fx[0] >= 0 && fx[1] >= 0 && fx[2] >= 0) {
+    opt._pts2fix = (int *)malloc(3 * sizeof(int));
+    for (int i = 0; i < 3; ++i) opt._pts2fix[i] = fx[i];
+  }
+  f >> J.num_rows >> J.num_cols >> numJ;
+  if (!f || J.num_rows < 0 || numJ < 0) return false;
+  J.rows.assign(J.num_rows + 1, 0);
+  J.cols.assign(numJ, 0);
+  J.values.assign(numJ, 0.0);
+  for (int i = 0; i <= J.num_rows; ++i) f >> J.rows[i];
+  for (int i = 0; i < numJ; ++i) f >> J.cols[i];
+  for (int i = 0; i < numJ; ++i) f >> J.values[i];
+  if (!f) {
+    std::cerr << "\nTruncated .jacob file.\n";
+    return false;
+  }
+  return true;
+}
+
+// saveResults (IO.cpp:15-60)
+bool save_results(const std::string &path, const cov::Options &o, const cov::Statistic &s,
+                  int per, const double *cam, const double *pt, double tscale) {
+  std::ofstream out(path);
+  if (!out.good()) return false;
+  out << "# ---- Covariance v0.1 ----\n";
+  out << "# Number of cameras: " << o._numCams << "\n";
+  out << "# Number of camera parameters: " << o._camParams << "\n";
+  out << "# Number of points in 3D: " << o._numPoints << "\n";
+  out << "# Number of observations: " << o._numObs << "\n";
+  out << "# Used algorithm: TAYLOR_EXPANSION\n";
+  if (s.fixedPts != NULL)
+    out << "# Fixed points: " << s.fixedPts[0] << ", " << s.fixedPts[1] << ", " << s.fixedPts[2] << "\n";
+  out << "# Used lambda: " << s.lambda << "\n";
+  out << "# Loading jacobian time: " << s.timeCreateJ * tscale << "s\n";
+  out << "# Normalization of jacobian time: " << s.timeNormJ * tscale << "s\n";
+  out << "# Compose information matrix time: " << s.timeMultiplyJJ * tscale << "s\n";
+  out << "# Split infromation matrix time: " << s.timeSplitJJ * tscale << "s\n";
+  out << "# Inverse V time: " << s.timeInvV * tscale << "s\n";
+  out << "# Compose Z time: " << s.timeComposeZ * tscale << "s\n";
+  out << "# Inverse Z time: " << s.timeInvZ * tscale << "s\n";
+  out << "# Taylor expansion time: " << s.timeTE * tscale << "s\n";
+  out << "# Point uncertainty time: " << s.timePtsUnc * tscale << "s\n";
+  out << "# TE number of iterations: " << s.cycle_change.size() << "\n";
+  out << "# TE cycle change: ";
+  for (size_t i = 0; i < s.cycle_change.size(); ++i) out << s.cycle_change[i] << " ";
+  out << "\n";
+  out << "# Time of the algorithm: " << s.timeAll * tscale << "s\n";
+  out << "# -----------------------\n";
+  for (int i = 0; i < o._numCams; ++i) {
+    for (int j = 0; j < per; ++j) out << cam[(size_t)i * per + j] << " ";
+    out << "\n";
+  }
+  for (int i = 0; i < o._numPoints; ++i) {
+    for (int j = 0; j < 6; ++j) out << pt[(size_t)i * 6 + j] << " ";
+    out << "\n";
+  }
+  return true;
+}
+
+bool flag(const char *arg, const char *name, std::string *val) {
+  // gflags accepts -name=value and --name=value
+  const char *a = arg;
+  while (*a == '-') ++a;
+  size_t n = std::strlen(name);
+  if (std::strncmp(a, name, n) != 0) return false;
+  if (a[n] == '=') {
+    *val = a + n + 1;
+    return true;
+  }
+  if (a[n] == 0) {
+    *val = "true";
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string in = ".", in_form = "JACOBIAN", out = ".", alg = "TAYLOR_EXPANSION", cam = "RADIAL";
+  std::string v;
+  bool legacy_units = false;
+  for (int i = 1; i < argc; ++i) {
+    if (flag(argv[i], "in_form", &v)) in_form = v;
+    else if (flag(argv[i], "in", &v)) in = v;
+    else if (flag(argv[i], "out", &v)) out = v;
+    else if (flag(argv[i], "alg", &v)) alg = v;
+    else if (flag(argv[i], "cam", &v)) cam = v;
+    else if (flag(argv[i], "debug", &v)) {
+    } else if (flag(argv[i], "legacy_time_units", &v)) legacy_units = (v != "false");
+    else {
+      std::cerr << "unknown flag " << argv[i] << "\n";
+      return 2;
+    }
+  }
+  const bool is_jacob = in.size() > 6 && in.substr(in.size() - 6) == ".jacob";
+  if (in_form != "JACOBIAN" && !is_jacob) {
+    std::cerr << "only JACOBIAN (.jacob) input is supported by this build\n";
+    return 2;
+  }
+  cov::Options opt;
+  cov::Statistic stat = cov::Statistic();
+  ceres::CRSMatrix J;
+  if (!load_jacobian(in, J, opt)) return 1;
+  const int per = opt._camParams * (opt._camParams + 1) / 2;
+  std::vector<double> camu((size_t)opt._numCams * per, 0.0), ptu((size_t)opt._numPoints * 6, 0.0);
+  getCovariances(opt, stat, J, camu.data(), ptu.data());
+  int rc = getCovariancesLastError();
+  if (rc != UNC_OK) {
+    std::cerr << "uncertainty: " << unc_error_string(rc) << " (code " << rc << ")\n";
+    return 1;
+  }
+  if (!save_results(out, opt, stat, per, camu.data(), ptu.data(), legacy_units ? 10.0 : 1.0)) {
+    std::cerr << "cannot write " << out << "\n";
+    return 1;
+  }
+  std::cout << "Main function... [done]\n";
+  return 0;
+}

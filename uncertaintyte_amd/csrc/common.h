@@ -1,0 +1,98 @@
+// common.h -- shared definitions of the gfx950 covariance pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/uncertainty_amd.h"
+
+namespace unc {
+
+// Device-side error word bits (OR-ed by validation kernels).
+enum : unsigned int {
+  ERRBIT_LAYOUT = 1u,        // row pointers / column pattern not the .jacob layout
+  ERRBIT_RANGE = 2u,         // camera or point id out of range
+  ERRBIT_ZERO_COLUMN = 4u,   // a Jacobian column is all zero (d = 1/0)
+  ERRBIT_SINGULAR_V = 8u,    // a 3x3 point block has no finite inverse
+  ERRBIT_NOT_CAM_MAJOR = 16u,// observations of a camera are not contiguous
+  ERRBIT_GJ_PIVOT = 32u,     // non-positive pivot in the dense inverse
+  ERRBIT_SCHUR_OVERFLOW = 64u,  // a staged Schur batch exceeded its pair buffer
+};
+
+// Control block of the Taylor chain, resident in device memory so the whole
+// chain is enqueued without host round trips (compute.cpp:142-160).
+struct TeCtrl {
+  unsigned long long maxbits[21];  // max|T_i| as IEEE bits (non-negative doubles order as integers)
+  double old_change;
+  double coef;                     // k of the term accepted by the last control step
+  double change[20];
+  int active;                      // 1 while terms are being accepted
+  int nterms;
+  int pad[2];
+};
+
+#define UNC_HIP_CHECK(expr)                                                       \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      std::fprintf(stderr, "uncertainty_amd: %s failed at %s:%d: %s\n", #expr,  \
+                   __FILE__, __LINE__, hipGetErrorString(e_));                   \
+      return UNC_ERR_DEVICE;                                                      \
+    }                                                                             \
+  } while (0)
+
+static inline unsigned int cdiv(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+struct Structure {
+  int cp, nc, np, npc, nobs, N;     // N = nc*cp
+  int fix[3];                       // partially sorted triple
+  const int *rows, *cols;           // input CRS (device)
+  const double *vals;
+  // outputs of decode
+  int *obs_cam;                     // camera per observation, -1 when dropped
+  int *obs_pt;                      // compact point per observation, -1 when dropped
+  int *cam_first, *cam_last;        // observation range of each camera
+  int *pt_cnt, *pt_off, *pt_list;   // point-grouped observations (sorted by index)
+  int *pt_cursor;
+  unsigned int *err;                // device error word
+};
+
+struct Blocks {
+  double *Us;     // nc*cp*cp
+  double *d;      // N + 3*npc
+  double *Vi;     // npc*9
+  double *Ws;     // nobs*cp*3
+  double *Y;      // nobs*cp*3
+  int *pt_has;    // npc
+};
+
+hipError_t launch_decode(const Structure &s, hipStream_t st);
+hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st);
+hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st);
+hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, hipStream_t st,
+                        int max_lds_bytes);
+hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *X,
+                             double *cam_out, hipStream_t st);
+hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *X,
+                               double *pt_out, int npts_out, hipStream_t st);
+
+// dense
+hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                    int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
+                    const int *active, hipStream_t st);
+hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st);
+hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
+                            hipStream_t st);
+hipError_t copy_block(int m, int n, const double *src, int lds, double *dst, int ldd, double scale,
+                      hipStream_t st);
+hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st);
+hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st);
+hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st);
+hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st);
+hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
+
+}  // namespace unc

@@ -1,0 +1,326 @@
+// k_assemble.hip -- (a) Jacobian decode + J^T J block assembly with column
+// scaling, and (b) batched FP64 3x3 point-block inverses, for gfx950.
+//
+// Reference correspondence:
+//   decode / fixPts row dropping      compute.cpp:283-368, :370-380
+//   column scale d = 1/||J_:,j||      compute.cpp:76-83, ScaledSparseMatrix.cpp:334-358
+//   U, W, V blocks of J_s^T J_s       ScaledSparseMatrix.cpp:384-388, :459-465, :238-299
+//   3x3 cofactor inverse              ScaledSparseMatrix.cpp:210-230
+//
+// Arithmetic contract (built with -ffp-contract=off): each block entry is a
+// sequential chain of explicit fma() over the group's observations in row
+// order, exactly the order of the CPU oracle, so U_s, d, V_s^-1, W_s and Y are
+// bit-identical to it.  Scaling is applied to the assembled blocks
+// (M_s = D M D) rather than to J, so J is read once.
+#include "common.h"
+
+namespace unc {
+
+// ---------------------------------------------------------------------------
+// decode: one thread per observation (two CRS rows)
+// ---------------------------------------------------------------------------
+__global__ void k_decode(Structure s) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= s.nobs) return;
+  const int cp = s.cp, rl = cp + 3, N = s.N;
+  const int r0 = 2 * o;
+  unsigned int e = 0;
+  if (s.rows[r0] != r0 * rl || s.rows[r0 + 1] != (r0 + 1) * rl || s.rows[r0 + 2] != (r0 + 2) * rl)
+    e |= ERRBIT_LAYOUT;
+  int a = -1, p = -1;
+  if (!e) {
+    const int *c0 = s.cols + (size_t)r0 * rl;
+    const int *c1 = c0 + rl;
+    int cam0 = c0[0];
+    a = cam0 / cp;
+    if (cam0 < 0 || cam0 % cp != 0 || a >= s.nc) e |= ERRBIT_RANGE;
+    int pc = c0[cp] - N;
+    p = pc / 3;
+    if (pc < 0 || pc % 3 != 0 || p >= s.np) e |= ERRBIT_RANGE;
+    if (!e) {
+      for (int j = 0; j < cp; ++j)
+        if (c0[j] != cam0 + j || c1[j] != cam0 + j) e |= ERRBIT_LAYOUT;
+      for (int j = 0; j < 3; ++j)
+        if (c0[cp + j] != N + pc + j || c1[cp + j] != N + pc + j) e |= ERRBIT_LAYOUT;
+      if (o > 0) {
+        int prev = s.cols[(size_t)(r0 - 2) * rl];
+        if (prev >= 0 && prev / cp > a) e |= ERRBIT_NOT_CAM_MAJOR;
+      }
+    }
+  }
+  if (e) {
+    atomicOr(s.err, e);
+    s.obs_cam[o] = -1;
+    s.obs_pt[o] = -1;
+    return;
+  }
+  const bool dropped = (p == s.fix[0] || p == s.fix[1] || p == s.fix[2]);
+  if (dropped) {
+    s.obs_cam[o] = -1;
+    s.obs_pt[o] = -1;
+    return;
+  }
+  // compact id after removing the three fixed points' columns (newIds, compute.cpp:301-311)
+  int pcmp = p - (s.fix[0] < p) - (s.fix[1] < p) - (s.fix[2] < p);
+  s.obs_cam[o] = a;
+  s.obs_pt[o] = pcmp;
+  atomicAdd(&s.pt_cnt[pcmp], 1);
+  atomicMin(&s.cam_first[a], o);
+  atomicMax(&s.cam_last[a], o);
+}
+
+__global__ void k_init_lists(Structure s) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < s.npc) { s.pt_cnt[i] = 0; s.pt_cursor[i] = 0; }
+  if (i < s.nc) { s.cam_first[i] = 0x7fffffff; s.cam_last[i] = -1; }
+  if (i == 0) *s.err = 0u;
+}
+
+hipError_t launch_decode(const Structure &s, hipStream_t st) {
+  int n = s.npc > s.nc ? s.npc : s.nc;
+  hipLaunchKernelGGL(k_init_lists, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, s);
+  if (s.nobs > 0) hipLaunchKernelGGL(k_decode, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of pt_cnt -> pt_off (npc + 1 entries), three-phase
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 1024;  // elements per block (256 threads x 4)
+
+__device__ inline int block_exclusive_scan_256(int v, int *sh, int *total) {
+  // Hillis-Steele inclusive scan in LDS over 256 values.
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int x = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  int incl = sh[t];
+  if (total) *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_blocks(const int *in, int n, int *out, int *block_sums) {
+  __shared__ int sh[256];
+  int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
+  int v[4];
+  int sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    sum += v[k];
+  }
+  int tot;
+  int ex = block_exclusive_scan_256(sum, sh, &tot);
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_sums(int *block_sums, int nb, int *total_out) {
+  __shared__ int sh[256];
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    int i = b0 + threadIdx.x;
+    int v = (i < nb) ? block_sums[i] : 0;
+    int tot;
+    int ex = block_exclusive_scan_256(v, sh, &tot);
+    if (i < nb) block_sums[i] = ex + carry;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(int *out, int n, const int *block_sums) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += block_sums[i / SCAN_BLOCK];
+}
+
+__global__ void k_fill_points(Structure s) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= s.nobs) return;
+  int p = s.obs_pt[o];
+  if (p < 0) return;
+  int pos = s.pt_off[p] + atomicAdd(&s.pt_cursor[p], 1);
+  s.pt_list[pos] = o;
+}
+
+// atomics place observations in arbitrary order inside a point's segment;
+// sorting each (short) segment by observation index restores row order
+__global__ void k_sort_segments(Structure s) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= s.npc) return;
+  int t0 = s.pt_off[p], t1 = s.pt_off[p + 1];
+  for (int i = t0 + 1; i < t1; ++i) {
+    int v = s.pt_list[i];
+    int j = i - 1;
+    while (j >= t0 && s.pt_list[j] > v) {
+      s.pt_list[j + 1] = s.pt_list[j];
+      --j;
+    }
+    s.pt_list[j + 1] = v;
+  }
+}
+
+hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st) {
+  int n = s.npc;
+  int nb = (int)cdiv(n, SCAN_BLOCK);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, s.pt_cnt, n, s.pt_off, scan_tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, scan_tmp, nb, s.pt_off + n);
+  hipLaunchKernelGGL(k_scan_add, dim3(cdiv(n, 256)), dim3(256), 0, st, s.pt_off, n, scan_tmp);
+  if (s.nobs > 0) hipLaunchKernelGGL(k_fill_points, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
+  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(n, 256)), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// camera blocks: one wave per camera, lane = upper-triangle entry (i, j)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_cam_assemble(Structure s, Blocks b) {
+  __shared__ double U[81];
+  __shared__ double dsh[9];
+  const int a = blockIdx.x, l = threadIdx.x;
+  const int cp = s.cp, rl = cp + 3;
+  const int npair = cp * (cp + 1) / 2;
+  int i = 0, j = 0;
+  {
+    int r = l;
+    for (int ii = 0; ii < cp; ++ii) {
+      int len = cp - ii;
+      if (r < len) { i = ii; j = ii + r; break; }
+      r -= len;
+    }
+  }
+  const int first = s.cam_first[a], last = s.cam_last[a];
+  double acc = 0.0;
+  if (l < npair) {
+    for (int o = first; o <= last; ++o) {
+      if (s.obs_cam[o] < 0) continue;  // observation of a fixed point
+      const double *r0 = s.vals + (size_t)o * 2 * rl;
+      const double *r1 = r0 + rl;
+      acc = fma(r0[i], r0[j], acc);
+      acc = fma(r1[i], r1[j], acc);
+    }
+    U[i * cp + j] = acc;
+    U[j * cp + i] = acc;
+  }
+  __syncthreads();
+  if (l < cp) {
+    double u = U[l * cp + l];
+    if (!(u > 0.0)) atomicOr(s.err, ERRBIT_ZERO_COLUMN);
+    double dv = 1.0 / sqrt(u);
+    dsh[l] = dv;
+    b.d[a * cp + l] = dv;
+  }
+  __syncthreads();
+  for (int e = l; e < cp * cp; e += 64) {
+    int ii = e / cp, jj = e % cp;
+    b.Us[(size_t)a * cp * cp + e] = (U[e] * dsh[ii]) * dsh[jj];
+  }
+}
+
+// inv3x3blockSymmDiag (ScaledSparseMatrix.cpp:210-230) with the reference's
+// expression order; reads the upper entries only.
+__device__ inline void inv3x3_ref(const double v[9], double o[9]) {
+  double div = v[0] * v[4] * v[8] - v[0] * v[5] * v[5] - v[1] * v[1] * v[8] +
+               2 * v[1] * v[2] * v[5] - v[2] * v[2] * v[4];
+  o[0] = (v[4] * v[8] - v[5] * v[5]) / div;
+  o[1] = (-v[1] * v[8] + v[2] * v[5]) / div;
+  o[2] = (v[1] * v[5] - v[2] * v[4]) / div;
+  o[3] = o[1];
+  o[4] = (v[0] * v[8] - v[2] * v[2]) / div;
+  o[5] = (-v[0] * v[5] + v[1] * v[2]) / div;
+  o[6] = o[2];
+  o[7] = o[5];
+  o[8] = (v[0] * v[4] - v[1] * v[1]) / div;
+}
+
+// ---------------------------------------------------------------------------
+// point blocks: one thread per compact point; V_p, d_p, V_s^-1, then the
+// per-observation W_s = D_c (J_c^T J_p) D_p and Y = W_s V_s^-1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_pt_assemble(Structure s, Blocks b) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= s.npc) return;
+  const int cp = s.cp, rl = cp + 3, N = s.N;
+  const int t0 = s.pt_off[p], t1 = s.pt_off[p + 1];
+  double *dp = b.d + N + 3 * p;
+  double *Vi = b.Vi + (size_t)p * 9;
+  if (t0 == t1) {
+    b.pt_has[p] = 0;
+    dp[0] = dp[1] = dp[2] = 0.0;
+    for (int k = 0; k < 9; ++k) Vi[k] = 0.0;
+    return;
+  }
+  double v00 = 0, v01 = 0, v02 = 0, v11 = 0, v12 = 0, v22 = 0;
+  for (int t = t0; t < t1; ++t) {
+    int o = s.pt_list[t];
+    const double *q0 = s.vals + (size_t)o * 2 * rl + cp;
+    const double *q1 = q0 + rl;
+    double a0 = q0[0], a1 = q0[1], a2 = q0[2], c0 = q1[0], c1 = q1[1], c2 = q1[2];
+    v00 = fma(a0, a0, v00); v00 = fma(c0, c0, v00);
+    v01 = fma(a0, a1, v01); v01 = fma(c0, c1, v01);
+    v02 = fma(a0, a2, v02); v02 = fma(c0, c2, v02);
+    v11 = fma(a1, a1, v11); v11 = fma(c1, c1, v11);
+    v12 = fma(a1, a2, v12); v12 = fma(c1, c2, v12);
+    v22 = fma(a2, a2, v22); v22 = fma(c2, c2, v22);
+  }
+  double V[9] = {v00, v01, v02, v01, v11, v12, v02, v12, v22};
+  double dd[3];
+  bool bad = false;
+  for (int k = 0; k < 3; ++k) {
+    if (!(V[k * 4] > 0.0)) bad = true;
+    dd[k] = 1.0 / sqrt(V[k * 4]);
+    dp[k] = dd[k];
+  }
+  double Vs[9];
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) Vs[k * 3 + l] = (V[k * 3 + l] * dd[k]) * dd[l];
+  double iv[9];
+  inv3x3_ref(Vs, iv);
+  for (int k = 0; k < 9; ++k) {
+    if (!isfinite(iv[k])) bad = true;
+    Vi[k] = iv[k];
+  }
+  if (bad) atomicOr(s.err, ERRBIT_SINGULAR_V);
+  b.pt_has[p] = 1;
+  for (int t = t0; t < t1; ++t) {
+    int o = s.pt_list[t];
+    int a = s.obs_cam[o];
+    const double *r0 = s.vals + (size_t)o * 2 * rl;
+    const double *r1 = r0 + rl;
+    const double *dc = b.d + a * cp;
+    double *Ws = b.Ws + (size_t)o * cp * 3;
+    double *Y = b.Y + (size_t)o * cp * 3;
+    for (int i = 0; i < cp; ++i) {
+      double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
+      double w[3];
+      for (int k = 0; k < 3; ++k) {
+        double wv = fma(ci1, r1[cp + k], ci0 * r0[cp + k]);
+        w[k] = (wv * dci) * dd[k];
+        Ws[i * 3 + k] = w[k];
+      }
+      for (int m = 0; m < 3; ++m) {
+        double y = w[0] * iv[0 * 3 + m];
+        y = fma(w[1], iv[1 * 3 + m], y);
+        y = fma(w[2], iv[2 * 3 + m], y);
+        Y[i * 3 + m] = y;
+      }
+    }
+  }
+}
+
+hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st) {
+  hipLaunchKernelGGL(k_cam_assemble, dim3(s.nc), dim3(64), 0, st, s, b);
+  if (s.npc > 0) hipLaunchKernelGGL(k_pt_assemble, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b);
+  return hipGetLastError();
+}
+
+}  // namespace unc

@@ -1,0 +1,348 @@
+// k_dense.hip -- (d) dense FP64 kernels of the Z inverse + Taylor chain, gfx950.
+//
+// Reference: teInverse (compute.cpp:121-166) with SDM::inv (LAPACK dgetrf +
+// dgetri, ScaledDenseMatrix.cpp:232-279), SDM::operator*= (dgemm,
+// ScaledDenseMatrix.cpp:314-353), absMax (:207-216) and iZupdate
+// (compute.cpp:64-71).
+//
+// gemm_f64: C = alpha A B + beta C (column-major, no transposes) on the FP64
+// matrix cores (v_mfma_f64_16x16x4_f64).  128x128 tiles, 4 waves of 64x64,
+// K staged through LDS 16 at a time with the next slab prefetched into
+// registers.  Optional fused epilogue: max|C| folded into one 64-bit atomicMax
+// per workgroup (non-negative doubles order like their bit patterns, so the
+// result is order independent), and an early exit when the Taylor control
+// block says the series has stopped (so the whole chain is enqueued at once).
+#include "common.h"
+
+namespace unc {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+constexpr int GB_M = 128, GB_N = 128, GB_K = 16, G_THREADS = 256;
+constexpr int G_PAD = 4;
+
+__global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, double alpha,
+                                                       const double *__restrict__ A, int lda,
+                                                       const double *__restrict__ B, int ldb,
+                                                       double beta, double *__restrict__ C,
+                                                       int ldc, unsigned long long *absmax,
+                                                       const int *active) {
+  if (active && *active == 0) return;
+  __shared__ double As[GB_K][GB_M + G_PAD];
+  __shared__ double Bs[GB_K][GB_N + G_PAD];
+  __shared__ double wmax[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  // tiles: 8-wide groups of tile-rows walk the columns so neighbouring
+  // workgroups share A panels in L2
+  const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
+  const int bid = blockIdx.x;
+  const int group = 8;
+  const int per_group = group * tiles_n;
+  const int gidx = bid / per_group;
+  const int first_m = gidx * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int tm = first_m + (bid % per_group) % gsize;
+  const int tn = (bid % per_group) / gsize;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+
+  double4_t acc[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  double ra[8], rb[8];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int idx = tid + G_THREADS * r;
+      int k = idx >> 7, m = idx & 127;  // A: 16 columns of 128 rows
+      int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      int n = idx >> 4, kk = idx & 15;  // B: 128 columns of 16 rows
+      int gn = n0 + n, gk2 = k0 + kk;
+      rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int idx = tid + G_THREADS * r;
+      As[idx >> 7][idx & 127] = ra[r];
+      Bs[idx & 15][idx >> 4] = rb[r];
+    }
+  };
+
+  load_tiles(0);
+  for (int k0 = 0; k0 < K; k0 += GB_K) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (k0 + GB_K < K) load_tiles(k0 + GB_K);
+#pragma unroll
+    for (int ks = 0; ks < GB_K; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  double lmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+        if (row < M && col < N) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+          lmax = fmax(lmax, fabs(v));
+        }
+      }
+    }
+  if (absmax) {
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if (lane == 0) wmax[wave] = lmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+      atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                    int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
+                    const int *active, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
+  hipLaunchKernelGGL(k_gemm_f64, dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha, A, lda, B,
+                     ldb, beta, C, ldc, absmax, active);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// R = I - A X with compensated dot products (Ogita-Rump-Oishi Dot2): the
+// product error comes from fma, the running sum uses TwoSum, the errors are
+// summed separately and added once at the end.  Needs -ffp-contract=off.
+// ---------------------------------------------------------------------------
+constexpr int R_T = 64, R_K = 16;
+
+__global__ __launch_bounds__(256) void k_residual_dd(int n, const double *__restrict__ A,
+                                                     const double *__restrict__ X,
+                                                     double *__restrict__ R) {
+  __shared__ double As[R_K][R_T + 1];
+  __shared__ double Xs[R_K][R_T + 1];
+  const int tid = threadIdx.x;
+  const int tr = tid & 15, tc = tid >> 4;
+  const int i0 = blockIdx.x * R_T, j0 = blockIdx.y * R_T;
+  double s[4][4], c[4][4];
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
+      s[p][q] = (gi == gj) ? 1.0 : 0.0;
+      c[p][q] = 0.0;
+    }
+  for (int k0 = 0; k0 < n; k0 += R_K) {
+    __syncthreads();
+    for (int x = tid; x < R_K * R_T; x += 256) {
+      int m = x & 63, k = x >> 6;
+      int gi = i0 + m, gk = k0 + k;
+      As[k][m] = (gi < n && gk < n) ? A[(size_t)gi + (size_t)gk * n] : 0.0;
+      int kk = x & 15, jj = x >> 4;
+      int gj = j0 + jj, gk2 = k0 + kk;
+      Xs[kk][jj] = (gj < n && gk2 < n) ? X[(size_t)gk2 + (size_t)gj * n] : 0.0;
+    }
+    __syncthreads();
+    const int kn = min(R_K, n - k0);
+    for (int k = 0; k < kn; ++k) {
+      double av[4], xv[4];
+      for (int p = 0; p < 4; ++p) av[p] = As[k][tr + 16 * p];
+      for (int q = 0; q < 4; ++q) xv[q] = Xs[k][tc + 16 * q];
+      for (int p = 0; p < 4; ++p)
+        for (int q = 0; q < 4; ++q) {
+          double P = av[p] * xv[q];
+          double E = fma(av[p], xv[q], -P);
+          double pm = -P;
+          double t = s[p][q] + pm;
+          double bb = t - s[p][q];
+          double err = (s[p][q] - (t - bb)) + (pm - bb);
+          s[p][q] = t;
+          c[p][q] = c[p][q] + (err - E);
+        }
+    }
+  }
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
+      if (gi < n && gj < n) R[(size_t)gi + (size_t)gj * n] = s[p][q] + c[p][q];
+    }
+}
+
+hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st) {
+  dim3 grid(cdiv(n, R_T), cdiv(n, R_T));
+  hipLaunchKernelGGL(k_residual_dd, grid, dim3(256), 0, st, n, A, X, R);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// in-LDS Gauss-Jordan inverse of one b x b diagonal block (b <= 64), no
+// pivoting (the matrix is symmetric positive definite)
+// ---------------------------------------------------------------------------
+constexpr int GJ_B = 64;
+
+__global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ldw, double *P,
+                                                  unsigned int *err) {
+  __shared__ double Mx[GJ_B * GJ_B];
+  __shared__ double colp[GJ_B], rowp[GJ_B];
+  const int tid = threadIdx.x;
+  for (int x = tid; x < b * b; x += 256) {
+    int i = x % b, j = x / b;
+    Mx[x] = W[(size_t)i + (size_t)j * ldw];
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int p = 0; p < b; ++p) {
+    if (tid < b) {
+      colp[tid] = Mx[tid + p * b];
+      rowp[tid] = Mx[p + tid * b];
+    }
+    __syncthreads();
+    const double piv = colp[p];
+    if (!(piv > 0.0)) bad = true;
+    const double ip = 1.0 / piv;
+    for (int x = tid; x < b * b; x += 256) {
+      int i = x % b, j = x / b;
+      double v;
+      if (i == p && j == p) v = ip;
+      else if (i == p) v = rowp[j] * ip;
+      else if (j == p) v = -colp[i] * ip;
+      else v = Mx[x] - colp[i] * (rowp[j] * ip);
+      Mx[x] = v;
+    }
+    __syncthreads();
+  }
+  if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+  for (int x = tid; x < b * b; x += 256) P[x] = Mx[x];
+}
+
+hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
+                            hipStream_t st) {
+  if (b > GJ_B) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(256), 0, st, b, W, ldw, P, err);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// small element-wise helpers
+// ---------------------------------------------------------------------------
+__global__ void k_copy_block(int m, int n, const double *src, int lds, double *dst, int ldd,
+                             double scale) {
+  size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (size_t)m * n) return;
+  int i = (int)(x % m), j = (int)(x / m);
+  double v = src[(size_t)i + (size_t)j * lds];
+  dst[(size_t)i + (size_t)j * ldd] = (scale == 1.0) ? v : scale * v;
+}
+
+hipError_t copy_block(int m, int n, const double *src, int lds, double *dst, int ldd, double scale,
+                      hipStream_t st) {
+  size_t tot = (size_t)m * n;
+  if (!tot) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_block, dim3(cdiv(tot, 256)), dim3(256), 0, st, m, n, src, lds, dst, ldd,
+                     scale);
+  return hipGetLastError();
+}
+
+__global__ void k_add_diag(int n, double *A, int lda, double v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(size_t)i + (size_t)i * lda] = A[(size_t)i + (size_t)i * lda] + v;
+}
+
+hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_diag, dim3(cdiv(n, 256)), dim3(256), 0, st, n, A, lda, v);
+  return hipGetLastError();
+}
+
+// out = sym(X0 + D): out[r,c] = 0.5 ((X0[r,c] + D[r,c]) + (X0[c,r] + D[c,r]))
+__global__ void k_sym_update(int n, const double *X0, const double *D, double *out) {
+  size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (size_t)n * n) return;
+  int r = (int)(x % n), c = (int)(x / n);
+  size_t t = (size_t)c + (size_t)r * n;
+  out[x] = 0.5 * ((X0[x] + D[x]) + (X0[t] + D[t]));
+}
+
+hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st) {
+  size_t tot = (size_t)n * n;
+  hipLaunchKernelGGL(k_sym_update, dim3(cdiv(tot, 256)), dim3(256), 0, st, n, X0, D, out);
+  return hipGetLastError();
+}
+
+// Taylor stopping rule (compute.cpp:146-156) evaluated on the device
+__global__ void k_te_control(TeCtrl *c, int i, double k) {
+  if (threadIdx.x != 0 || !c->active) return;
+  double m = __longlong_as_double((long long)c->maxbits[i]);
+  double change = fabs(k * m);
+  if (change < 1e-5 || change > c->old_change) {
+    c->active = 0;
+    return;
+  }
+  c->old_change = change;
+  c->change[c->nterms] = change;
+  c->nterms = c->nterms + 1;
+  c->coef = k;
+}
+
+hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st) {
+  hipLaunchKernelGGL(k_te_control, dim3(1), dim3(64), 0, st, ctrl, i, k);
+  return hipGetLastError();
+}
+
+// iZupdate (compute.cpp:64-71): X = X + k T while the series is active
+__global__ void k_te_axpy(size_t nn, double *X, const double *T, const TeCtrl *c) {
+  if (!c->active) return;
+  const double k = c->coef;
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < nn;
+       x += (size_t)gridDim.x * blockDim.x)
+    X[x] = X[x] + k * T[x];
+}
+
+hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st) {
+  size_t nn = (size_t)n * n;
+  unsigned int grid = cdiv(nn, 256);
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(k_te_axpy, dim3(grid), dim3(256), 0, st, nn, X, T, ctrl);
+  return hipGetLastError();
+}
+
+__global__ void k_absmax(size_t n, const double *A, unsigned long long *out) {
+  double m = 0.0;
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n;
+       x += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(A[x]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st) {
+  unsigned int grid = cdiv(n, 256);
+  if (grid > 4096) grid = 4096;
+  hipLaunchKernelGGL(k_absmax, dim3(grid), dim3(256), 0, st, n, A, out);
+  return hipGetLastError();
+}
+
+}  // namespace unc

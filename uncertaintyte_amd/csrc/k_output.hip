@@ -1,0 +1,118 @@
+// k_output.hip -- camera output packing and (e) point-covariance
+// back-propagation, gfx950.
+//
+// Camera output: removeScaleJ4Z (compute.cpp:168-183), upper triangle row-major
+// per camera of (X[j,k] + X[k,j]/2) d_j d_k -- the reference's operator
+// precedence, kept for drop-in output parity.
+//
+// Points: the reference builds, per point, A = J_p, B = [J_c, -I2] per
+// observation and Sigma = blockdiag(camera's own 9x9 block, I2), then
+// pinv(A) B Sigma B^T pinv(A)^T with an Eigen COD (compute.cpp:382-441, 516-527).
+// For full-column-rank A this equals
+//     Sigma_p = D_p [Vi + Vi (sum_o W_s,o^T X_cc W_s,o) Vi] D_p
+// with the scaled blocks already on the device, so one thread per point does
+// ~300 FMA per observation and no pseudo-inverse.  Same fma order as the
+// oracle.  Output slot: compact id mapped back past the partially sorted
+// fixed triple (compute.cpp:443-447); fixed points keep the zero written first.
+#include "common.h"
+
+namespace unc {
+
+__global__ void k_cam_output(int nc, int cp, int N, const double *__restrict__ X,
+                             const double *__restrict__ d, double *__restrict__ out) {
+  const int a = blockIdx.x * blockDim.y + threadIdx.y;
+  if (a >= nc) return;
+  const int per = cp * (cp + 1) / 2;
+  for (int l = threadIdx.x; l < per; l += blockDim.x) {
+    int r = l, j = 0;
+    while (r >= cp - j) { r -= cp - j; ++j; }
+    const int k = j + r;
+    const int J = a * cp + j, Kc = a * cp + k;
+    out[(size_t)a * per + l] =
+        ((X[(size_t)J + (size_t)Kc * N] + X[(size_t)Kc + (size_t)J * N] / 2) * d[J]) * d[Kc];
+  }
+}
+
+hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *X,
+                             double *cam_out, hipStream_t st) {
+  dim3 blk(64, 4);
+  hipLaunchKernelGGL(k_cam_output, dim3(cdiv(s.nc, 4)), blk, 0, st, s.nc, s.cp, s.N, X, b.d,
+                     cam_out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
+                                                      const double *__restrict__ X,
+                                                      double *__restrict__ out, int npts_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= s.npc || !b.pt_has[p]) return;
+  const int cp = s.cp, N = s.N;
+  double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = s.pt_off[p]; t < s.pt_off[p + 1]; ++t) {
+    const int o = s.pt_list[t];
+    const int a = s.obs_cam[o];
+    const double *Ws = b.Ws + (size_t)o * cp * 3;
+    const double *Xa = X + (size_t)a * cp + (size_t)a * cp * N;
+    double w[27];
+    for (int q = 0; q < cp * 3; ++q) w[q] = Ws[q];
+    double H[27];
+    for (int i = 0; i < cp; ++i) {
+      double h0 = Xa[i] * w[0], h1 = Xa[i] * w[1], h2 = Xa[i] * w[2];
+      for (int j = 1; j < cp; ++j) {
+        const double x = Xa[(size_t)i + (size_t)j * N];
+        h0 = fma(x, w[j * 3 + 0], h0);
+        h1 = fma(x, w[j * 3 + 1], h1);
+        h2 = fma(x, w[j * 3 + 2], h2);
+      }
+      H[i * 3 + 0] = h0;
+      H[i * 3 + 1] = h1;
+      H[i * 3 + 2] = h2;
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 3; ++l) {
+        double m = w[k] * H[l];
+        for (int i = 1; i < cp; ++i) m = fma(w[i * 3 + k], H[i * 3 + l], m);
+        G[k * 3 + l] = G[k * 3 + l] + m;
+      }
+  }
+  const double *Vi = b.Vi + (size_t)p * 9;
+  double v[9];
+  for (int k = 0; k < 9; ++k) v[k] = Vi[k];
+  double R[9], Q[9];
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      double r = v[k * 3 + 0] * G[0 * 3 + l];
+      r = fma(v[k * 3 + 1], G[1 * 3 + l], r);
+      r = fma(v[k * 3 + 2], G[2 * 3 + l], r);
+      R[k * 3 + l] = r;
+    }
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      double q = R[k * 3 + 0] * v[0 * 3 + l];
+      q = fma(R[k * 3 + 1], v[1 * 3 + l], q);
+      q = fma(R[k * 3 + 2], v[2 * 3 + l], q);
+      Q[k * 3 + l] = v[k * 3 + l] + q;
+    }
+  const double *dp = b.d + N + 3 * p;
+  int id = p;
+  for (int i = 0; i < 3; ++i)
+    if (id >= s.fix[i]) ++id;
+  if (id < 0 || id >= npts_out) return;
+  double *o6 = out + (size_t)id * 6;
+  o6[0] = (Q[0] * dp[0]) * dp[0];
+  o6[1] = (Q[1] * dp[0]) * dp[1];
+  o6[2] = (Q[2] * dp[0]) * dp[2];
+  o6[3] = (Q[4] * dp[1]) * dp[1];
+  o6[4] = (Q[5] * dp[1]) * dp[2];
+  o6[5] = (Q[8] * dp[2]) * dp[2];
+}
+
+hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *X,
+                               double *pt_out, int npts_out, hipStream_t st) {
+  if (s.npc <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_output, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b, X, pt_out,
+                     npts_out);
+  return hipGetLastError();
+}
+
+}  // namespace unc

@@ -1,0 +1,608 @@
+// pipeline.hip -- orchestration of the covariance pipeline on one MI355X and
+// the C ABI of include/uncertainty_amd.h.
+//
+// Stage order mirrors computeCovariances (compute.cpp:458-540, TAYLOR branch):
+//   decode + fixPts -> point lists -> J^T J blocks, scales, 3x3 inverses ->
+//   Schur complement -> (Z + lambda I)^-1 (+ compensated Newton step) ->
+//   Taylor chain -> camera output -> point back-propagation.
+// Everything after the input upload runs on one stream; the only host round
+// trips are the error-word reads after decode and assembly and the final
+// read of the Taylor control block.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+using namespace unc;
+
+struct unc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char *ws = nullptr;
+  size_t ws_bytes = 0;
+  char *io = nullptr;
+  size_t io_bytes = 0;
+  int max_lds = 65536;
+  hipEvent_t stage_ev[16];
+  hipEvent_t gemm_ev[2 * 24];
+  std::mutex mu;
+};
+
+namespace {
+
+const char *kStageNames[12] = {"upload",   "decode",   "point_lists", "assemble",
+                               "reserved", "schur",    "inverse",     "taylor",
+                               "cam_out",  "point_out", "download",   "total"};
+
+struct Sizes {
+  int nc, cp, np, npc, nobs, N;
+  size_t nn;
+};
+
+struct Layout {
+  size_t off[32];
+  size_t total;
+};
+
+enum {
+  B_OBS_CAM, B_OBS_PT, B_CAM_FIRST, B_CAM_LAST, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
+  B_ERR, B_SCAN, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
+  B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
+};
+
+constexpr int GJ_BLOCK = 64;
+
+Layout make_layout(const Sizes &z, bool refine) {
+  size_t sz[B_COUNT];
+  sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
+  sz[B_CAM_FIRST] = sz[B_CAM_LAST] = sizeof(int) * (size_t)z.nc;
+  sz[B_PT_CNT] = sz[B_PT_CURSOR] = sz[B_PT_HAS] = sizeof(int) * (size_t)z.npc;
+  sz[B_PT_OFF] = sizeof(int) * ((size_t)z.npc + 1);
+  sz[B_ERR] = 64;
+  sz[B_SCAN] = sizeof(int) * ((size_t)z.npc / 1024 + 2);
+  sz[B_US] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
+  sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
+  sz[B_VI] = sizeof(double) * 9 * (size_t)z.npc;
+  sz[B_WS] = sz[B_Y] = sizeof(double) * 3 * (size_t)z.cp * z.nobs;
+  sz[B_CTRL] = sizeof(TeCtrl);
+  sz[B_P] = sizeof(double) * GJ_BLOCK * GJ_BLOCK;
+  sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
+  sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
+  sz[B_DE] = refine ? sizeof(double) * z.nn : 0;
+  Layout L;
+  size_t o = 0;
+  for (int i = 0; i < B_COUNT; ++i) {
+    L.off[i] = o;
+    o += (sz[i] + 255) & ~(size_t)255;
+  }
+  L.total = o;
+  return L;
+}
+
+int map_err(unsigned int e) {
+  if (e & (ERRBIT_LAYOUT | ERRBIT_RANGE)) return UNC_ERR_BAD_INPUT;
+  if (e & ERRBIT_NOT_CAM_MAJOR) return UNC_ERR_UNSUPPORTED;
+  if (e & (ERRBIT_ZERO_COLUMN | ERRBIT_SINGULAR_V | ERRBIT_GJ_PIVOT)) return UNC_ERR_SINGULAR;
+  if (e & ERRBIT_SCHUR_OVERFLOW) return UNC_ERR_UNSUPPORTED;
+  return UNC_ERR_BAD_INPUT;
+}
+
+int fact32(int n) {  // compute.cpp:57-62, with the int32 wrap made explicit
+  uint32_t f = 1u;
+  for (int i = 2; i <= n; ++i) f *= (uint32_t)i;
+  return (int)(int32_t)f;
+}
+
+int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[3]) {
+  if (!o) return UNC_ERR_BAD_INPUT;
+  if (o->algorithm != UNC_ALG_TAYLOR_EXPANSION) return UNC_ERR_UNSUPPORTED;
+  if (o->precision != UNC_PREC_FP64 && o->precision != UNC_PREC_MIXED) return UNC_ERR_BAD_INPUT;
+  if (o->precision == UNC_PREC_MIXED) return UNC_ERR_UNSUPPORTED;
+  if (o->cam_params < 1 || o->cam_params > 9) return UNC_ERR_UNSUPPORTED;
+  if (o->num_cams <= 0 || o->num_points < 4) return UNC_ERR_BAD_INPUT;
+  const long long N = (long long)o->num_cams * o->cam_params;
+  if ((long long)ncols != N + 3LL * o->num_points) return UNC_ERR_BAD_INPUT;
+  if (nrows <= 0 || (nrows & 1)) return UNC_ERR_BAD_INPUT;
+  if ((long long)nrows * (o->cam_params + 3) > 0x7fffffffLL) return UNC_ERR_UNSUPPORTED;
+  if (N > 200000) return UNC_ERR_UNSUPPORTED;
+  if (o->pts2fix[0] < 0 || o->pts2fix[1] < 0 || o->pts2fix[2] < 0) return UNC_ERR_NO_FIXED_POINTS;
+  for (int i = 0; i < 3; ++i)
+    if (o->pts2fix[i] >= o->num_points) return UNC_ERR_BAD_INPUT;
+  if (o->pts2fix[0] == o->pts2fix[1] || o->pts2fix[0] == o->pts2fix[2] ||
+      o->pts2fix[1] == o->pts2fix[2])
+    return UNC_ERR_BAD_INPUT;
+  fix[0] = o->pts2fix[0];
+  fix[1] = o->pts2fix[1];
+  fix[2] = o->pts2fix[2];
+  if (fix[1] < fix[0]) std::swap(fix[0], fix[1]);  // sort(pts, pts + 2), compute.cpp:285
+  z->nc = o->num_cams;
+  z->cp = o->cam_params;
+  z->np = o->num_points;
+  z->npc = o->num_points - 3;
+  z->nobs = nrows / 2;
+  z->N = (int)N;
+  z->nn = (size_t)N * (size_t)N;
+  return UNC_OK;
+}
+
+int ensure(char **buf, size_t *have, size_t need) {
+  if (*have >= need) return UNC_OK;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *have = 0;
+  if (hipMalloc((void **)buf, need) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    return UNC_ERR_OOM;
+  }
+  *have = need;
+  return UNC_OK;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
+  return ms;
+}
+
+#define HC(expr)                                                                        \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      std::fprintf(stderr, "uncertainty_amd: %s failed (%s:%d): %s\n", #expr, __FILE__, \
+                   __LINE__, hipGetErrorString(e_));                                    \
+      return UNC_ERR_DEVICE;                                                            \
+    }                                                                                   \
+  } while (0)
+
+// Blocked in-place Gauss-Jordan inverse of the SPD matrix in W (n x n).
+int gj_inverse(int n, double *W, double *P, double *T, double *R, unsigned int *err,
+               hipStream_t s) {
+  for (int k0 = 0; k0 < n; k0 += GJ_BLOCK) {
+    const int kb = std::min(GJ_BLOCK, n - k0);
+    double *Wkk = W + k0 + (size_t)k0 * n;
+    HC(gj_small_inverse(kb, Wkk, n, P, err, s));
+    // T = W[:,K] P ; R = W[K,:]
+    HC(gemm_f64(n, kb, kb, 1.0, W + (size_t)k0 * n, n, P, kb, 0.0, T, n, nullptr, nullptr, s));
+    HC(copy_block(kb, n, W + k0, n, R, kb, 1.0, s));
+    // W -= T R ; W[K,:] = P R ; W[:,K] = -T ; W[K,K] = P
+    HC(gemm_f64(n, n, kb, -1.0, T, n, R, kb, 1.0, W, n, nullptr, nullptr, s));
+    HC(gemm_f64(kb, n, kb, 1.0, P, kb, R, kb, 0.0, W + k0, n, nullptr, nullptr, s));
+    HC(copy_block(n, kb, T, n, W + (size_t)k0 * n, n, -1.0, s));
+    HC(copy_block(kb, kb, P, kb, Wkk, n, 1.0, s));
+  }
+  return UNC_OK;
+}
+
+int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
+        const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
+        unc_debug *dbg, hipStream_t s, bool host_api_upload_timed) {
+  Sizes z;
+  int fix[3];
+  int rc = check_options(o, nrows, ncols, &z, fix);
+  if (rc) return rc;
+  const bool refine = o->refine != 0;
+  Layout L = make_layout(z, refine);
+  rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total);
+  if (rc) return rc;
+  char *w = ctx->ws;
+  auto ptr = [&](int b) { return (void *)(w + L.off[b]); };
+
+  Structure S;
+  S.cp = z.cp; S.nc = z.nc; S.np = z.np; S.npc = z.npc; S.nobs = z.nobs; S.N = z.N;
+  S.fix[0] = fix[0]; S.fix[1] = fix[1]; S.fix[2] = fix[2];
+  S.rows = d_rows; S.cols = d_cols; S.vals = d_vals;
+  S.obs_cam = (int *)ptr(B_OBS_CAM); S.obs_pt = (int *)ptr(B_OBS_PT);
+  S.cam_first = (int *)ptr(B_CAM_FIRST); S.cam_last = (int *)ptr(B_CAM_LAST);
+  S.pt_cnt = (int *)ptr(B_PT_CNT); S.pt_off = (int *)ptr(B_PT_OFF);
+  S.pt_list = (int *)ptr(B_PT_LIST); S.pt_cursor = (int *)ptr(B_PT_CURSOR);
+  S.err = (unsigned int *)ptr(B_ERR);
+  Blocks Bk;
+  Bk.Us = (double *)ptr(B_US); Bk.d = (double *)ptr(B_D); Bk.Vi = (double *)ptr(B_VI);
+  Bk.Ws = (double *)ptr(B_WS); Bk.Y = (double *)ptr(B_Y); Bk.pt_has = (int *)ptr(B_PT_HAS);
+  TeCtrl *ctrl = (TeCtrl *)ptr(B_CTRL);
+  double *dA = (double *)ptr(B_DA), *dB = (double *)ptr(B_DB), *dC = (double *)ptr(B_DC);
+  double *dD = (double *)ptr(B_DD), *dE = (double *)ptr(B_DE);
+  const int N = z.N;
+  const size_t nn = z.nn;
+  hipEvent_t *E = ctx->stage_ev;
+  if (!host_api_upload_timed) HC(hipEventRecord(E[0], s));
+  HC(hipEventRecord(E[1], s));
+
+  // ---- decode + fixPts
+  HC(launch_decode(S, s));
+  unsigned int herr = 0;
+  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr) return map_err(herr);
+  HC(hipEventRecord(E[2], s));
+  HC(launch_point_lists(S, (int *)ptr(B_SCAN), s));
+  HC(hipEventRecord(E[3], s));
+  // ---- (a) + (b)
+  HC(launch_assemble(S, Bk, s));
+  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr) return map_err(herr);
+  HC(hipEventRecord(E[4], s));
+  HC(hipEventRecord(E[5], s));
+  // ---- (c) Schur into dA
+  HC(hipMemsetAsync(dA, 0, sizeof(double) * nn, s));
+  HC(launch_schur(S, Bk, dA, s, ctx->max_lds));
+  HC(hipEventRecord(E[6], s));
+  if (dbg && dbg->Z) HC(hipMemcpyAsync(dbg->Z, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  // ---- (d) A = Z + lambda I ; X0 = A^-1
+  const double lambda = (o->lambda == -1) ? pow(10, -1.2653 * log10((double)N) - 2.9415) : o->lambda;
+  HC(add_diag(N, dA, N, lambda, s));
+  HC(hipMemcpyAsync(dB, dA, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+  rc = gj_inverse(N, dB, (double *)ptr(B_P), (double *)ptr(B_TP), (double *)ptr(B_RP), S.err, s);
+  if (rc) return rc;
+  hipEvent_t *GE = ctx->gemm_ev;
+  int gemm_pairs = 0;
+  double *X0 = dB;
+  if (refine) {
+    HC(residual_dd(N, dA, dB, dC, s));
+    HC(hipEventRecord(GE[2 * gemm_pairs], s));
+    HC(gemm_f64(N, N, N, 1.0, dB, N, dC, N, 0.0, dD, N, nullptr, nullptr, s));
+    HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
+    ++gemm_pairs;
+    HC(sym_update(N, dB, dD, dE, s));
+    X0 = dE;
+  }
+  HC(hipEventRecord(E[7], s));
+  if (dbg && dbg->X0) HC(hipMemcpyAsync(dbg->X0, X0, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  // ---- Taylor chain (compute.cpp:142-160); X in dA (A is dead), T ping-pongs
+  double *X = dA;
+  double *Tb0 = refine ? dC : dC, *Tb1 = dD;
+  HC(hipMemcpyAsync(X, X0, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+  {
+    TeCtrl h;
+    std::memset(&h, 0, sizeof(h));
+    const double dmin = 2.2250738585072014e-308;  // DBL_MIN, absMax's start value
+    unsigned long long mb;
+    std::memcpy(&mb, &dmin, sizeof(mb));
+    for (int i = 0; i < 21; ++i) h.maxbits[i] = mb;
+    h.old_change = 1.7976931348623157e308;  // DBL_MAX
+    h.active = 1;
+    HC(hipMemcpyAsync(ctrl, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    HC(hipStreamSynchronize(s));  // h is a stack object
+  }
+  const int first_chain_pair = gemm_pairs;
+  HC(hipEventRecord(GE[2 * gemm_pairs], s));
+  HC(gemm_f64(N, N, N, 1.0, X0, N, X0, N, 0.0, Tb0, N, &ctrl->maxbits[1], nullptr, s));
+  HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
+  ++gemm_pairs;
+  double *Tc = Tb0, *Tn = Tb1;
+  for (int i = 1; i < 20; ++i) {
+    const double k = pow(lambda, i) / (double)fact32(i - 1);
+    HC(te_control(ctrl, i, k, s));
+    HC(te_axpy(N, X, Tc, ctrl, s));
+    HC(hipEventRecord(GE[2 * gemm_pairs], s));
+    HC(gemm_f64(N, N, N, 1.0, Tc, N, X0, N, 0.0, Tn, N, &ctrl->maxbits[i + 1], &ctrl->active, s));
+    HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
+    ++gemm_pairs;
+    std::swap(Tc, Tn);
+  }
+  HC(hipEventRecord(E[8], s));
+  if (dbg && dbg->X) HC(hipMemcpyAsync(dbg->X, X, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  // ---- outputs
+  HC(launch_cam_output(S, Bk, X, d_cam, s));
+  HC(hipEventRecord(E[9], s));
+  HC(hipMemsetAsync(d_pt, 0, sizeof(double) * 6 * (size_t)z.np, s));
+  HC(launch_point_output(S, Bk, X, d_pt, z.np, s));
+  HC(hipEventRecord(E[10], s));
+  if (dbg) {
+    if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
+    if (dbg->Vinv_s) HC(hipMemcpyAsync(dbg->Vinv_s, Bk.Vi, sizeof(double) * 9 * (size_t)z.npc, hipMemcpyDeviceToHost, s));
+    if (dbg->d) HC(hipMemcpyAsync(dbg->d, Bk.d, sizeof(double) * ((size_t)N + 3 * (size_t)z.npc), hipMemcpyDeviceToHost, s));
+  }
+  TeCtrl hc;
+  HC(hipMemcpyAsync(&hc, ctrl, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr) return map_err(herr);
+
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->lambda = lambda;
+    st->fixed_pts[0] = fix[0]; st->fixed_pts[1] = fix[1]; st->fixed_pts[2] = fix[2];
+    st->num_terms = hc.nterms;
+    for (int i = 0; i < hc.nterms && i < 20; ++i) st->cycle_change[i] = hc.change[i];
+    double gms = 0.0;
+    int gcount = 0;
+    // executed GEMMs: refinement (if any), T1, and one per accepted term
+    const int executed = first_chain_pair + 1 + hc.nterms;
+    for (int p = 0; p < executed && p < gemm_pairs; ++p) {
+      gms += ev_ms(GE[2 * p], GE[2 * p + 1]);
+      ++gcount;
+    }
+    st->gemm_ms = gms;
+    st->gemm_count = gcount;
+    st->gemm_flops = 2.0 * (double)N * (double)N * (double)N * gcount;
+    st->stage_ms[1] = ev_ms(E[1], E[2]);
+    st->stage_ms[2] = ev_ms(E[2], E[3]);
+    st->stage_ms[3] = ev_ms(E[3], E[4]);
+    st->stage_ms[5] = ev_ms(E[5], E[6]);
+    st->stage_ms[6] = ev_ms(E[6], E[7]);
+    st->stage_ms[7] = ev_ms(E[7], E[8]);
+    st->stage_ms[8] = ev_ms(E[8], E[9]);
+    st->stage_ms[9] = ev_ms(E[9], E[10]);
+    st->stage_ms[11] = ev_ms(E[host_api_upload_timed ? 0 : 1], E[10]);
+    if (host_api_upload_timed) st->stage_ms[0] = ev_ms(E[0], E[1]);
+    st->time_create_j = st->stage_ms[0] * 1e-3;
+    st->time_fix_j = (st->stage_ms[1] + st->stage_ms[2]) * 1e-3;
+    st->time_multiply_jj = st->stage_ms[3] * 1e-3;
+    st->time_compose_z = st->stage_ms[5] * 1e-3;
+    st->time_inv_z = st->stage_ms[6] * 1e-3;
+    st->time_te = st->stage_ms[7] * 1e-3;
+    st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
+    st->time_all = st->stage_ms[11] * 1e-3;
+    st->num_obs_kept = -1;
+  }
+  return UNC_OK;
+}
+
+std::mutex g_default_mu;
+unc_context *g_default[64] = {nullptr};
+
+unc_context *default_context(int device, int *rc) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (device < 0 || device >= 64) {
+    *rc = UNC_ERR_BAD_INPUT;
+    return nullptr;
+  }
+  if (!g_default[device]) {
+    unc_context *c = nullptr;
+    *rc = unc_context_create(device, &c);
+    if (*rc) return nullptr;
+    g_default[device] = c;
+  }
+  *rc = UNC_OK;
+  return g_default[device];
+}
+
+int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
+                 const int *rows, const int *cols, const double *values, double *cam_unc,
+                 double *pt_unc, unc_stats *stats, unc_debug *dbg) {
+  if (!opts || !rows || !cols || !values || !cam_unc || !pt_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(opts->device, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  if (num_rows <= 0 || (num_rows & 1)) return UNC_ERR_BAD_INPUT;
+  const int cp = opts->cam_params;
+  if (cp < 1 || cp > 9) return UNC_ERR_UNSUPPORTED;
+  if (rows[0] != 0 || rows[num_rows] != num_rows * (cp + 3)) return UNC_ERR_BAD_INPUT;
+  const size_t nnz = (size_t)rows[num_rows];
+  const size_t per = (size_t)cp * (cp + 1) / 2;
+  const size_t cam_n = (size_t)opts->num_cams * per, pt_n = (size_t)opts->num_points * 6;
+  const size_t b_rows = ((sizeof(int) * (num_rows + 1) + 255) / 256) * 256;
+  const size_t b_cols = ((sizeof(int) * nnz + 255) / 256) * 256;
+  const size_t b_vals = ((sizeof(double) * nnz + 255) / 256) * 256;
+  const size_t b_cam = ((sizeof(double) * cam_n + 255) / 256) * 256;
+  const size_t b_pt = ((sizeof(double) * pt_n + 255) / 256) * 256;
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_rows + b_cols + b_vals + b_cam + b_pt);
+  if (rc) return rc;
+  int *d_rows = (int *)ctx->io;
+  int *d_cols = (int *)(ctx->io + b_rows);
+  double *d_vals = (double *)(ctx->io + b_rows + b_cols);
+  double *d_cam = (double *)(ctx->io + b_rows + b_cols + b_vals);
+  double *d_pt = (double *)(ctx->io + b_rows + b_cols + b_vals + b_cam);
+  hipStream_t s = ctx->stream;
+  HC(hipEventRecord(ctx->stage_ev[0], s));
+  HC(hipMemcpyAsync(d_rows, rows, sizeof(int) * (num_rows + 1), hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(d_cols, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(d_vals, values, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  rc = run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_vals, d_cam, d_pt, stats, dbg, s, true);
+  if (rc) return rc;
+  HC(hipMemcpyAsync(cam_unc, d_cam, sizeof(double) * cam_n, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(pt_unc, d_pt, sizeof(double) * pt_n, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  for (size_t i = 0; i < cam_n; ++i)
+    if (!std::isfinite(cam_unc[i])) return UNC_ERR_NUMERIC;
+  return UNC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void unc_default_options(unc_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->algorithm = UNC_ALG_TAYLOR_EXPANSION;
+  o->epsilon = 1e-10;
+  o->lambda = -1;
+  o->cam_params = 9;
+  o->svd_remove_n = -1;
+  o->max_iter_te = -1;
+  o->pts2fix[0] = o->pts2fix[1] = o->pts2fix[2] = -1;
+  o->precision = UNC_PREC_FP64;
+  o->refine = 1;
+  o->device = 0;
+}
+
+int unc_context_create(int device, unc_context **out) {
+  if (!out) return UNC_ERR_BAD_INPUT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return UNC_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) return UNC_ERR_BAD_INPUT;
+  unc_context *c = new unc_context();
+  c->device = device;
+  HC(hipSetDevice(device));
+  HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    c->max_lds = lds;
+  for (auto &e : c->stage_ev) HC(hipEventCreate(&e));
+  for (auto &e : c->gemm_ev) HC(hipEventCreate(&e));
+  *out = c;
+  return UNC_OK;
+}
+
+void unc_context_destroy(unc_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &e : c->stage_ev) (void)hipEventDestroy(e);
+  for (auto &e : c->gemm_ev) (void)hipEventDestroy(e);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->io) (void)hipFree(c->io);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int unc_compute(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
+                const int *rows, const int *cols, const double *values, double *cam_unc,
+                double *pt_unc, unc_stats *stats) {
+  return compute_host(ctx, opts, num_rows, num_cols, rows, cols, values, cam_unc, pt_unc, stats,
+                      nullptr);
+}
+
+int unc_compute_debug(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
+                      const int *rows, const int *cols, const double *values, double *cam_unc,
+                      double *pt_unc, unc_stats *stats, unc_debug *dbg) {
+  return compute_host(ctx, opts, num_rows, num_cols, rows, cols, values, cam_unc, pt_unc, stats,
+                      dbg);
+}
+
+int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
+                       const int *d_rows, const int *d_cols, const double *d_values,
+                       double *d_cam_unc, double *d_pt_unc, unc_stats *stats, void *stream) {
+  if (!opts || !d_rows || !d_cols || !d_values || !d_cam_unc || !d_pt_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(opts->device, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  return run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_values, d_cam_unc, d_pt_unc, stats,
+             nullptr, s, false);
+}
+
+unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
+  if (!o || o->num_cams <= 0 || o->cam_params <= 0 || o->num_points < 4 || num_rows <= 0) return 0;
+  Sizes z;
+  z.nc = o->num_cams; z.cp = o->cam_params; z.np = o->num_points; z.npc = z.np - 3;
+  z.nobs = num_rows / 2; z.N = z.nc * z.cp; z.nn = (size_t)z.N * z.N;
+  return (unsigned long long)make_layout(z, o->refine != 0).total;
+}
+
+const char *unc_error_string(int code) {
+  switch (code) {
+    case UNC_OK: return "ok";
+    case UNC_ERR_BAD_INPUT: return "bad input (Jacobian layout, sizes or ids)";
+    case UNC_ERR_NO_FIXED_POINTS: return "no fixed points given (the reference crashes here)";
+    case UNC_ERR_SINGULAR: return "singular input (zero Jacobian column or singular 3x3 point block)";
+    case UNC_ERR_OOM: return "out of device memory";
+    case UNC_ERR_DEVICE: return "HIP runtime error";
+    case UNC_ERR_UNSUPPORTED: return "unsupported option or layout";
+    case UNC_ERR_NUMERIC: return "non-finite result";
+    default: return "unknown error";
+  }
+}
+
+const char *unc_stage_name(int stage) {
+  return (stage >= 0 && stage < 12) ? kStageNames[stage] : "";
+}
+
+int unc_abi_version(void) { return UNC_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// kernel-level diagnostics (host pointers; used by the parity tests to pin one
+// kernel at a time)
+// ---------------------------------------------------------------------------
+int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc, double *absmax_out) {
+  if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < K || ldc < M) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * (na + nb + nc) + 256));
+  double *dA = (double *)buf, *dB = dA + na, *dC = dB + nb;
+  unsigned long long *dm = (unsigned long long *)(dC + nc);
+  HC(hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
+  HC(gemm_f64(M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, dm, nullptr, s));
+  unsigned long long mb = 0;
+  HC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
+  return UNC_OK;
+}
+
+int unc_diag_inverse(int n, const double *A, double *X, int refine) {
+  if (n <= 0) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n;
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * (5 * nn + (size_t)GJ_BLOCK * (GJ_BLOCK + 2 * n)) + 256));
+  double *dA = (double *)buf, *dW = dA + nn, *dR = dW + nn, *dD = dR + nn, *dX = dD + nn;
+  double *P = dX + nn, *T = P + GJ_BLOCK * GJ_BLOCK, *Rp = T + (size_t)GJ_BLOCK * n;
+  unsigned int *err = (unsigned int *)(Rp + (size_t)GJ_BLOCK * n);
+  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dW, dA, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+  rc = gj_inverse(n, dW, P, T, Rp, err, s);
+  if (rc) { (void)hipFree(buf); return rc; }
+  double *res = dW;
+  if (refine) {
+    HC(residual_dd(n, dA, dW, dR, s));
+    HC(gemm_f64(n, n, n, 1.0, dW, n, dR, n, 0.0, dD, n, nullptr, nullptr, s));
+    HC(sym_update(n, dW, dD, dX, s));
+    res = dX;
+  }
+  unsigned int herr = 0;
+  HC(hipMemcpyAsync(X, res, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  return herr ? map_err(herr) : UNC_OK;
+}
+
+int unc_diag_residual_dd(int n, const double *A, const double *X, double *R) {
+  if (n <= 0) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n;
+  double *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * nn));
+  HC(hipMemcpyAsync(buf, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(buf + nn, X, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  HC(hipMemcpyAsync(R, buf + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  return UNC_OK;
+}
+
+}  // extern "C"
