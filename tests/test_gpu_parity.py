@@ -83,7 +83,19 @@ def _cases():
         "daliborka": lambda: jacob.read_jacob(os.path.join(os.path.dirname(__file__), "golden", "04_Daliborka.jacob")),
         "toy10x1000": lambda: scene.scene_jacobian(scene.ring_scene(10, 1000, seed=1)),
         "rings40x4000": lambda: scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11)),
+        "rings40x4000_shuffled": lambda: _shuffled(scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11))),
     }
+
+
+def _shuffled(j):
+    """Same Jacobian with its observations (row pairs) in a random order."""
+    rng = np.random.default_rng(5)
+    rl = j.cam_params + 3
+    perm = rng.permutation(j.num_rows // 2)
+    j = copy.deepcopy(j)
+    j.cols = j.cols.reshape(-1, 2 * rl)[perm].reshape(-1).copy()
+    j.vals = j.vals.reshape(-1, 2 * rl)[perm].reshape(-1).copy()
+    return j
 
 
 @pytest.fixture(scope="module", params=list(_cases().keys()))
@@ -105,14 +117,14 @@ def test_pipeline_parity(case):
     assert np.array_equal(g["Z"], o["Z"]), "Schur complement"
     assert r.lambda_ == o["lambda_"]
     assert r.fixed == o["fixed"]
-    # tolerance stages
+    # tolerance stages: X0 after one compensated Newton step is ~cond * err0^2
     ex0 = np.linalg.norm(g["X0"] - o["X0"]) / np.linalg.norm(o["X0"])
-    assert ex0 < 1e-11, ex0
-    assert r.n_terms == o["n_terms"]
-    np.testing.assert_allclose(r.cycle_change, o["cycle_change"], rtol=1e-9)
     rc = block_rel(r.cam, o["cam"])
     rp = block_rel(r.pt, o["pt"])
     say(f"  {name}: X0 rel {ex0:.2e}  cam max {rc.max():.2e}  pt max {rp.max():.2e}")
+    assert ex0 < 1e-10, ex0
+    assert r.n_terms == o["n_terms"]
+    np.testing.assert_allclose(r.cycle_change, o["cycle_change"], rtol=1e-9)
     assert rc.max() <= TOL_BLOCK, rc.max()
     assert rp.max() <= TOL_BLOCK, rp.max()
     assert np.array_equal(r.pt == 0, o["pt"] == 0)

@@ -81,6 +81,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m uncertaintyte_amd.build`")
+    # One HIP runtime per process: when torch is installed it ships its own
+    # libamdhip64.so.7, and whichever copy is loaded first serves the process.
+    # Loading torch first keeps torch usable next to this library.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     L.unc_default_options.argtypes = [ctypes.POINTER(Options)]

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libuncertainty_amd.so")
 CLI = os.path.join(OUT, "uncertainty")
 
-HIP_SOURCES = ["k_assemble.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "pipeline.hip"]
+HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "pipeline.hip"]
 CPP_SOURCES = ["uncertainty_api.cpp"]
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")

@@ -15,7 +15,6 @@ enum : unsigned int {
   ERRBIT_RANGE = 2u,         // camera or point id out of range
   ERRBIT_ZERO_COLUMN = 4u,   // a Jacobian column is all zero (d = 1/0)
   ERRBIT_SINGULAR_V = 8u,    // a 3x3 point block has no finite inverse
-  ERRBIT_NOT_CAM_MAJOR = 16u,// observations of a camera are not contiguous
   ERRBIT_GJ_PIVOT = 32u,     // non-positive pivot in the dense inverse
   ERRBIT_SCHUR_OVERFLOW = 64u,  // a staged Schur batch exceeded its pair buffer
 };
@@ -55,7 +54,7 @@ struct Structure {
   // outputs of decode
   int *obs_cam;                     // camera per observation, -1 when dropped
   int *obs_pt;                      // compact point per observation, -1 when dropped
-  int *cam_first, *cam_last;        // observation range of each camera
+  int *cam_off, *cam_list;          // camera-grouped observations (row order)
   int *pt_cnt, *pt_off, *pt_list;   // point-grouped observations (sorted by index)
   int *pt_cursor;
   unsigned int *err;                // device error word
@@ -72,12 +71,17 @@ struct Blocks {
 
 hipError_t launch_decode(const Structure &s, hipStream_t st);
 hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st);
+size_t camera_sort_temp_bytes(int nobs, int nc);
+hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in, int *keys_out,
+                              int *vals_in, int *cam_list, int *cam_off, void *temp,
+                              size_t temp_bytes, hipStream_t st);
 hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st);
 hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, hipStream_t st,
                         int max_lds_bytes);
 hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *X,
                              double *cam_out, hipStream_t st);
-hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *X,
+hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, hipStream_t st);
+hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *Xd,
                                double *pt_out, int npts_out, hipStream_t st);
 
 // dense

@@ -42,10 +42,6 @@ __global__ void k_decode(Structure s) {
         if (c0[j] != cam0 + j || c1[j] != cam0 + j) e |= ERRBIT_LAYOUT;
       for (int j = 0; j < 3; ++j)
         if (c0[cp + j] != N + pc + j || c1[cp + j] != N + pc + j) e |= ERRBIT_LAYOUT;
-      if (o > 0) {
-        int prev = s.cols[(size_t)(r0 - 2) * rl];
-        if (prev >= 0 && prev / cp > a) e |= ERRBIT_NOT_CAM_MAJOR;
-      }
     }
   }
   if (e) {
@@ -65,14 +61,11 @@ __global__ void k_decode(Structure s) {
   s.obs_cam[o] = a;
   s.obs_pt[o] = pcmp;
   atomicAdd(&s.pt_cnt[pcmp], 1);
-  atomicMin(&s.cam_first[a], o);
-  atomicMax(&s.cam_last[a], o);
 }
 
 __global__ void k_init_lists(Structure s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < s.npc) { s.pt_cnt[i] = 0; s.pt_cursor[i] = 0; }
-  if (i < s.nc) { s.cam_first[i] = 0x7fffffff; s.cam_last[i] = -1; }
   if (i == 0) *s.err = 0u;
 }
 
@@ -169,12 +162,17 @@ __global__ void k_sort_segments(Structure s) {
   }
 }
 
+static void exclusive_scan(const int *cnt, int n, int *off, int *scan_tmp, hipStream_t st) {
+  int nb = (int)cdiv(n, SCAN_BLOCK);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, cnt, n, off, scan_tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, scan_tmp, nb, off + n);
+  hipLaunchKernelGGL(k_scan_add, dim3(cdiv(n, 256)), dim3(256), 0, st, off, n, scan_tmp);
+}
+
+
 hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st) {
   int n = s.npc;
-  int nb = (int)cdiv(n, SCAN_BLOCK);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, s.pt_cnt, n, s.pt_off, scan_tmp);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, scan_tmp, nb, s.pt_off + n);
-  hipLaunchKernelGGL(k_scan_add, dim3(cdiv(n, 256)), dim3(256), 0, st, s.pt_off, n, scan_tmp);
+  exclusive_scan(s.pt_cnt, n, s.pt_off, scan_tmp, st);
   if (s.nobs > 0) hipLaunchKernelGGL(k_fill_points, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
   hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(n, 256)), dim3(256), 0, st, s);
   return hipGetLastError();
@@ -184,6 +182,9 @@ hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st)
 // camera blocks: one wave per camera, lane = upper-triangle entry (i, j)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_cam_assemble(Structure s, Blocks b) {
+  // 64 observations at a time are staged in LDS (lane t loads observation t's
+  // two camera rows), then every lane runs its fma chain over them in order.
+  __shared__ double stage[64][18];
   __shared__ double U[81];
   __shared__ double dsh[9];
   const int a = blockIdx.x, l = threadIdx.x;
@@ -198,16 +199,27 @@ __global__ __launch_bounds__(64) void k_cam_assemble(Structure s, Blocks b) {
       r -= len;
     }
   }
-  const int first = s.cam_first[a], last = s.cam_last[a];
+  const int t0 = s.cam_off[a], t1 = s.cam_off[a + 1];
   double acc = 0.0;
-  if (l < npair) {
-    for (int o = first; o <= last; ++o) {
-      if (s.obs_cam[o] < 0) continue;  // observation of a fixed point
-      const double *r0 = s.vals + (size_t)o * 2 * rl;
-      const double *r1 = r0 + rl;
-      acc = fma(r0[i], r0[j], acc);
-      acc = fma(r1[i], r1[j], acc);
+  for (int tb = t0; tb < t1; tb += 64) {
+    const int nb = min(64, t1 - tb);
+    if (l < nb) {
+      const double *r0 = s.vals + (size_t)s.cam_list[tb + l] * 2 * rl;
+      for (int q = 0; q < cp; ++q) {
+        stage[l][q] = r0[q];
+        stage[l][9 + q] = r0[rl + q];
+      }
     }
+    __syncthreads();
+    if (l < npair) {
+      for (int q = 0; q < nb; ++q) {
+        acc = fma(stage[q][i], stage[q][j], acc);
+        acc = fma(stage[q][9 + i], stage[q][9 + j], acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (l < npair) {
     U[i * cp + j] = acc;
     U[j * cp + i] = acc;
   }

@@ -94,7 +94,8 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, dou
     }
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  // epilogue: the f64 16x16x4 result has one row group per register -- lane
+  // holds rows lane/16 + 4*r (r = 0..3) of column lane%16 of each 16x16 block
   double lmax = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, dou
       const int col = n0 + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
         if (row < M && col < N) {
           double v = alpha * acc[i][j][r];
           double *cp = C + (size_t)row + (size_t)col * ldc;
@@ -204,39 +205,50 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipSt
 // ---------------------------------------------------------------------------
 constexpr int GJ_B = 64;
 
+// 256 threads as a 16 x 16 grid; thread (ti, tj) owns rows ti + 16u and
+// columns tj + 16v (u, v < 4), so no integer division sits in the pivot loop.
 __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ldw, double *P,
                                                   unsigned int *err) {
-  __shared__ double Mx[GJ_B * GJ_B];
+  __shared__ double Mx[GJ_B][GJ_B + 1];  // [col][row]
   __shared__ double colp[GJ_B], rowp[GJ_B];
-  const int tid = threadIdx.x;
-  for (int x = tid; x < b * b; x += 256) {
-    int i = x % b, j = x / b;
-    Mx[x] = W[(size_t)i + (size_t)j * ldw];
-  }
+  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+  for (int v = 0; v < 4; ++v)
+    for (int u = 0; u < 4; ++u) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      Mx[j][i] = (i < b && j < b) ? W[(size_t)i + (size_t)j * ldw] : (i == j ? 1.0 : 0.0);
+    }
   __syncthreads();
   bool bad = false;
   for (int p = 0; p < b; ++p) {
-    if (tid < b) {
-      colp[tid] = Mx[tid + p * b];
-      rowp[tid] = Mx[p + tid * b];
+    if (tid < GJ_B) {
+      colp[tid] = Mx[p][tid];
+      rowp[tid] = Mx[tid][p];
     }
     __syncthreads();
     const double piv = colp[p];
     if (!(piv > 0.0)) bad = true;
     const double ip = 1.0 / piv;
-    for (int x = tid; x < b * b; x += 256) {
-      int i = x % b, j = x / b;
-      double v;
-      if (i == p && j == p) v = ip;
-      else if (i == p) v = rowp[j] * ip;
-      else if (j == p) v = -colp[i] * ip;
-      else v = Mx[x] - colp[i] * (rowp[j] * ip);
-      Mx[x] = v;
+    for (int v = 0; v < 4; ++v) {
+      const int j = tj + 16 * v;
+      const double rj = rowp[j] * ip;
+      for (int u = 0; u < 4; ++u) {
+        const int i = ti + 16 * u;
+        double x;
+        if (i == p && j == p) x = ip;
+        else if (i == p) x = rj;
+        else if (j == p) x = -colp[i] * ip;
+        else x = Mx[j][i] - colp[i] * rj;
+        Mx[j][i] = x;
+      }
     }
     __syncthreads();
   }
   if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
-  for (int x = tid; x < b * b; x += 256) P[x] = Mx[x];
+  for (int v = 0; v < 4; ++v)
+    for (int u = 0; u < 4; ++u) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      if (i < b && j < b) P[(size_t)i + (size_t)j * b] = Mx[j][i];
+    }
 }
 
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,

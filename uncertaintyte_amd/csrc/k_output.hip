@@ -41,8 +41,24 @@ hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *
   return hipGetLastError();
 }
 
+// compact copy of the cameras' own 9x9 blocks of X (row-major per block)
+__global__ void k_extract_diag(int nc, int cp, int N, const double *__restrict__ X,
+                               double *__restrict__ Xd) {
+  const int a = blockIdx.x;
+  const int ne = cp * cp;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int i = e / cp, j = e % cp;
+    Xd[(size_t)a * ne + e] = X[(size_t)(a * cp + i) + (size_t)(a * cp + j) * N];
+  }
+}
+
+hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, hipStream_t st) {
+  hipLaunchKernelGGL(k_extract_diag, dim3(s.nc), dim3(128), 0, st, s.nc, s.cp, s.N, X, Xd);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
-                                                      const double *__restrict__ X,
+                                                      const double *__restrict__ Xd,
                                                       double *__restrict__ out, int npts_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= s.npc || !b.pt_has[p]) return;
@@ -52,14 +68,15 @@ __global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
     const int o = s.pt_list[t];
     const int a = s.obs_cam[o];
     const double *Ws = b.Ws + (size_t)o * cp * 3;
-    const double *Xa = X + (size_t)a * cp + (size_t)a * cp * N;
+    const double *Xa = Xd + (size_t)a * cp * cp;  // row-major block
     double w[27];
     for (int q = 0; q < cp * 3; ++q) w[q] = Ws[q];
     double H[27];
     for (int i = 0; i < cp; ++i) {
-      double h0 = Xa[i] * w[0], h1 = Xa[i] * w[1], h2 = Xa[i] * w[2];
+      const double x0 = Xa[i * cp];
+      double h0 = x0 * w[0], h1 = x0 * w[1], h2 = x0 * w[2];
       for (int j = 1; j < cp; ++j) {
-        const double x = Xa[(size_t)i + (size_t)j * N];
+        const double x = Xa[i * cp + j];
         h0 = fma(x, w[j * 3 + 0], h0);
         h1 = fma(x, w[j * 3 + 1], h1);
         h2 = fma(x, w[j * 3 + 2], h2);
@@ -107,10 +124,10 @@ __global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
   o6[5] = (Q[8] * dp[2]) * dp[2];
 }
 
-hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *X,
+hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *Xd,
                                double *pt_out, int npts_out, hipStream_t st) {
   if (s.npc <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_point_output, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b, X, pt_out,
+  hipLaunchKernelGGL(k_point_output, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b, Xd, pt_out,
                      npts_out);
   return hipGetLastError();
 }

@@ -22,6 +22,7 @@ namespace unc {
 constexpr int SCHUR_THREADS = 256;
 constexpr int SCHUR_BATCH = 64;      // observations of camera a staged per step
 constexpr int SCHUR_MAXP = 2048;     // (o1, o2) pairs staged per step
+constexpr int SCHUR_GROUPS = 21;     // off-diagonal groups of 9 threads (threads 64..252)
 
 struct Pair {
   int o2;
@@ -31,6 +32,15 @@ struct Pair {
 
 __device__ inline int popc(unsigned int x) { return __popc(x); }
 
+// Work split inside the workgroup:
+//  * wave 0 owns the upper triangle of the diagonal block (a, a): one element
+//    per lane, accumulated in a register; its operands (Y_o1, W_o1) are staged.
+//  * threads 64..252 form 21 groups of 9; group g owns the off-diagonal slots
+//    s with (s - 1) % 21 == g, lane j of the group owns column j of those
+//    blocks (9 accumulators in LDS) and reads W_o2 row j from global memory,
+//    one pair ahead.
+// Each element still sees its (o1, o2) pairs in list order, so the fma chain is
+// the oracle's.
 __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl, double *Z,
                                                          int slots_per_chunk) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -45,16 +55,17 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
   unsigned int *bits = (unsigned int *)smem;
   int *wpre = (int *)(bits + ncw);
   double *ybuf = (double *)(smem + ((8 * ncw + 15) / 16) * 16);
-  Pair *pairs = (Pair *)(ybuf + SCHUR_BATCH * 27);
-  int *cnt = (int *)(pairs + SCHUR_MAXP);           // SCHUR_BATCH + 1 ints
-  double *acc = (double *)(((uintptr_t)(cnt + SCHUR_BATCH + 2) + 15) & ~(uintptr_t)15);
+  double *wbuf = ybuf + SCHUR_BATCH * 27;
+  Pair *pairs = (Pair *)(wbuf + SCHUR_BATCH * 27);
+  int *cnt = (int *)(pairs + SCHUR_MAXP);  // SCHUR_BATCH + 2 ints
+  int *o1buf = cnt + SCHUR_BATCH + 2;      // SCHUR_BATCH ints
+  double *acc = (double *)(((uintptr_t)(o1buf + SCHUR_BATCH) + 15) & ~(uintptr_t)15);
 
-  const int first = s.cam_first[a], last = s.cam_last[a];
+  const int c0 = s.cam_off[a], c1 = s.cam_off[a + 1];
   for (int w = tid; w < ncw; w += SCHUR_THREADS) bits[w] = 0u;
   __syncthreads();
-  for (int o1 = first + tid; o1 <= last; o1 += SCHUR_THREADS) {
-    if (s.obs_cam[o1] < 0) continue;
-    int p = s.obs_pt[o1];
+  for (int t1 = c0 + tid; t1 < c1; t1 += SCHUR_THREADS) {
+    int p = s.obs_pt[s.cam_list[t1]];
     for (int t = s.pt_off[p]; t < s.pt_off[p + 1]; ++t) {
       int b = s.obs_cam[s.pt_list[t]];
       if (b >= a) atomicOr(&bits[b >> 5], 1u << (b & 31));
@@ -72,20 +83,33 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
   __syncthreads();
   const int K = cnt[SCHUR_BATCH + 1];
   const int ne = cp * cp;
-  const int G = SCHUR_THREADS / ne;  // element-owner groups
-  const int g = tid / ne, e = tid % ne;
-  const int ei = e / cp, ej = e % cp;
-  const bool owner = tid < G * ne;
+  // diagonal-block role (wave 0): upper element (di, dj)
+  const int nupper = cp * (cp + 1) / 2;
+  int di = 0, dj = 0;
+  {
+    int r = tid;
+    for (int ii = 0; ii < cp; ++ii) {
+      int len = cp - ii;
+      if (r < len) { di = ii; dj = ii + r; break; }
+      r -= len;
+    }
+  }
+  const bool diag_owner = tid < nupper;
+  // off-diagonal role
+  const int og = (tid - 64) / 9, oj = (tid - 64) % 9;
+  const bool off_owner = tid >= 64 && og < SCHUR_GROUPS && oj < cp;
 
   for (int s0 = 0; s0 < K; s0 += slots_per_chunk) {
     const int S = min(slots_per_chunk, K - s0);
     for (int x = tid; x < S * ne; x += SCHUR_THREADS) acc[x] = 0.0;
+    double dacc = 0.0;
     __syncthreads();
-    for (int ob = first; ob <= last; ob += SCHUR_BATCH) {
-      // stage: per-o1 pair counts (first wave), Y rows, then the ordered pair list
-      int myc = 0, o1 = ob + tid, p = -1;
-      bool valid = tid < SCHUR_BATCH && o1 <= last && s.obs_cam[o1] >= 0;
+    for (int tb = c0; tb < c1; tb += SCHUR_BATCH) {
+      // stage: per-o1 pair counts, Y and W rows of o1, then the ordered pair list
+      int myc = 0, o1 = -1, p = -1;
+      const bool valid = tid < SCHUR_BATCH && tb + tid < c1;
       if (valid) {
+        o1 = s.cam_list[tb + tid];
         p = s.obs_pt[o1];
         for (int t = s.pt_off[p]; t < s.pt_off[p + 1]; ++t) {
           int b = s.obs_cam[s.pt_list[t]];
@@ -94,7 +118,10 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
           if (sl >= s0 && sl < s0 + S) ++myc;
         }
       }
-      if (tid < SCHUR_BATCH) cnt[tid] = myc;
+      if (tid < SCHUR_BATCH) {
+        cnt[tid] = myc;
+        o1buf[tid] = o1;
+      }
       __syncthreads();
       if (tid == 0) {
         int run = 0;
@@ -113,7 +140,11 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
       }
       if (valid) {
         const double *Y = bl.Y + (size_t)o1 * cp * 3;
-        for (int k = 0; k < cp * 3; ++k) ybuf[tid * 27 + k] = Y[k];
+        const double *W = bl.Ws + (size_t)o1 * cp * 3;
+        for (int k = 0; k < cp * 3; ++k) {
+          ybuf[tid * 27 + k] = Y[k];
+          wbuf[tid * 27 + k] = W[k];
+        }
         int w = cnt[tid];
         for (int t = s.pt_off[p]; t < s.pt_off[p + 1]; ++t) {
           int o2 = s.pt_list[t];
@@ -130,22 +161,61 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
         }
       }
       __syncthreads();
-      if (owner) {
-        for (int q = 0; q < npairs; ++q) {
-          Pair pr = pairs[q];
-          if (pr.slot % G != g) continue;
-          if (pr.slot + s0 == 0 && ej < ei) continue;  // diagonal block: upper triangle only
-          const double *y = ybuf + pr.t * 27 + ei * 3;
-          const double *w = bl.Ws + (size_t)pr.o2 * cp * 3 + ej * 3;
-          double v = acc[pr.slot * ne + e];
-          v = fma(y[0], w[0], v);
-          v = fma(y[1], w[1], v);
-          v = fma(y[2], w[2], v);
-          acc[pr.slot * ne + e] = v;
+      if (s0 == 0 && diag_owner) {
+        // diagonal block: slot 0 (camera a itself)
+        const int nb = min(SCHUR_BATCH, c1 - tb);
+        for (int t = 0; t < nb; ++t) {
+          const double *y = ybuf + t * 27 + di * 3;
+          for (int q = cnt[t]; q < cnt[t + 1]; ++q) {
+            Pair pr = pairs[q];
+            if (pr.slot != 0) continue;
+            const double *w = (pr.o2 == o1buf[t]) ? wbuf + t * 27 + dj * 3
+                                                  : bl.Ws + (size_t)pr.o2 * cp * 3 + dj * 3;
+            dacc = fma(y[0], w[0], dacc);
+            dacc = fma(y[1], w[1], dacc);
+            dacc = fma(y[2], w[2], dacc);
+          }
+        }
+      } else if (off_owner) {
+        auto match = [&](int q) {
+          while (q < npairs) {
+            int sl = pairs[q].slot + s0;
+            if (sl != 0 && (sl - 1) % SCHUR_GROUPS == og) break;
+            ++q;
+          }
+          return q;
+        };
+        int q = match(0);
+        double w0 = 0, w1 = 0, w2 = 0;
+        if (q < npairs) {
+          const double *w = bl.Ws + (size_t)pairs[q].o2 * cp * 3 + oj * 3;
+          w0 = w[0]; w1 = w[1]; w2 = w[2];
+        }
+        while (q < npairs) {
+          const int qn = match(q + 1);
+          double n0 = 0, n1 = 0, n2 = 0;
+          if (qn < npairs) {
+            const double *w = bl.Ws + (size_t)pairs[qn].o2 * cp * 3 + oj * 3;
+            n0 = w[0]; n1 = w[1]; n2 = w[2];
+          }
+          const Pair pr = pairs[q];
+          const double *y = ybuf + pr.t * 27;
+          double *ac = acc + pr.slot * ne + oj;
+          for (int i = 0; i < cp; ++i) {
+            double v = ac[i * cp];
+            v = fma(y[i * 3 + 0], w0, v);
+            v = fma(y[i * 3 + 1], w1, v);
+            v = fma(y[i * 3 + 2], w2, v);
+            ac[i * cp] = v;
+          }
+          w0 = n0; w1 = n1; w2 = n2;
+          q = qn;
         }
       }
       __syncthreads();
     }
+    if (s0 == 0 && diag_owner) acc[di * cp + dj] = dacc;
+    __syncthreads();
     // write the chunk's blocks (and their mirrors)
     for (int b = a + tid; b < nc; b += SCHUR_THREADS) {
       if (!((bits[b >> 5] >> (b & 31)) & 1u)) continue;
@@ -175,8 +245,8 @@ __global__ __launch_bounds__(SCHUR_THREADS) void k_schur(Structure s, Blocks bl,
 hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, hipStream_t st,
                         int max_lds_bytes) {
   const int ncw = (s.nc + 31) / 32;
-  size_t fixed = ((8 * (size_t)ncw + 15) / 16) * 16 + SCHUR_BATCH * 27 * sizeof(double) +
-                 SCHUR_MAXP * sizeof(Pair) + (SCHUR_BATCH + 2) * sizeof(int) + 32;
+  size_t fixed = ((8 * (size_t)ncw + 15) / 16) * 16 + 2 * SCHUR_BATCH * 27 * sizeof(double) +
+                 SCHUR_MAXP * sizeof(Pair) + (2 * SCHUR_BATCH + 2) * sizeof(int) + 32;
   const int ne = s.cp * s.cp;
   long long avail = (long long)max_lds_bytes - (long long)fixed;
   int slots = (int)(avail / (ne * (long long)sizeof(double)));

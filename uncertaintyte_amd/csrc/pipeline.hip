@@ -48,8 +48,9 @@ struct Layout {
 };
 
 enum {
-  B_OBS_CAM, B_OBS_PT, B_CAM_FIRST, B_CAM_LAST, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
-  B_ERR, B_SCAN, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
+  B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
+  B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
+  B_ERR, B_SCAN, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
   B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
 };
 
@@ -58,12 +59,14 @@ constexpr int GJ_BLOCK = 64;
 Layout make_layout(const Sizes &z, bool refine) {
   size_t sz[B_COUNT];
   sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
-  sz[B_CAM_FIRST] = sz[B_CAM_LAST] = sizeof(int) * (size_t)z.nc;
+  sz[B_CAM_OFF] = sizeof(int) * ((size_t)z.nc + 1);
+  sz[B_CAM_LIST] = sz[B_SORT_K0] = sz[B_SORT_K1] = sz[B_SORT_V] = sizeof(int) * (size_t)z.nobs;
+  sz[B_SORT_TMP] = camera_sort_temp_bytes(z.nobs, z.nc) + 256;
   sz[B_PT_CNT] = sz[B_PT_CURSOR] = sz[B_PT_HAS] = sizeof(int) * (size_t)z.npc;
   sz[B_PT_OFF] = sizeof(int) * ((size_t)z.npc + 1);
   sz[B_ERR] = 64;
-  sz[B_SCAN] = sizeof(int) * ((size_t)z.npc / 1024 + 2);
-  sz[B_US] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
+  sz[B_SCAN] = sizeof(int) * ((size_t)std::max(z.npc, z.nc) / 1024 + 2);
+  sz[B_US] = sz[B_XD] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
   sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
   sz[B_VI] = sizeof(double) * 9 * (size_t)z.npc;
   sz[B_WS] = sz[B_Y] = sizeof(double) * 3 * (size_t)z.cp * z.nobs;
@@ -84,7 +87,6 @@ Layout make_layout(const Sizes &z, bool refine) {
 
 int map_err(unsigned int e) {
   if (e & (ERRBIT_LAYOUT | ERRBIT_RANGE)) return UNC_ERR_BAD_INPUT;
-  if (e & ERRBIT_NOT_CAM_MAJOR) return UNC_ERR_UNSUPPORTED;
   if (e & (ERRBIT_ZERO_COLUMN | ERRBIT_SINGULAR_V | ERRBIT_GJ_PIVOT)) return UNC_ERR_SINGULAR;
   if (e & ERRBIT_SCHUR_OVERFLOW) return UNC_ERR_UNSUPPORTED;
   return UNC_ERR_BAD_INPUT;
@@ -199,7 +201,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   S.fix[0] = fix[0]; S.fix[1] = fix[1]; S.fix[2] = fix[2];
   S.rows = d_rows; S.cols = d_cols; S.vals = d_vals;
   S.obs_cam = (int *)ptr(B_OBS_CAM); S.obs_pt = (int *)ptr(B_OBS_PT);
-  S.cam_first = (int *)ptr(B_CAM_FIRST); S.cam_last = (int *)ptr(B_CAM_LAST);
+  S.cam_off = (int *)ptr(B_CAM_OFF);
+  S.cam_list = (int *)ptr(B_CAM_LIST);
   S.pt_cnt = (int *)ptr(B_PT_CNT); S.pt_off = (int *)ptr(B_PT_OFF);
   S.pt_list = (int *)ptr(B_PT_LIST); S.pt_cursor = (int *)ptr(B_PT_CURSOR);
   S.err = (unsigned int *)ptr(B_ERR);
@@ -223,6 +226,9 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (herr) return map_err(herr);
   HC(hipEventRecord(E[2], s));
   HC(launch_point_lists(S, (int *)ptr(B_SCAN), s));
+  HC(sort_obs_by_camera(z.nobs, z.nc, S.obs_cam, (int *)ptr(B_SORT_K0), (int *)ptr(B_SORT_K1),
+                        (int *)ptr(B_SORT_V), S.cam_list, S.cam_off, ptr(B_SORT_TMP),
+                        camera_sort_temp_bytes(z.nobs, z.nc) + 256, s));
   HC(hipEventRecord(E[3], s));
   // ---- (a) + (b)
   HC(launch_assemble(S, Bk, s));
@@ -294,7 +300,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   HC(launch_cam_output(S, Bk, X, d_cam, s));
   HC(hipEventRecord(E[9], s));
   HC(hipMemsetAsync(d_pt, 0, sizeof(double) * 6 * (size_t)z.np, s));
-  HC(launch_point_output(S, Bk, X, d_pt, z.np, s));
+  HC(launch_extract_diag(S, X, (double *)ptr(B_XD), s));
+  HC(launch_point_output(S, Bk, (double *)ptr(B_XD), d_pt, z.np, s));
   HC(hipEventRecord(E[10], s));
   if (dbg) {
     if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
