@@ -138,16 +138,26 @@ unsigned long long unc_workspace_bytes(const unc_options *opts, int num_rows);
  *                         *absmax_out = max|C| from its fused epilogue
  *   unc_diag_inverse      X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels,
  *                         plus the compensated Newton step when refine != 0
- *   unc_diag_residual_dd  R = I - A X with the compensated (Dot2) kernel */
+ *   unc_diag_residual_dd  R = I - A X with the compensated (Dot2) kernel
+ *   unc_diag_gemm_bench   timing of the GEMM alone */
 int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                       const double *B, int ldb, double beta, double *C, int ldc,
                       double *absmax_out);
 int unc_diag_inverse(int n, const double *A, double *X, int refine);
 int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
+/* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
+ * device-resident operands (throughput probe); variant 0 is the production
+ * kernel (K slab 32), 1-3 are tuning alternatives (occupancy 2, K slab 16,
+ * LDS double buffer). */
+int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
+/* Select the GEMM kernel used by every later call in this process (tuning). */
+int unc_diag_set_gemm_variant(int variant);
 
 const char *unc_error_string(int code);
 const char *unc_stage_name(int stage);
 int unc_abi_version(void);
+/* Largest LDS allocation of one workgroup on the device (bytes), -1 on error. */
+int unc_device_lds_bytes(int device);
 
 #ifdef __cplusplus
 }

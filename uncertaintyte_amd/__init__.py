@@ -113,6 +113,9 @@ def lib():
                                     ctypes.c_int, dp]
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
+    L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
+    L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
     _lib = L
     return L
 
@@ -249,6 +252,13 @@ def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alph
                                  Cf.ctypes.data, M, ctypes.byref(m))
     _check(rc, "unc_diag_gemm_f64")
     return np.asarray(Cf), m.value
+
+
+def diag_gemm_bench(n: int, reps: int = 10, variant: int = 0) -> float:
+    """Average ms of one n^3 FP64 GEMM launch on device-resident operands."""
+    ms = ctypes.c_double()
+    _check(lib().unc_diag_gemm_bench(n, reps, variant, ctypes.byref(ms)), "unc_diag_gemm_bench")
+    return ms.value
 
 
 def diag_inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:

@@ -22,6 +22,9 @@ LIB = os.path.join(OUT, "libuncertainty_amd.so")
 CLI = os.path.join(OUT, "uncertainty")
 
 HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "pipeline.hip"]
+# MFMA accumulators in VGPR form: the loop-carried accumulators of the GEMM
+# then stay in place instead of being copied between AGPRs and VGPRs each slab.
+PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CPP_SOURCES = ["uncertainty_api.cpp"]
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")
@@ -57,7 +60,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     for src in HIP_SOURCES + CPP_SOURCES:
         obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [hipcc] + HIPCC_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc] + HIPCC_FLAGS + PER_FILE_FLAGS.get(src, []) + lang + \
+            ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                                             text=True)))
         objs.append(obj)

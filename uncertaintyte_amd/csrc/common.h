@@ -76,8 +76,10 @@ hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in
                               int *vals_in, int *cam_list, int *cam_off, void *temp,
                               size_t temp_bytes, hipStream_t st);
 hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st);
-hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, hipStream_t st,
-                        int max_lds_bytes);
+hipError_t launch_covis(const Structure &s, unsigned int *covis, int *kcnt, int *kmax,
+                        hipStream_t st);
+hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, const unsigned int *covis,
+                        const int *kcnt, int kmax, hipStream_t st, int max_lds_bytes);
 hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *X,
                              double *cam_out, hipStream_t st);
 hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, hipStream_t st);
@@ -88,6 +90,7 @@ hipError_t launch_point_output(const Structure &s, const Blocks &b, const double
 hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
                     const int *active, hipStream_t st);
+void set_gemm_variant(int v);
 hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st);
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st);

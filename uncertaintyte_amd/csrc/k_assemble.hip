@@ -66,7 +66,10 @@ __global__ void k_decode(Structure s) {
 __global__ void k_init_lists(Structure s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < s.npc) { s.pt_cnt[i] = 0; s.pt_cursor[i] = 0; }
-  if (i == 0) *s.err = 0u;
+  if (i == 0) {
+    s.err[0] = 0u;
+    s.err[1] = 0u;  // largest co-visibility count (filled by k_covis)
+  }
 }
 
 hipError_t launch_decode(const Structure &s, hipStream_t st) {

@@ -18,10 +18,11 @@ namespace unc {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GB_M = 128, GB_N = 128, GB_K = 16, G_THREADS = 256;
+constexpr int GB_M = 128, GB_N = 128, G_THREADS = 256;
 constexpr int G_PAD = 4;
 
-__global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, double alpha,
+template <int GB_K, int MINB>
+__global__ __launch_bounds__(G_THREADS, MINB) void k_gemm_f64(int M, int N, int K, double alpha,
                                                        const double *__restrict__ A, int lda,
                                                        const double *__restrict__ B, int ldb,
                                                        double beta, double *__restrict__ C,
@@ -50,25 +51,26 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, dou
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
-  double ra[8], rb[8];
+  constexpr int NL = GB_K * 128 / G_THREADS;  // elements of each tile per thread
+  double ra[NL], rb[NL];
   auto load_tiles = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int idx = tid + G_THREADS * r;
-      int k = idx >> 7, m = idx & 127;  // A: 16 columns of 128 rows
+      int k = idx >> 7, m = idx & 127;  // A: GB_K columns of 128 rows
       int gm = m0 + m, gk = k0 + k;
       ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
-      int n = idx >> 4, kk = idx & 15;  // B: 128 columns of 16 rows
+      int n = idx / GB_K, kk = idx % GB_K;  // B: 128 columns of GB_K rows
       int gn = n0 + n, gk2 = k0 + kk;
       rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < NL; ++r) {
       int idx = tid + G_THREADS * r;
       As[idx >> 7][idx & 127] = ra[r];
-      Bs[idx & 15][idx >> 4] = rb[r];
+      Bs[idx % GB_K][idx / GB_K] = rb[r];
     }
   };
 
@@ -78,19 +80,28 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, dou
     store_tiles();
     __syncthreads();
     if (k0 + GB_K < K) load_tiles(k0 + GB_K);
+    // fragments of k-substep ks+1 are read from LDS while the 16 MFMAs of
+    // substep ks run (register double buffering)
+    double af[2][4], bf[2][4];
 #pragma unroll
-    for (int ks = 0; ks < GB_K; ks += 4) {
-      const int kr = ks + (lane >> 4);
-      double af[4], bf[4];
+    for (int i = 0; i < 4; ++i) af[0][i] = As[lane >> 4][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+    for (int j = 0; j < 4; ++j) bf[0][j] = Bs[lane >> 4][wn * 64 + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+    for (int ks = 0; ks < GB_K / 4; ++ks) {
+      const int cur = ks & 1, nxt = cur ^ 1;
+      if (ks + 1 < GB_K / 4) {
+        const int kr = 4 * (ks + 1) + (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) af[nxt][i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bf[nxt][j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[cur][i], bf[cur][j], acc[i][j], 0, 0, 0);
     }
   }
 
@@ -125,13 +136,163 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(int M, int N, int K, dou
   }
 }
 
+// Pipelined variant: K slabs of 32 in two LDS buffers (A as [k][m] with a row
+// stride of 144 doubles, B as [n][k] with 34, both conflict-free for the
+// fragment reads), one barrier per slab, global loads two slabs ahead.
+constexpr int G3_K = 32, G3_AS = 144, G3_BS = 34;
+constexpr size_t G3_LDS = sizeof(double) * 2 * (G3_K * G3_AS + 128 * G3_BS);
+
+__global__ __launch_bounds__(G_THREADS, 1) void k_gemm_f64_p(int M, int N, int K, double alpha,
+                                                            const double *__restrict__ A, int lda,
+                                                            const double *__restrict__ B, int ldb,
+                                                            double beta, double *__restrict__ C,
+                                                            int ldc, unsigned long long *absmax,
+                                                            const int *active) {
+  if (active && *active == 0) return;
+  extern __shared__ double g3_smem[];
+  double *As = g3_smem;                      // [2][G3_K][G3_AS]
+  double *Bs = g3_smem + 2 * G3_K * G3_AS;   // [2][128][G3_BS]
+  __shared__ double wmax[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
+  const int bid = blockIdx.x;
+  const int group = 8;
+  const int per_group = group * tiles_n;
+  const int first_m = (bid / per_group) * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int tm = first_m + (bid % per_group) % gsize;
+  const int tn = (bid % per_group) / gsize;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+
+  double4_t acc[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  double ra[16], rb[16];
+  // every load address is clamped into the matrix and out-of-range values are
+  // replaced by zero with a select: the slab loop is straight-line code, which
+  // keeps the accumulators resident in AGPRs across iterations
+  const int am = min(m0 + (tid & 127), M - 1);
+  const bool am_ok = m0 + (tid & 127) < M;
+  const int bn0 = n0 + (tid >> 5);
+  // the zero for out-of-range elements is applied with a bit mask, so the load
+  // itself is unconditional (a select would let the compiler sink the load
+  // into a branch and wait on it)
+  auto keep = [](bool c) { return c ? ~0ull : 0ull; };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gk = k0 + 2 * r + (tid >> 7);
+      const double va = A[(size_t)am + (size_t)min(gk, K - 1) * lda];
+      ra[r] = __longlong_as_double(__double_as_longlong(va) & keep(am_ok && gk < K));
+      const int gn = bn0 + 8 * r, gk2 = k0 + (tid & 31);
+      const double vb = B[(size_t)min(gk2, K - 1) + (size_t)min(gn, N - 1) * ldb];
+      rb[r] = __longlong_as_double(__double_as_longlong(vb) & keep(gn < N && gk2 < K));
+    }
+  };
+  auto store = [&](int buf) {
+    double *a = As + buf * G3_K * G3_AS;
+    double *b = Bs + buf * 128 * G3_BS;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int idx = tid + G_THREADS * r;
+      a[(idx >> 7) * G3_AS + (idx & 127)] = ra[r];
+      b[(idx >> 5) * G3_BS + (idx & 31)] = rb[r];
+    }
+  };
+
+  const int nk = (K + G3_K - 1) / G3_K;
+  load(0);
+  store(0);
+  load(G3_K);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const double *a = As + cur * G3_K * G3_AS + wm * 64 + (lane & 15);
+    const double *b = Bs + cur * 128 * G3_BS + (wn * 64 + (lane & 15)) * G3_BS;
+#pragma unroll
+    for (int ks = 0; ks < G3_K; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = a[kr * G3_AS + i * 16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = b[j * 16 * G3_BS + kr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    store(cur ^ 1);            // slab kt+1 (past the end: unused zeros)
+    load((kt + 2) * G3_K);     // slab kt+2 (clamped addresses past the end)
+    __syncthreads();
+  }
+
+  double lmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+        if (row < M && col < N) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+          lmax = fmax(lmax, fabs(v));
+        }
+      }
+    }
+  if (absmax) {
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if (lane == 0) wmax[wave] = lmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+      atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+static int g_gemm_variant = 0;
+
+void set_gemm_variant(int v) { g_gemm_variant = v; }
+
 hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
                     const int *active, hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
   const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
-  hipLaunchKernelGGL(k_gemm_f64, dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha, A, lda, B,
-                     ldb, beta, C, ldc, absmax, active);
+  switch (g_gemm_variant) {
+    case 1:
+      hipLaunchKernelGGL((k_gemm_f64<16, 2>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
+                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_gemm_f64<16, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
+                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+      break;
+    case 3: {
+      static bool attr = false;
+      if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_gemm_f64_p,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+      }
+      hipLaunchKernelGGL(k_gemm_f64_p, dim3(tiles), dim3(G_THREADS), G3_LDS, st, M, N, K, alpha, A,
+                         lda, B, ldb, beta, C, ldc, absmax, active);
+      break;
+    }
+    default:  // production: K slabs of 32
+      hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
+                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+  }
   return hipGetLastError();
 }
 
@@ -205,49 +366,76 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipSt
 // ---------------------------------------------------------------------------
 constexpr int GJ_B = 64;
 
-// 256 threads as a 16 x 16 grid; thread (ti, tj) owns rows ti + 16u and
-// columns tj + 16v (u, v < 4), so no integer division sits in the pivot loop.
+// 256 threads as a 16 x 16 grid; thread (ti, tj) keeps rows ti + 16u and
+// columns tj + 16v (u, v < 4) of the block in registers.  Each pivot step
+// publishes the pivot row and column through LDS (two barriers) and updates the
+// 16 register elements.
 __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ldw, double *P,
                                                   unsigned int *err) {
-  __shared__ double Mx[GJ_B][GJ_B + 1];  // [col][row]
   __shared__ double colp[GJ_B], rowp[GJ_B];
   const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-  for (int v = 0; v < 4; ++v)
-    for (int u = 0; u < 4; ++u) {
+  double m[4][4];  // m[u][v] = element (ti + 16u, tj + 16v)
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
       const int i = ti + 16 * u, j = tj + 16 * v;
-      Mx[j][i] = (i < b && j < b) ? W[(size_t)i + (size_t)j * ldw] : (i == j ? 1.0 : 0.0);
+      m[u][v] = (i < b && j < b) ? W[(size_t)i + (size_t)j * ldw] : (i == j ? 1.0 : 0.0);
     }
-  __syncthreads();
   bool bad = false;
   for (int p = 0; p < b; ++p) {
-    if (tid < GJ_B) {
-      colp[tid] = Mx[p][tid];
-      rowp[tid] = Mx[tid][p];
+    const int pu = p >> 4, pl = p & 15;
+    // publish column p (owned by tj == pl) and row p (owned by ti == pl)
+    if (tj == pl) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
+        colp[ti + 16 * u] = x;
+      }
+    }
+    if (ti == pl) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x = (u == pu) ? m[u][v] : x;
+        rowp[tj + 16 * v] = x;
+      }
     }
     __syncthreads();
     const double piv = colp[p];
     if (!(piv > 0.0)) bad = true;
     const double ip = 1.0 / piv;
-    for (int v = 0; v < 4; ++v) {
-      const int j = tj + 16 * v;
-      const double rj = rowp[j] * ip;
-      for (int u = 0; u < 4; ++u) {
-        const int i = ti + 16 * u;
+    double ci[4], rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rj[v] = rowp[tj + 16 * v] * ip;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = ti + 16 * u;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = tj + 16 * v;
         double x;
         if (i == p && j == p) x = ip;
-        else if (i == p) x = rj;
-        else if (j == p) x = -colp[i] * ip;
-        else x = Mx[j][i] - colp[i] * rj;
-        Mx[j][i] = x;
+        else if (i == p) x = rj[v];
+        else if (j == p) x = -ci[u] * ip;
+        else x = m[u][v] - ci[u] * rj[v];
+        m[u][v] = x;
       }
     }
     __syncthreads();
   }
   if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
-  for (int v = 0; v < 4; ++v)
-    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
       const int i = ti + 16 * u, j = tj + 16 * v;
-      if (i < b && j < b) P[(size_t)i + (size_t)j * b] = Mx[j][i];
+      if (i < b && j < b) P[(size_t)i + (size_t)j * b] = m[u][v];
     }
 }
 

@@ -50,7 +50,7 @@ struct Layout {
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
-  B_ERR, B_SCAN, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
+  B_ERR, B_SCAN, B_COVIS, B_KCNT, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
   B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
 };
 
@@ -65,6 +65,8 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_PT_CNT] = sz[B_PT_CURSOR] = sz[B_PT_HAS] = sizeof(int) * (size_t)z.npc;
   sz[B_PT_OFF] = sizeof(int) * ((size_t)z.npc + 1);
   sz[B_ERR] = 64;
+  sz[B_COVIS] = sizeof(unsigned int) * (size_t)z.nc * (((size_t)z.nc + 31) / 32);
+  sz[B_KCNT] = sizeof(int) * (size_t)z.nc;
   sz[B_SCAN] = sizeof(int) * ((size_t)std::max(z.npc, z.nc) / 1024 + 2);
   sz[B_US] = sz[B_XD] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
   sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
@@ -230,16 +232,23 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
                         (int *)ptr(B_SORT_V), S.cam_list, S.cam_off, ptr(B_SORT_TMP),
                         camera_sort_temp_bytes(z.nobs, z.nc) + 256, s));
   HC(hipEventRecord(E[3], s));
-  // ---- (a) + (b)
+  // ---- (a) + (b), and the co-visibility rows the Schur kernel is sized by
   HC(launch_assemble(S, Bk, s));
-  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  unsigned int *covis = (unsigned int *)ptr(B_COVIS);
+  int *kcnt = (int *)ptr(B_KCNT);
+  int *kmax_d = (int *)(S.err + 1);
+  HC(launch_covis(S, covis, kcnt, kmax_d, s));
+  unsigned int hbuf[2] = {0u, 0u};
+  HC(hipMemcpyAsync(hbuf, S.err, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
+  herr = hbuf[0];
   if (herr) return map_err(herr);
+  const int kmax = (int)hbuf[1];
   HC(hipEventRecord(E[4], s));
   HC(hipEventRecord(E[5], s));
   // ---- (c) Schur into dA
   HC(hipMemsetAsync(dA, 0, sizeof(double) * nn, s));
-  HC(launch_schur(S, Bk, dA, s, ctx->max_lds));
+  HC(launch_schur(S, Bk, dA, covis, kcnt, kmax, s, ctx->max_lds));
   HC(hipEventRecord(E[6], s));
   if (dbg && dbg->Z) HC(hipMemcpyAsync(dbg->Z, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   // ---- (d) A = Z + lambda I ; X0 = A^-1
@@ -526,6 +535,15 @@ const char *unc_stage_name(int stage) {
 
 int unc_abi_version(void) { return UNC_ABI_VERSION; }
 
+int unc_device_lds_bytes(int device) {
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return lds;
+}
+
 // ---------------------------------------------------------------------------
 // kernel-level diagnostics (host pointers; used by the parity tests to pin one
 // kernel at a time)
@@ -555,6 +573,46 @@ int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int ld
   HC(hipStreamSynchronize(s));
   HC(hipFree(buf));
   if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
+  return UNC_OK;
+}
+
+__global__ void k_fill_pattern(size_t n, double *A, double scale) {
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n;
+       x += (size_t)gridDim.x * blockDim.x)
+    A[x] = scale * (double)((x * 2654435761ull) % 1000003ull) / 1000003.0 - 0.5 * scale;
+}
+
+int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n;
+  double *buf = nullptr;
+  if (hipMalloc((void **)&buf, sizeof(double) * 3 * nn) != hipSuccess) {
+    (void)hipGetLastError();
+    return UNC_ERR_OOM;
+  }
+  hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
+  set_gemm_variant(variant);
+  HC(gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr, nullptr, s));
+  HC(hipEventRecord(ctx->gemm_ev[0], s));
+  for (int r = 0; r < reps; ++r)
+    HC(gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr, nullptr, s));
+  HC(hipEventRecord(ctx->gemm_ev[1], s));
+  HC(hipEventSynchronize(ctx->gemm_ev[1]));
+  *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
+  set_gemm_variant(0);
+  HC(hipFree(buf));
+  return UNC_OK;
+}
+
+int unc_diag_set_gemm_variant(int variant) {
+  if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
+  set_gemm_variant(variant);
   return UNC_OK;
 }
 
