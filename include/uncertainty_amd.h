@@ -48,8 +48,15 @@ extern "C" {
 #define UNC_ALG_TAYLOR_EXPANSION 2
 
 /* arithmetic of the Z inverse (the only stage whose precision is selectable) */
-#define UNC_PREC_FP64 0   /* plain FP64 TE path: reference output, <=1e-9 per block     */
-#define UNC_PREC_MIXED 1  /* gauge-fixed Z^+ with FP32 MFMA + FP64 refinement, <=1e-5 */
+#define UNC_PREC_FP64 0        /* plain FP64 TE path: reference output, <=1e-9 per block  */
+#define UNC_PREC_MIXED 1       /* gauge-fixed Z^+, FP32 MFMA inverses + FP64 residuals,
+                                  <=1e-5 per block vs the FP64 pseudo-inverse          */
+#define UNC_PREC_ZPLUS_FP64 2  /* gauge-fixed Z^+ with FP64 inverses (the FP64 Z^+)     */
+/* Output semantics: UNC_PREC_FP64 reproduces the reference TAYLOR output
+ * (including its (X[j,k] + X[k,j]/2) camera packing).  The two Z^+ modes return
+ * the covariance D Z^+ D of the gauge-fixed problem (camera upper triangles, no
+ * 1.5 factor) and the points propagated from it; they are compared with an FP64
+ * pseudo-inverse that drops the 7 gauge eigenvalues, not with the reference. */
 
 typedef struct unc_options {
   int algorithm;      /* UNC_ALG_*; the device implements TAYLOR_EXPANSION           */
@@ -63,8 +70,10 @@ typedef struct unc_options {
   int max_iter_te;    /* unused by the reference; kept for layout parity              */
   int pts2fix[3];     /* fixed point ids, -1 -1 -1 = none                             */
   int precision;      /* UNC_PREC_*                                                   */
-  int refine;         /* FP64 path: one Newton step with a compensated residual on
-                         (Z + lambda I)^-1 (default 1)                                */
+  int refine;         /* Newton steps on the inverse; -1 = automatic (default):
+                         FP64 path 1 step with a compensated residual on
+                         (Z + lambda I)^-1, MIXED 2 steps and ZPLUS_FP64 1 step on
+                         (Z + N N^T)^-1; 0 disables refinement                    */
   int device;         /* HIP device ordinal                                           */
   int flags;          /* reserved, 0                                                  */
 } unc_options;
@@ -98,7 +107,7 @@ typedef struct unc_debug {
 
 typedef struct unc_context unc_context;
 
-/* Fill defaults: TAYLOR_EXPANSION, lambda -1, FP64, refine 1, device 0. */
+/* Fill defaults: TAYLOR_EXPANSION, lambda -1, FP64, refine -1 (auto), device 0. */
 void unc_default_options(unc_options *opts);
 
 /* Per-device context: stream, reusable workspace, timing events.  Reentrant
@@ -145,6 +154,10 @@ int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int ld
                       double *absmax_out);
 int unc_diag_inverse(int n, const double *A, double *X, int refine);
 int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
+/* Cholesky A = L L^T of SPD A (n x n, column-major) and X = L^-1 by the recursive
+   kernels of the Z^+ path; L's upper triangle above the 64 x 64 diagonal blocks is
+   left as A's (only the lower triangle is the factor), X's upper triangle is 0. */
+int unc_diag_chol_inverse(int n, const double *A, double *L, double *X);
 /* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
  * device-resident operands (throughput probe); variant 0 is the production
  * kernel (K slab 32), 1-3 are tuning alternatives (occupancy 2, K slab 16,

@@ -206,3 +206,34 @@ def factorial32(n: int) -> int:
 
 def auto_lambda(N: int) -> float:
     return lib().orc_auto_lambda(N)
+
+
+def zplus_reference(jac, fix=None, nthreads: int = 0):
+    """FP64 gauge-fixed covariance: X = pinv_7(Z_s) (eigen-decomposition of the
+    oracle's Schur complement, the 7 smallest eigenvalues dropped), camera
+    output = upper triangles of D X D per camera, points propagated from X's
+    diagonal blocks with the same formula as the TAYLOR path.
+    Returns dict(cam, pt, X, eig) -- the reference for the Z^+ precision modes."""
+    r = pipeline(jac, nthreads=nthreads, fix=fix, debug=True)
+    Z, d = r["Z"], r["d"]
+    w, Q = np.linalg.eigh(Z)
+    inv = np.zeros_like(w)
+    inv[7:] = 1.0 / w[7:]
+    X = (Q * inv) @ Q.T
+    cp, nc = jac.cam_params, jac.num_cams
+    iu = np.triu_indices(cp)
+    cam = np.empty((nc, cp * (cp + 1) // 2))
+    for a in range(nc):
+        s = slice(a * cp, (a + 1) * cp)
+        B = X[s, s] * d[s][:, None] * d[s][None, :]
+        cam[a] = B[iu]
+    nr, ncl, rows, cols, vals, fx = fixpts(jac, fix)
+    pt = np.zeros(jac.num_points * 6)
+    Xc = np.ascontiguousarray(X.T).reshape(-1)  # column-major
+    rc = lib().orc_points_from_X(nc, cp, jac.num_points,
+                                 np.array(fix if fix is not None else jac.fix_array(), np.int32),
+                                 nr, np.ascontiguousarray(cols, np.int32),
+                                 np.ascontiguousarray(vals, np.float64), Xc, pt)
+    if rc:
+        raise OracleError(rc, "orc_points_from_X")
+    return dict(cam=cam, pt=pt.reshape(-1, 6), X=X, eig=w)

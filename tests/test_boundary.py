@@ -36,6 +36,17 @@ def test_library_exports_every_declared_symbol():
     assert "getCovariancesLastError()" in out
 
 
+def test_pointer_taking_entry_points_have_ctypes_signatures():
+    """Every declared function with pointer parameters must have argtypes in the
+    binding (without them ctypes passes 64-bit addresses as 32-bit ints)."""
+    txt = open(os.path.join(INC, "uncertainty_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    L = u.lib()
+    for name, params in re.findall(r"\b(unc_[a-z0-9_]+)\s*\(([^)]*)\)", txt):
+        if "*" in params:
+            assert getattr(L, name).argtypes is not None, name
+
+
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "lay.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "uncertainty_amd.h"\n'
@@ -53,7 +64,7 @@ def test_struct_layouts_match_c(tmp_path):
 def test_default_options_and_strings():
     o = u.Options()
     u.lib().unc_default_options(ctypes.byref(o))
-    assert o.algorithm == 2 and o.lambda_ == -1 and o.precision == 0 and o.refine == 1
+    assert o.algorithm == 2 and o.lambda_ == -1 and o.precision == 0 and o.refine == -1
     assert list(o.pts2fix) == [-1, -1, -1]
     assert u.lib().unc_abi_version() == 1
     for c in range(8):

@@ -1,0 +1,95 @@
+"""GPU: the gauge-fixed Z^+ modes against the FP64 pseudo-inverse reference.
+
+Reference (oracle.zplus_reference): X = pinv_7(Z_s) from an FP64 eigen-
+decomposition of the oracle's Schur complement (bit-identical to the GPU's),
+camera output = upper triangles of D X D, points propagated from X's diagonal
+blocks.  Contracts, relative Frobenius error per block:
+  UNC_PREC_MIXED       <= 1e-5 for cameras and points;
+  UNC_PREC_ZPLUS_FP64  <= 1e-8 for cameras, <= 1e-6 for points (short-baseline
+                       points amplify the ~1e-11 camera differences by up to
+                       1e4 through V^-1; both sides are FP64).
+The recursive Cholesky + inverse kernels are checked directly against LAPACK.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import uncertaintyte_amd as u
+from uncertaintyte_amd import jacob, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def block_rel(a, b):
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    return np.where(nb > 0, d / np.where(nb > 0, nb, 1), d)
+
+
+@pytest.mark.parametrize("n", [1, 37, 64, 65, 130, 200, 777])
+def test_chol_inverse_matches_lapack(n):
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    print(f"chol_inverse n={n}", flush=True)
+    L, X = u.diag_chol_inverse(A)
+    Lr = np.linalg.cholesky(A)
+    assert np.allclose(L, Lr, rtol=0, atol=1e-12 * np.abs(Lr).max())
+    assert np.abs(np.triu(X, 1)).max() == 0.0
+    R = X @ Lr - np.eye(n)
+    assert np.abs(R).max() < 1e-12
+
+
+def test_chol_inverse_rejects_indefinite():
+    A = np.diag([1.0, -1.0, 2.0])
+    with pytest.raises(u.UncError) as e:
+        u.diag_chol_inverse(A)
+    assert e.value.code == u.UNC_ERR_SINGULAR
+
+
+CASES = {
+    "daliborka": lambda: jacob.read_jacob(os.path.join(os.path.dirname(__file__), "golden",
+                                                       "04_Daliborka.jacob")),
+    "cube": lambda: jacob.read_jacob(os.path.join(os.path.dirname(__file__), "golden",
+                                                  "01_Cube.jacob")),
+    "rings200": lambda: scene.make_workload("rings200"),
+    "walk300": lambda: scene.scene_jacobian(scene.walk_scene(300, 60000, 5.0, seed=3)),
+}
+TOL = {u.PREC_MIXED: (1e-5, 1e-5), u.PREC_ZPLUS_FP64: (1e-8, 1e-6)}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def zcase(request):
+    j = CASES[request.param]()
+    return request.param, j, oracle.zplus_reference(j)
+
+
+@pytest.mark.parametrize("prec", [u.PREC_MIXED, u.PREC_ZPLUS_FP64])
+def test_zplus_modes(zcase, prec):
+    name, j, ref = zcase
+    print(f"zplus {name} precision={prec}", flush=True)
+    r = u.compute_covariances(j, precision=prec)
+    rc = block_rel(r.cam, ref["cam"])
+    mask = np.linalg.norm(ref["pt"], axis=1) > 0
+    rp = block_rel(r.pt[mask], ref["pt"][mask])
+    print(f"  {name}: cam max {rc.max():.2e} median {np.median(rc):.2e}  pt max {rp.max():.2e}",
+          flush=True)
+    tc, tp = TOL[prec]
+    assert rc.max() <= tc
+    assert rp.max() <= tp
+    assert np.all(r.pt[~mask] == 0)
+
+
+def test_zplus_repeatable_and_context_reuse(daliborka):
+    ctx = u.Context()
+    a = u.compute_covariances(daliborka, precision=u.PREC_ZPLUS_FP64, ctx=ctx.handle)
+    b = u.compute_covariances(scene.make_workload("toy"), precision=u.PREC_ZPLUS_FP64,
+                              ctx=ctx.handle)
+    c = u.compute_covariances(daliborka, precision=u.PREC_ZPLUS_FP64, ctx=ctx.handle)
+    assert np.array_equal(a.cam, c.cam) and np.array_equal(a.pt, c.pt)
+    assert b.cam.shape[0] == 10
+    ctx.close()

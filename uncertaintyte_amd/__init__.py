@@ -37,7 +37,7 @@ UNC_ERR_OOM = 4
 UNC_ERR_DEVICE = 5
 UNC_ERR_UNSUPPORTED = 6
 UNC_ERR_NUMERIC = 7
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_ZPLUS_FP64 = 0, 1, 2
 TAYLOR_EXPANSION = 2
 
 
@@ -113,6 +113,7 @@ def lib():
                                     ctypes.c_int, dp]
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
+    L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
     L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
@@ -141,7 +142,8 @@ def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: i
     for i in range(3):
         o.pts2fix[i] = int(fx[i])
     o.precision = precision
-    o.refine = 1 if refine else 0
+    # True -> automatic number of Newton steps, False -> none, int -> that many
+    o.refine = -1 if refine is True else (0 if refine is False else int(refine))
     o.device = device
     return o
 
@@ -268,6 +270,17 @@ def diag_inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:
     _check(lib().unc_diag_inverse(n, Af.ctypes.data, X.ctypes.data, 1 if refine else 0),
            "unc_diag_inverse")
     return np.asarray(X)
+
+
+def diag_chol_inverse(A: np.ndarray):
+    """(L, X): Cholesky factor of SPD A (lower triangle) and X = L^-1."""
+    n = A.shape[0]
+    Af = np.asfortranarray(A, dtype=np.float64)
+    L = np.zeros((n, n), order="F")
+    X = np.zeros((n, n), order="F")
+    _check(lib().unc_diag_chol_inverse(n, Af.ctypes.data, L.ctypes.data, X.ctypes.data),
+           "unc_diag_chol_inverse")
+    return np.tril(np.asarray(L)), np.asarray(X)
 
 
 def diag_residual_dd(A: np.ndarray, X: np.ndarray) -> np.ndarray:

@@ -91,6 +91,34 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                     int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
                     const int *active, hipStream_t st);
 void set_gemm_variant(int v);
+// triangular-operand flags of gemm_f64_ex
+constexpr int GEMM_A_LOWER = 1;   // A lower triangular: K range ends at the tile's last row
+constexpr int GEMM_B_LOWER = 2;   // B lower triangular: K range starts at the tile's first column
+constexpr int GEMM_BT_LOWER = 4;  // op(B) = S^T with S lower: K range ends at the tile's last column
+constexpr int GEMM_C_LOWER = 8;   // only tiles on or below the diagonal of C are written
+hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                       int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
+                       hipStream_t st);
+// gauge-fixed Z^+ path (k_zplus.hip)
+hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
+                         hipStream_t st);
+hipError_t skinny_tn(int n, int kc, const double *A, int lda, const double *B, double *C,
+                     bool lower, hipStream_t st);
+hipError_t skinny_nn(int n, int kc, const double *A, int lda, const double *B, double *C,
+                     double *part, bool lower, hipStream_t st);
+size_t skinny_part_doubles(int n);
+hipError_t gram(int n, int k1, const double *V, int k2, const double *W, double *partial,
+                double *G, hipStream_t st);
+hipError_t small_orth(int k, const double *G, double *T, unsigned int *err, hipStream_t st);
+hipError_t small_woodbury(int k, const double *G, double *S, unsigned int *err, hipStream_t st);
+hipError_t small_apply(int n, int k1, int k2, double alpha, const double *A, double beta,
+                       const double *B, const double *T, double *out, hipStream_t st);
+hipError_t fill_hash(size_t cnt, double *V, unsigned long long seed, hipStream_t st);
+hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st);
+hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const double *S,
+                    const double *Nb, double *Xd, hipStream_t st);
+hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,
+                          hipStream_t st);
 hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st);
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st);

@@ -21,13 +21,13 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int GB_M = 128, GB_N = 128, G_THREADS = 256;
 constexpr int G_PAD = 4;
 
-template <int GB_K, int MINB>
+template <int GB_K, int MINB, bool TRANS_B = false>
 __global__ __launch_bounds__(G_THREADS, MINB) void k_gemm_f64(int M, int N, int K, double alpha,
                                                        const double *__restrict__ A, int lda,
                                                        const double *__restrict__ B, int ldb,
                                                        double beta, double *__restrict__ C,
                                                        int ldc, unsigned long long *absmax,
-                                                       const int *active) {
+                                                       const int *active, int tri = 0) {
   if (active && *active == 0) return;
   __shared__ double As[GB_K][GB_M + G_PAD];
   __shared__ double Bs[GB_K][GB_N + G_PAD];
@@ -46,6 +46,12 @@ __global__ __launch_bounds__(G_THREADS, MINB) void k_gemm_f64(int M, int N, int 
   const int tm = first_m + (bid % per_group) % gsize;
   const int tn = (bid % per_group) / gsize;
   const int m0 = tm * GB_M, n0 = tn * GB_N;
+  // triangular operands: only the K range that can be non-zero is visited
+  if ((tri & GEMM_C_LOWER) && n0 >= m0 + GB_M) return;  // tile strictly above the diagonal
+  int kbeg = 0, kend = K;
+  if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GB_M);
+  if (tri & GEMM_B_LOWER) kbeg = n0;
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GB_N);
 
   double4_t acc[4][4];
   for (int i = 0; i < 4; ++i)
@@ -59,10 +65,15 @@ __global__ __launch_bounds__(G_THREADS, MINB) void k_gemm_f64(int M, int N, int 
       int idx = tid + G_THREADS * r;
       int k = idx >> 7, m = idx & 127;  // A: GB_K columns of 128 rows
       int gm = m0 + m, gk = k0 + k;
-      ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
-      int n = idx / GB_K, kk = idx % GB_K;  // B: 128 columns of GB_K rows
-      int gn = n0 + n, gk2 = k0 + kk;
-      rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      ra[r] = (gm < M && gk < kend) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      if (TRANS_B) {  // B(k, n) = Bmem[n + k ldb]: 128 contiguous n per k
+        int gn = n0 + m;
+        rb[r] = (gn < N && gk < kend) ? B[(size_t)gn + (size_t)gk * ldb] : 0.0;
+      } else {
+        int n = idx / GB_K, kk = idx % GB_K;  // B: 128 columns of GB_K rows
+        int gn = n0 + n, gk2 = k0 + kk;
+        rb[r] = (gn < N && gk2 < kend) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      }
     }
   };
   auto store_tiles = [&]() {
@@ -70,16 +81,17 @@ __global__ __launch_bounds__(G_THREADS, MINB) void k_gemm_f64(int M, int N, int 
     for (int r = 0; r < NL; ++r) {
       int idx = tid + G_THREADS * r;
       As[idx >> 7][idx & 127] = ra[r];
-      Bs[idx % GB_K][idx / GB_K] = rb[r];
+      if (TRANS_B) Bs[idx >> 7][idx & 127] = rb[r];
+      else Bs[idx % GB_K][idx / GB_K] = rb[r];
     }
   };
 
-  load_tiles(0);
-  for (int k0 = 0; k0 < K; k0 += GB_K) {
+  load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GB_K) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (k0 + GB_K < K) load_tiles(k0 + GB_K);
+    if (k0 + GB_K < kend) load_tiles(k0 + GB_K);
     // fragments of k-substep ks+1 are read from LDS while the 16 MFMAs of
     // substep ks run (register double buffering)
     double af[2][4], bf[2][4];
@@ -271,11 +283,11 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
   switch (g_gemm_variant) {
     case 1:
       hipLaunchKernelGGL((k_gemm_f64<16, 2>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
-                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+                         A, lda, B, ldb, beta, C, ldc, absmax, active, 0);
       break;
     case 2:
       hipLaunchKernelGGL((k_gemm_f64<16, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
-                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+                         A, lda, B, ldb, beta, C, ldc, absmax, active, 0);
       break;
     case 3: {
       static bool attr = false;
@@ -291,8 +303,24 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
     }
     default:  // production: K slabs of 32
       hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
-                         A, lda, B, ldb, beta, C, ldc, absmax, active);
+                         A, lda, B, ldb, beta, C, ldc, absmax, active, 0);
   }
+  return hipGetLastError();
+}
+
+// C = alpha A op(B) + beta C with op(B) = B or B^T and the triangular flags
+// GEMM_* (lower-triangular A / B / B^T operands, lower-only C tiles)
+hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                       int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
+                       hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
+  if (trans_b)
+    hipLaunchKernelGGL((k_gemm_f64<32, 1, true>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K,
+                       alpha, A, lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<32, 1, false>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K,
+                       alpha, A, lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
   return hipGetLastError();
 }
 
@@ -370,17 +398,18 @@ constexpr int GJ_B = 64;
 // columns tj + 16v (u, v < 4) of the block in registers.  Each pivot step
 // publishes the pivot row and column through LDS (two barriers) and updates the
 // 16 register elements.
-__global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ldw, double *P,
+template <typename T>
+__global__ __launch_bounds__(256) void k_gj_small(int b, const T *W, int ldw, T *P,
                                                   unsigned int *err) {
-  __shared__ double colp[GJ_B], rowp[GJ_B];
+  __shared__ T colp[GJ_B], rowp[GJ_B];
   const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-  double m[4][4];  // m[u][v] = element (ti + 16u, tj + 16v)
+  T m[4][4];  // m[u][v] = element (ti + 16u, tj + 16v)
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int i = ti + 16 * u, j = tj + 16 * v;
-      m[u][v] = (i < b && j < b) ? W[(size_t)i + (size_t)j * ldw] : (i == j ? 1.0 : 0.0);
+      m[u][v] = (i < b && j < b) ? W[(size_t)i + (size_t)j * ldw] : (T)(i == j ? 1 : 0);
     }
   bool bad = false;
   for (int p = 0; p < b; ++p) {
@@ -389,7 +418,7 @@ __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ld
     if (tj == pl) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        double x = 0.0;
+        T x = 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
         colp[ti + 16 * u] = x;
@@ -398,17 +427,17 @@ __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ld
     if (ti == pl) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        double x = 0.0;
+        T x = 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) x = (u == pu) ? m[u][v] : x;
         rowp[tj + 16 * v] = x;
       }
     }
     __syncthreads();
-    const double piv = colp[p];
-    if (!(piv > 0.0)) bad = true;
-    const double ip = 1.0 / piv;
-    double ci[4], rj[4];
+    const T piv = colp[p];
+    if (!(piv > (T)0)) bad = true;
+    const T ip = (T)1 / piv;
+    T ci[4], rj[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
 #pragma unroll
@@ -419,7 +448,7 @@ __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ld
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int j = tj + 16 * v;
-        double x;
+        T x;
         if (i == p && j == p) x = ip;
         else if (i == p) x = rj[v];
         else if (j == p) x = -ci[u] * ip;
@@ -442,28 +471,28 @@ __global__ __launch_bounds__(256) void k_gj_small(int b, const double *W, int ld
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st) {
   if (b > GJ_B) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(256), 0, st, b, W, ldw, P, err);
+  hipLaunchKernelGGL(k_gj_small<double>, dim3(1), dim3(256), 0, st, b, W, ldw, P, err);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // small element-wise helpers
 // ---------------------------------------------------------------------------
-__global__ void k_copy_block(int m, int n, const double *src, int lds, double *dst, int ldd,
-                             double scale) {
+template <typename T>
+__global__ void k_copy_block(int m, int n, const T *src, int lds, T *dst, int ldd, T scale) {
   size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= (size_t)m * n) return;
   int i = (int)(x % m), j = (int)(x / m);
-  double v = src[(size_t)i + (size_t)j * lds];
-  dst[(size_t)i + (size_t)j * ldd] = (scale == 1.0) ? v : scale * v;
+  T v = src[(size_t)i + (size_t)j * lds];
+  dst[(size_t)i + (size_t)j * ldd] = (scale == (T)1) ? v : scale * v;
 }
 
 hipError_t copy_block(int m, int n, const double *src, int lds, double *dst, int ldd, double scale,
                       hipStream_t st) {
   size_t tot = (size_t)m * n;
   if (!tot) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_block, dim3(cdiv(tot, 256)), dim3(256), 0, st, m, n, src, lds, dst, ldd,
-                     scale);
+  hipLaunchKernelGGL(k_copy_block<double>, dim3(cdiv(tot, 256)), dim3(256), 0, st, m, n, src, lds,
+                     dst, ldd, scale);
   return hipGetLastError();
 }
 

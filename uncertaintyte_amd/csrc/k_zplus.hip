@@ -1,0 +1,587 @@
+// k_zplus.hip -- kernels of the gauge-fixed Z^+ path (gfx950).
+//
+// Z_s has the 7-dimensional similarity gauge as its (numerical) null space:
+// fixPts drops the fixed points' rows together with their columns
+// (compute.cpp:337-364), so nothing pins the gauge.  The reference regularises
+// with lambda I and runs its Taylor scheme (compute.cpp:121-166); this path
+// computes the gauge-fixed pseudo-inverse Z^+ = Z_g^-1 - N N^T with
+// Z_g = Z_s + N N^T and N an orthonormal basis of the 7 smallest eigenvectors,
+// and only the parts the outputs need: the cameras' own 9x9 blocks.
+//
+//   1. Z~ = Z_s + N0 N0^T for a fixed pseudo-random orthonormal N0 (SPD for any
+//      N0 that is not orthogonal to the null space);
+//   2. Z~ = L L^T and X = L^-1 by a recursive blocked Cholesky whose leaves are
+//      64x64 register-resident factorisations and whose inner nodes are FP64
+//      MFMA GEMMs (pipeline.hip);
+//   3. N = orth(Z~^-1 N0) (exact when the null space is exact), then Newton
+//      steps N <- orth(N - P X^T X P (Z N - N N^T Z N)) with P = I - N N^T;
+//   4. Z_g^-1 = Z~^-1 - M U S U^T M  (Woodbury, U = [N N0], C = diag(I, -I),
+//      S = (C + U^T M U)^-1, M = X^T X), so camera block a of Z^+ is
+//      X_a^T X_a - (MU)_a S (MU)_a^T - N_a N_a^T with X_a the a-th block column.
+//
+// Skinny products with 7 or 14 right-hand columns stream X or Z once from HBM.
+#include "common.h"
+
+namespace unc {
+
+// ---------------------------------------------------------------------------
+// 64x64 leaf: Cholesky A = L L^T and X = L^-1, both lower triangular with zero
+// upper parts.  256 threads as a 16 x 16 grid, thread (ti, tj) keeps rows
+// ti + 16u and columns tj + 16v (u, v < 4) in registers; each pivot step
+// publishes one column through LDS.  A's lower triangle is read (the upper
+// one may hold stale values).
+// ---------------------------------------------------------------------------
+constexpr int CL_B = 64;
+
+__global__ __launch_bounds__(256) void k_chol_inv64(int b, double *A, int lda, double *Xo, int ldx,
+                                                    unsigned int *err) {
+  __shared__ double colp[CL_B], rowp[CL_B];
+  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+  double m[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      const int r = i >= j ? i : j, c = i >= j ? j : i;  // lower-triangle element
+      m[u][v] = (i < b && j < b) ? A[(size_t)r + (size_t)c * lda] : (i == j ? 1.0 : 0.0);
+    }
+  bool bad = false;
+  // Cholesky, right-looking: after step p column p holds L(:, p)
+  for (int p = 0; p < b; ++p) {
+    const int pu = p >> 4, pl = p & 15;
+    if (tj == pl) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
+        colp[ti + 16 * u] = x;
+      }
+    }
+    __syncthreads();
+    const double piv = colp[p];
+    if (!(piv > 0.0)) bad = true;
+    const double s = sqrt(piv), is = 1.0 / s;
+    double ci[4], cj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cj[v] = colp[tj + 16 * v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = ti + 16 * u;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = tj + 16 * v;
+        double x = m[u][v];
+        if (j == p) x = (i > p) ? ci[u] * is : (i == p ? s : 0.0);
+        else if (i == p) x = (j < p) ? x : 0.0;
+        else if (i > p && j > p) x = x - (ci[u] * cj[v]) / piv;
+        m[u][v] = x;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      if (i < b && j < b) A[(size_t)i + (size_t)j * lda] = m[u][v];
+    }
+  // in-register Gauss-Jordan inverse of the lower-triangular factor
+  for (int p = 0; p < b; ++p) {
+    const int pu = p >> 4, pl = p & 15;
+    if (tj == pl) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
+        colp[ti + 16 * u] = x;
+      }
+    }
+    if (ti == pl) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x = (u == pu) ? m[u][v] : x;
+        rowp[tj + 16 * v] = x;
+      }
+    }
+    __syncthreads();
+    const double ip = 1.0 / colp[p];
+    double ci[4], rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rj[v] = rowp[tj + 16 * v] * ip;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = ti + 16 * u;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = tj + 16 * v;
+        double x;
+        if (i == p && j == p) x = ip;
+        else if (i == p) x = rj[v];
+        else if (j == p) x = -ci[u] * ip;
+        else x = m[u][v] - ci[u] * rj[v];
+        m[u][v] = x;
+      }
+    }
+    __syncthreads();
+  }
+  if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      if (i < b && j < b) Xo[(size_t)i + (size_t)j * ldx] = (i >= j) ? m[u][v] : 0.0;
+    }
+}
+
+hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
+                         hipStream_t st) {
+  if (b < 1 || b > CL_B) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_chol_inv64, dim3(1), dim3(256), 0, st, b, A, lda, X, ldx, err);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// skinny products with KC right-hand columns (n x KC, column-major, ld n)
+// ---------------------------------------------------------------------------
+// C = A^T B.  A workgroup owns 16 output rows (columns of A), a wave 4 of
+// them; rows of B are staged through LDS 64 at a time.  lower: A is lower
+// triangular, so column i only has rows r >= i.
+constexpr int TN_COLS = 16;
+
+template <int KC>
+__global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restrict__ A, int lda,
+                                                   const double *__restrict__ B,
+                                                   double *__restrict__ C, int lower) {
+  __shared__ double Bs[64][KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i0 = blockIdx.x * TN_COLS;
+  const int ibase = i0 + 4 * wave;
+  double acc[4][KC];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[c][j] = 0.0;
+  const int rstart = lower ? (i0 & ~63) : 0;
+  for (int r0 = rstart; r0 < n; r0 += 64) {
+    __syncthreads();
+    for (int x = tid; x < 64 * KC; x += 256) {
+      const int rr = x & 63, j = x >> 6;
+      Bs[rr][j] = (r0 + rr < n) ? B[(size_t)(r0 + rr) + (size_t)j * n] : 0.0;
+    }
+    __syncthreads();
+    const int r = r0 + lane;
+    if (r < n) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = ibase + c;
+        if (i < n && (!lower || r >= i)) {
+          const double a = A[(size_t)r + (size_t)i * lda];
+#pragma unroll
+          for (int j = 0; j < KC; ++j) acc[c][j] = fma(a, Bs[lane][j], acc[c][j]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+      double v = acc[c][j];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      acc[c][j] = v;
+    }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = ibase + c;
+      if (i < n)
+#pragma unroll
+        for (int j = 0; j < KC; ++j) C[(size_t)i + (size_t)j * n] = acc[c][j];
+    }
+  }
+}
+
+// C = A B, split over the columns of A into NN_SPLIT partial sums (reduced by
+// k_skinny_reduce).  Thread = output row; B's rows for 64 columns of A at a
+// time are staged through LDS.  lower: row r only has columns c <= r.
+constexpr int NN_SPLIT = 16;
+
+template <int KC>
+__global__ __launch_bounds__(256) void k_skinny_nn(int n, const double *__restrict__ A, int lda,
+                                                   const double *__restrict__ B,
+                                                   double *__restrict__ part, int lower) {
+  __shared__ double Bs[64][KC];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * 256, r = r0 + tid;
+  const int per = (n + NN_SPLIT - 1) / NN_SPLIT;
+  const int cb = blockIdx.y * per;
+  int ce = min(n, cb + per);
+  if (lower) ce = min(ce, r0 + 256);
+  double acc[KC];
+#pragma unroll
+  for (int j = 0; j < KC; ++j) acc[j] = 0.0;
+  for (int c0 = cb; c0 < ce; c0 += 64) {
+    const int cnt = min(64, ce - c0);
+    __syncthreads();
+    for (int x = tid; x < 64 * KC; x += 256) {
+      const int cc = x & 63, j = x >> 6;
+      Bs[cc][j] = (cc < cnt) ? B[(size_t)(c0 + cc) + (size_t)j * n] : 0.0;
+    }
+    __syncthreads();
+    if (r < n) {
+      for (int cc = 0; cc < cnt; ++cc) {
+        const int c = c0 + cc;
+        const double a = (!lower || c <= r) ? A[(size_t)r + (size_t)c * lda] : 0.0;
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] = fma(a, Bs[cc][j], acc[j]);
+      }
+    }
+  }
+  if (r < n) {
+#pragma unroll
+    for (int j = 0; j < KC; ++j) part[((size_t)blockIdx.y * KC + j) * n + r] = acc[j];
+  }
+}
+
+template <int KC>
+__global__ void k_skinny_reduce(int n, const double *part, double *C) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+#pragma unroll
+  for (int j = 0; j < KC; ++j) {
+    double s = 0.0;
+    for (int q = 0; q < NN_SPLIT; ++q) s += part[((size_t)q * KC + j) * n + r];
+    C[(size_t)r + (size_t)j * n] = s;
+  }
+}
+
+template <int KC>
+static hipError_t skinny_tn_k(int n, const double *A, int lda, const double *B, double *C,
+                              bool lower, hipStream_t st) {
+  hipLaunchKernelGGL(k_skinny_tn<KC>, dim3(cdiv(n, TN_COLS)), dim3(256), 0, st, n, A, lda, B, C,
+                     lower ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <int KC>
+static hipError_t skinny_nn_k(int n, const double *A, int lda, const double *B, double *C,
+                              double *part, bool lower, hipStream_t st) {
+  hipLaunchKernelGGL(k_skinny_nn<KC>, dim3(cdiv(n, 256), NN_SPLIT), dim3(256), 0, st, n, A, lda, B,
+                     part, lower ? 1 : 0);
+  hipLaunchKernelGGL(k_skinny_reduce<KC>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, part, C);
+  return hipGetLastError();
+}
+
+hipError_t skinny_tn(int n, int kc, const double *A, int lda, const double *B, double *C,
+                     bool lower, hipStream_t st) {
+  if (kc == 7) return skinny_tn_k<7>(n, A, lda, B, C, lower, st);
+  if (kc == 14) return skinny_tn_k<14>(n, A, lda, B, C, lower, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t skinny_nn(int n, int kc, const double *A, int lda, const double *B, double *C,
+                     double *part, bool lower, hipStream_t st) {
+  if (kc == 7) return skinny_nn_k<7>(n, A, lda, B, C, part, lower, st);
+  if (kc == 14) return skinny_nn_k<14>(n, A, lda, B, C, part, lower, st);
+  return hipErrorInvalidValue;
+}
+
+size_t skinny_part_doubles(int n) { return (size_t)NN_SPLIT * 16 * (size_t)n; }
+
+// ---------------------------------------------------------------------------
+// small dense helpers on n x k blocks (k <= 16) and k x k matrices (row-major)
+// ---------------------------------------------------------------------------
+// G = V^T W (k1 x k2, row-major): 64 partial sums over contiguous row ranges,
+// then one block adds them in a fixed order.
+constexpr int GM_BLOCKS = 64, GM_ROWS = 64;
+
+__global__ __launch_bounds__(256) void k_gram_part(int n, int k1, const double *V, int k2,
+                                                   const double *W, double *partial) {
+  __shared__ double sv[GM_ROWS][33];
+  const int tid = threadIdx.x;
+  const int kk = k1 + k2;
+  const int per = (n + GM_BLOCKS - 1) / GM_BLOCKS;
+  const int r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+  const int a = tid / k2, b = tid % k2;
+  const bool act = tid < k1 * k2;
+  double s = 0.0;
+  for (int c0 = r0; c0 < r1; c0 += GM_ROWS) {
+    const int cnt = min(GM_ROWS, r1 - c0);
+    __syncthreads();
+    for (int x = tid; x < GM_ROWS * kk; x += 256) {
+      const int q = x / GM_ROWS, rr = x % GM_ROWS;
+      double v = 0.0;
+      if (rr < cnt) v = q < k1 ? V[(size_t)q * n + c0 + rr] : W[(size_t)(q - k1) * n + c0 + rr];
+      sv[rr][q] = v;
+    }
+    __syncthreads();
+    if (act)
+      for (int rr = 0; rr < cnt; ++rr) s = fma(sv[rr][a], sv[rr][k1 + b], s);
+  }
+  if (act) partial[(size_t)blockIdx.x * 256 + tid] = s;
+}
+
+__global__ void k_gram_sum(int cnt, const double *partial, double *G) {
+  const int e = threadIdx.x;
+  if (e >= cnt) return;
+  double s = 0.0;
+  for (int q = 0; q < GM_BLOCKS; ++q) s += partial[(size_t)q * 256 + e];
+  G[e] = s;
+}
+
+hipError_t gram(int n, int k1, const double *V, int k2, const double *W, double *partial,
+                double *G, hipStream_t st) {
+  if (k1 < 1 || k2 < 1 || k1 + k2 > 32 || k1 * k2 > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gram_part, dim3(GM_BLOCKS), dim3(256), 0, st, n, k1, V, k2, W, partial);
+  hipLaunchKernelGGL(k_gram_sum, dim3(1), dim3(256), 0, st, k1 * k2, partial, G);
+  return hipGetLastError();
+}
+
+// T = L^-T for G = L L^T (k x k, row-major): V T has orthonormal columns
+__global__ void k_small_orth(int k, const double *G, double *T, unsigned int *err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double L[16][16] = {}, Li[16][16] = {};
+  bool bad = false;
+  for (int j = 0; j < k; ++j) {
+    double s = G[j * k + j];
+    for (int q = 0; q < j; ++q) s -= L[j][q] * L[j][q];
+    if (!(s > 0.0)) {
+      bad = true;
+      s = 1e-300;
+    }
+    L[j][j] = sqrt(s);
+    for (int i = j + 1; i < k; ++i) {
+      double t = G[i * k + j];
+      for (int q = 0; q < j; ++q) t -= L[i][q] * L[j][q];
+      L[i][j] = t / L[j][j];
+    }
+  }
+  for (int j = 0; j < k; ++j) {
+    Li[j][j] = 1.0 / L[j][j];
+    for (int i = j + 1; i < k; ++i) {
+      double s = 0.0;
+      for (int q = j; q < i; ++q) s += L[i][q] * Li[q][j];
+      Li[i][j] = -s / L[i][i];
+    }
+  }
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) T[i * k + j] = Li[j][i];
+  if (bad) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
+// S = (C + G)^-1 with C = diag(I_h, -I_h), k = 2h (row-major); Gauss-Jordan
+// with partial pivoting
+__global__ void k_small_woodbury(int k, const double *G, double *S, unsigned int *err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[16][32];
+  const int h = k / 2;
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < 2 * k; ++j)
+      a[i][j] = j < k ? G[i * k + j] + (i == j ? (i < h ? 1.0 : -1.0) : 0.0) : (j - k == i ? 1.0 : 0.0);
+  bool bad = false;
+  for (int p = 0; p < k; ++p) {
+    int best = p;
+    for (int i = p + 1; i < k; ++i)
+      if (fabs(a[i][p]) > fabs(a[best][p])) best = i;
+    if (best != p)
+      for (int j = 0; j < 2 * k; ++j) {
+        double t = a[p][j];
+        a[p][j] = a[best][j];
+        a[best][j] = t;
+      }
+    if (!(fabs(a[p][p]) > 0.0)) {
+      bad = true;
+      break;
+    }
+    const double ip = 1.0 / a[p][p];
+    for (int j = 0; j < 2 * k; ++j) a[p][j] *= ip;
+    for (int i = 0; i < k; ++i) {
+      if (i == p) continue;
+      const double f = a[i][p];
+      if (f != 0.0)
+        for (int j = 0; j < 2 * k; ++j) a[i][j] -= f * a[p][j];
+    }
+  }
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) S[i * k + j] = 0.5 * (a[i][k + j] + a[j][k + i]);
+  if (bad) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
+// out(n x k2) = alpha A + beta B T (A n x k2, B n x k1, T k1 x k2 row-major,
+// T == nullptr: identity).  Row-local, so out may alias A or B.
+__global__ void k_small_apply(int n, int k1, int k2, double alpha, const double *A, double beta,
+                              const double *B, const double *T, double *out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double b[16], o[16];
+  for (int q = 0; q < k1; ++q) b[q] = B[(size_t)r + (size_t)q * n];
+  for (int j = 0; j < k2; ++j) {
+    double s = 0.0;
+    if (T) {
+      for (int q = 0; q < k1; ++q) s = fma(b[q], T[q * k2 + j], s);
+    } else {
+      s = b[j];
+    }
+    o[j] = (alpha != 0.0 ? alpha * A[(size_t)r + (size_t)j * n] : 0.0) + beta * s;
+  }
+  for (int j = 0; j < k2; ++j) out[(size_t)r + (size_t)j * n] = o[j];
+}
+
+hipError_t small_orth(int k, const double *G, double *T, unsigned int *err, hipStream_t st) {
+  if (k < 1 || k > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_orth, dim3(1), dim3(64), 0, st, k, G, T, err);
+  return hipGetLastError();
+}
+hipError_t small_woodbury(int k, const double *G, double *S, unsigned int *err, hipStream_t st) {
+  if (k < 2 || k > 16 || (k & 1)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_woodbury, dim3(1), dim3(64), 0, st, k, G, S, err);
+  return hipGetLastError();
+}
+hipError_t small_apply(int n, int k1, int k2, double alpha, const double *A, double beta,
+                       const double *B, const double *T, double *out, hipStream_t st) {
+  if (k1 < 1 || k1 > 16 || k2 < 1 || k2 > 16 || (!T && k1 != k2)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_apply, dim3(cdiv(n, 256)), dim3(256), 0, st, n, k1, k2, alpha, A, beta,
+                     B, T, out);
+  return hipGetLastError();
+}
+
+// deterministic pseudo-random fill in [-0.5, 0.5)
+__global__ void k_fill_hash(size_t cnt, double *V, unsigned long long seed) {
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < cnt;
+       x += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long h = (x + 1 + seed) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 27;
+    V[x] = (double)(h >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+}
+
+hipError_t fill_hash(size_t cnt, double *V, unsigned long long seed, hipStream_t st) {
+  size_t g = cdiv(cnt, 256);
+  hipLaunchKernelGGL(k_fill_hash, dim3((unsigned int)(g > 4096 ? 4096 : g)), dim3(256), 0, st, cnt,
+                     V, seed);
+  return hipGetLastError();
+}
+
+// out = A + V V^T (n x n; V n x k column-major)
+__global__ void k_rank_copy(int n, int k, const double *A, const double *V, double *out) {
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < (size_t)n * n;
+       x += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(x % n), c = (int)(x / n);
+    double s = 0.0;
+    for (int q = 0; q < k; ++q) s = fma(V[r + (size_t)q * n], V[c + (size_t)q * n], s);
+    out[x] = A[x] + s;
+  }
+}
+
+hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st) {
+  size_t g = cdiv((size_t)n * n, 256);
+  hipLaunchKernelGGL(k_rank_copy, dim3((unsigned int)(g > 16384 ? 16384 : g)), dim3(256), 0, st, n,
+                     k, A, V, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// camera blocks of Z^+: Xd[a] = X_a^T X_a - (MU)_a S (MU)_a^T - N_a N_a^T,
+// X_a = block column a of the lower-triangular X (rows >= a cp), row-major
+// cp x cp output.  One workgroup per camera.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cam_gram(int nc, int cp, int n, const double *X,
+                                                  const double *MU, const double *S,
+                                                  const double *Nb, double *Xd) {
+  // accumulators follow the 9x9 upper triangle (entry (p, q), p <= q, has the
+  // compile-time index p*9 - p(p-1)/2 + q - p); parameters beyond cp stay 0
+  constexpr int MAXE = 45;
+  __shared__ double red[4][MAXE];
+  __shared__ double g[MAXE];
+  const int a = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = a * cp;
+  double acc[MAXE];
+#pragma unroll
+  for (int e = 0; e < MAXE; ++e) acc[e] = 0.0;
+  for (int r = c0 + tid; r < n; r += 256) {
+    double x[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) x[p] = p < cp ? X[(size_t)r + (size_t)(c0 + p) * n] : 0.0;
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int q = p; q < 9; ++q) {
+        const int e = p * 9 - p * (p - 1) / 2 + (q - p);
+        acc[e] = fma(x[p], x[q], acc[e]);
+      }
+  }
+#pragma unroll
+  for (int e = 0; e < MAXE; ++e) {
+    double v = acc[e];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[wave][e] = v;
+  }
+  __syncthreads();
+  if (tid < MAXE) g[tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+  __syncthreads();
+  if (tid < MAXE) {
+    int r = tid, p = 0;
+    while (r >= 9 - p) {
+      r -= 9 - p;
+      ++p;
+    }
+    const int q = p + r;
+    if (q < cp) {
+      double w = 0.0;
+      for (int u = 0; u < 14; ++u) {
+        double t = 0.0;
+        for (int v = 0; v < 14; ++v) t = fma(S[u * 14 + v], MU[(size_t)(c0 + q) + (size_t)v * n], t);
+        w = fma(MU[(size_t)(c0 + p) + (size_t)u * n], t, w);
+      }
+      double nn = 0.0;
+      for (int u = 0; u < 7; ++u)
+        nn = fma(Nb[(size_t)(c0 + p) + (size_t)u * n], Nb[(size_t)(c0 + q) + (size_t)u * n], nn);
+      const double val = (g[tid] - w) - nn;
+      Xd[(size_t)a * cp * cp + p * cp + q] = val;
+      Xd[(size_t)a * cp * cp + q * cp + p] = val;
+    }
+  }
+}
+
+hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const double *S,
+                    const double *Nb, double *Xd, hipStream_t st) {
+  if (s.cp > 9) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cam_gram, dim3(s.nc), dim3(256), 0, st, s.nc, s.cp, s.N, X, MU, S, Nb, Xd);
+  return hipGetLastError();
+}
+
+// covariance output of the Z^+ path: upper triangle of D Z^+ D per camera
+__global__ void k_cam_output_cov(int nc, int cp, const double *Xd, const double *d, double *out) {
+  const int a = blockIdx.x;
+  const int per = cp * (cp + 1) / 2;
+  for (int l = threadIdx.x; l < per; l += blockDim.x) {
+    int r = l, j = 0;
+    while (r >= cp - j) {
+      r -= cp - j;
+      ++j;
+    }
+    const int k = j + r;
+    out[(size_t)a * per + l] = (Xd[(size_t)a * cp * cp + j * cp + k] * d[a * cp + j]) * d[a * cp + k];
+  }
+}
+
+hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_cam_output_cov, dim3(s.nc), dim3(64), 0, st, s.nc, s.cp, Xd, d, out);
+  return hipGetLastError();
+}
+
+}  // namespace unc

@@ -42,19 +42,20 @@ struct Sizes {
   size_t nn;
 };
 
-struct Layout {
-  size_t off[32];
-  size_t total;
-};
-
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
   B_ERR, B_SCAN, B_COVIS, B_KCNT, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
-  B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
+  B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
+};
+
+struct Layout {
+  size_t off[B_COUNT];
+  size_t total;
 };
 
 constexpr int GJ_BLOCK = 64;
+size_t zplus_work_doubles(int n);
 
 Layout make_layout(const Sizes &z, bool refine) {
   size_t sz[B_COUNT];
@@ -76,7 +77,9 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_P] = sizeof(double) * GJ_BLOCK * GJ_BLOCK;
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
   sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
-  sz[B_DE] = refine ? sizeof(double) * z.nn : 0;
+  sz[B_DE] = sizeof(double) * z.nn;
+  sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
+  (void)refine;
   Layout L;
   size_t o = 0;
   for (int i = 0; i < B_COUNT; ++i) {
@@ -103,8 +106,9 @@ int fact32(int n) {  // compute.cpp:57-62, with the int32 wrap made explicit
 int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[3]) {
   if (!o) return UNC_ERR_BAD_INPUT;
   if (o->algorithm != UNC_ALG_TAYLOR_EXPANSION) return UNC_ERR_UNSUPPORTED;
-  if (o->precision != UNC_PREC_FP64 && o->precision != UNC_PREC_MIXED) return UNC_ERR_BAD_INPUT;
-  if (o->precision == UNC_PREC_MIXED) return UNC_ERR_UNSUPPORTED;
+  if (o->precision != UNC_PREC_FP64 && o->precision != UNC_PREC_MIXED &&
+      o->precision != UNC_PREC_ZPLUS_FP64)
+    return UNC_ERR_BAD_INPUT;
   if (o->cam_params < 1 || o->cam_params > 9) return UNC_ERR_UNSUPPORTED;
   if (o->num_cams <= 0 || o->num_points < 4) return UNC_ERR_BAD_INPUT;
   const long long N = (long long)o->num_cams * o->cam_params;
@@ -184,6 +188,99 @@ int gj_inverse(int n, double *W, double *P, double *T, double *R, unsigned int *
   return UNC_OK;
 }
 
+// Recursive Cholesky of the SPD block A[off.., off..] (n x n, ld N) with the
+// inverse of its factor: L overwrites A's lower triangle, X = L^-1 goes to the
+// same block of X.  With A = [A11 .; A21 A22]:
+//   (L11, X11) = chol_inv(A11);  L21 = A21 X11^T;  A22 -= L21 L21^T;
+//   (L22, X22) = chol_inv(A22);  X21 = -X22 (L21 X11).
+// W is scratch of (n/2)^2 doubles at least.  Leaves are 64 x 64.
+int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned int *err,
+             hipStream_t s) {
+  if (n <= 64) {
+    HC(chol_inv_leaf(n, A + off + (size_t)off * N, N, X + off + (size_t)off * N, N, err, s));
+    return UNC_OK;
+  }
+  const int n1 = ((n / 2 + 63) / 64) * 64, n2 = n - n1;
+  int rc = chol_inv(N, A, X, W, off, n1, err, s);
+  if (rc) return rc;
+  double *A21 = A + (off + n1) + (size_t)off * N;
+  double *A22 = A + (off + n1) + (size_t)(off + n1) * N;
+  const double *X11 = X + off + (size_t)off * N;
+  const double *X22 = X + (off + n1) + (size_t)(off + n1) * N;
+  double *X21 = X + (off + n1) + (size_t)off * N;
+  HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, s));
+  HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
+  HC(gemm_f64_ex(n2, n2, n1, -1.0, A21, N, A21, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
+  rc = chol_inv(N, A, X, W, off + n1, n2, err, s);
+  if (rc) return rc;
+  HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, false, 0.0, W, n2, GEMM_B_LOWER, s));
+  HC(gemm_f64_ex(n2, n1, n2, -1.0, X22, N, W, n2, false, 0.0, X21, N, GEMM_A_LOWER, s));
+  return UNC_OK;
+}
+
+// Work area of the Z^+ path inside B_NULL (doubles, n = N):
+//   U n x 14 ([N | N0]), P n x 14, MU n x 14, Y n x 7, R n x 7, Q n x 7,
+//   skinny partials, gram partials, small matrices.
+constexpr int GAUGE = 7;
+size_t zplus_work_doubles(int n) {
+  return (size_t)(3 * 2 * GAUGE + 3 * GAUGE) * n + skinny_part_doubles(n) + 64 * 256 + 4096;
+}
+
+// Gauge-fixed Z^+: Z in dA on entry (kept), dB/dC/dD are N x N scratch.  Leaves
+// the cameras' own blocks of Z^+ in Xd (row-major cp x cp per camera).
+int zplus(int N, int newton_steps, double *dA, double *dB, double *dC, double *dD, double *work,
+          const Structure &S, double *Xd, unsigned int *err, hipStream_t s) {
+  const size_t nn = (size_t)N * N;
+  const int k = GAUGE;
+  double *U = work, *Pm = U + 2 * k * (size_t)N, *MU = Pm + 2 * k * (size_t)N;
+  double *Y = MU + 2 * k * (size_t)N, *R = Y + k * (size_t)N, *Q = R + k * (size_t)N;
+  double *part = Q + k * (size_t)N;
+  double *gpart = part + skinny_part_doubles(N);
+  double *G = gpart + 64 * 256, *T = G + 256, *Sw = T + 256, *Th = Sw + 256;
+  double *Nk = U, *N0 = U + k * (size_t)N;
+  // N0: fixed pseudo-random orthonormal block
+  HC(fill_hash((size_t)k * N, N0, 0x5eedull, s));
+  HC(gram(N, k, N0, k, N0, gpart, G, s));
+  HC(small_orth(k, G, T, err, s));
+  HC(small_apply(N, k, k, 0.0, nullptr, 1.0, N0, T, N0, s));
+  // Z~ = Z + N0 N0^T, Cholesky with the inverse factor X (upper part zero)
+  HC(rank_copy(N, k, dA, N0, dB, s));
+  HC(hipMemsetAsync(dC, 0, sizeof(double) * nn, s));
+  int rc = chol_inv(N, dB, dC, dD, 0, N, err, s);
+  if (rc) return rc;
+  const double *X = dC;
+  // N = orth(Z~^-1 N0) = orth(X^T X N0)
+  HC(skinny_nn(N, k, X, N, N0, R, part, true, s));
+  HC(skinny_tn(N, k, X, N, R, Y, true, s));
+  HC(gram(N, k, Y, k, Y, gpart, G, s));
+  HC(small_orth(k, G, T, err, s));
+  HC(small_apply(N, k, k, 0.0, nullptr, 1.0, Y, T, Nk, s));
+  // Newton steps on the invariant subspace: R = Z N - N (N^T Z N), projected;
+  // correction P X^T X P R
+  for (int it = 0; it < newton_steps; ++it) {
+    HC(skinny_tn(N, k, dA, N, Nk, R, false, s));            // Z N (Z symmetric)
+    HC(gram(N, k, Nk, k, R, gpart, Th, s));                  // N^T Z N
+    HC(small_apply(N, k, k, 1.0, R, -1.0, Nk, Th, R, s));    // R = ZN - N Th (already _|_ N)
+    HC(gram(N, k, Nk, k, R, gpart, G, s));
+    HC(small_apply(N, k, k, 1.0, R, -1.0, Nk, G, R, s));     // re-project
+    HC(skinny_nn(N, k, X, N, R, Q, part, true, s));
+    HC(skinny_tn(N, k, X, N, Q, Y, true, s));                // M P R
+    HC(gram(N, k, Nk, k, Y, gpart, G, s));
+    HC(small_apply(N, k, k, 1.0, Y, -1.0, Nk, G, Y, s));     // P M P R
+    HC(small_apply(N, k, k, 1.0, Nk, -1.0, Y, nullptr, Y, s));  // N - correction
+    HC(gram(N, k, Y, k, Y, gpart, G, s));
+    HC(small_orth(k, G, T, err, s));
+    HC(small_apply(N, k, k, 0.0, nullptr, 1.0, Y, T, Nk, s));
+  }
+  // Woodbury: U = [N N0], P = X U, MU = X^T P, S = (C + P^T P)^-1
+  HC(skinny_nn(N, 2 * k, X, N, U, Pm, part, true, s));
+  HC(skinny_tn(N, 2 * k, X, N, Pm, MU, true, s));
+  HC(gram(N, 2 * k, Pm, 2 * k, Pm, gpart, G, s));
+  HC(small_woodbury(2 * k, G, Sw, err, s));
+  HC(cam_gram(S, X, MU, Sw, Nk, Xd, s));
+  return UNC_OK;
+}
+
 int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
         const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
         unc_debug *dbg, hipStream_t s, bool host_api_upload_timed) {
@@ -251,6 +348,54 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   HC(launch_schur(S, Bk, dA, covis, kcnt, kmax, s, ctx->max_lds));
   HC(hipEventRecord(E[6], s));
   if (dbg && dbg->Z) HC(hipMemcpyAsync(dbg->Z, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  if (o->precision != UNC_PREC_FP64) {
+    // gauge-fixed Z^+ path
+    const int steps = o->refine >= 0 ? o->refine : 1;
+    int null_iters = steps;
+    double *Xd = (double *)ptr(B_XD);
+    rc = zplus(N, steps, dA, dB, dC, dD, (double *)ptr(B_NULL), S, Xd, S.err, s);
+    if (rc) return rc;
+    HC(hipEventRecord(E[7], s));
+    HC(hipEventRecord(E[8], s));
+    HC(cam_output_cov(S, Xd, Bk.d, d_cam, s));
+    HC(hipEventRecord(E[9], s));
+    HC(hipMemsetAsync(d_pt, 0, sizeof(double) * 6 * (size_t)z.np, s));
+    HC(launch_point_output(S, Bk, Xd, d_pt, z.np, s));
+    HC(hipEventRecord(E[10], s));
+    if (dbg) {
+      if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
+      if (dbg->Vinv_s) HC(hipMemcpyAsync(dbg->Vinv_s, Bk.Vi, sizeof(double) * 9 * (size_t)z.npc, hipMemcpyDeviceToHost, s));
+      if (dbg->d) HC(hipMemcpyAsync(dbg->d, Bk.d, sizeof(double) * ((size_t)N + 3 * (size_t)z.npc), hipMemcpyDeviceToHost, s));
+      if (dbg->X) HC(hipMemcpyAsync(dbg->X, dD, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+    }
+    HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    if (herr) return map_err(herr);
+    if (st) {
+      std::memset(st, 0, sizeof(*st));
+      st->lambda = 0.0;
+      st->fixed_pts[0] = fix[0]; st->fixed_pts[1] = fix[1]; st->fixed_pts[2] = fix[2];
+      st->num_terms = 0;
+      st->stage_ms[1] = ev_ms(E[1], E[2]);
+      st->stage_ms[2] = ev_ms(E[2], E[3]);
+      st->stage_ms[3] = ev_ms(E[3], E[4]);
+      st->stage_ms[5] = ev_ms(E[5], E[6]);
+      st->stage_ms[6] = ev_ms(E[6], E[7]);
+      st->stage_ms[8] = ev_ms(E[8], E[9]);
+      st->stage_ms[9] = ev_ms(E[9], E[10]);
+      st->stage_ms[11] = ev_ms(E[host_api_upload_timed ? 0 : 1], E[10]);
+      if (host_api_upload_timed) st->stage_ms[0] = ev_ms(E[0], E[1]);
+      st->time_create_j = st->stage_ms[0] * 1e-3;
+      st->time_fix_j = (st->stage_ms[1] + st->stage_ms[2]) * 1e-3;
+      st->time_multiply_jj = st->stage_ms[3] * 1e-3;
+      st->time_compose_z = st->stage_ms[5] * 1e-3;
+      st->time_inv_z = st->stage_ms[6] * 1e-3;
+      st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
+      st->time_all = st->stage_ms[11] * 1e-3;
+      st->num_obs_kept = null_iters;
+    }
+    return UNC_OK;
+  }
   // ---- (d) A = Z + lambda I ; X0 = A^-1
   const double lambda = (o->lambda == -1) ? pow(10, -1.2653 * log10((double)N) - 2.9415) : o->lambda;
   HC(add_diag(N, dA, N, lambda, s));
@@ -440,7 +585,7 @@ void unc_default_options(unc_options *o) {
   o->max_iter_te = -1;
   o->pts2fix[0] = o->pts2fix[1] = o->pts2fix[2] = -1;
   o->precision = UNC_PREC_FP64;
-  o->refine = 1;
+  o->refine = -1;
   o->device = 0;
 }
 
@@ -644,6 +789,33 @@ int unc_diag_inverse(int n, const double *A, double *X, int refine) {
   }
   unsigned int herr = 0;
   HC(hipMemcpyAsync(X, res, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  return herr ? map_err(herr) : UNC_OK;
+}
+
+int unc_diag_chol_inverse(int n, const double *A, double *L, double *X) {
+  if (n <= 0 || !A || !L || !X) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n;
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * nn + 256));
+  double *dA = (double *)buf, *dX = dA + nn, *dW = dX + nn;
+  unsigned int *err = (unsigned int *)(dW + nn);
+  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  HC(hipMemsetAsync(dX, 0, sizeof(double) * nn, s));
+  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  rc = chol_inv(n, dA, dX, dW, 0, n, err, s);
+  if (rc) { (void)hipFree(buf); return rc; }
+  unsigned int herr = 0;
+  HC(hipMemcpyAsync(L, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(X, dX, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
   HC(hipFree(buf));
