@@ -26,6 +26,8 @@
 #ifndef UNCERTAINTY_AMD_H
 #define UNCERTAINTY_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -171,6 +173,44 @@ const char *unc_stage_name(int stage);
 int unc_abi_version(void);
 /* Largest LDS allocation of one workgroup on the device (bytes), -1 on error. */
 int unc_device_lds_bytes(int device);
+
+/* ---- request server: many reconstructions of different sizes -----------------
+ * No counterpart in the reference (which handles one problem per call).
+ * unc_server_submit queues a request and returns a ticket at once; the host
+ * buffers must stay valid until unc_server_wait returns for that ticket.  Slots
+ * (each a context with its own HIP stream) run admitted requests concurrently; a
+ * request is admitted when a slot is idle and the device bytes all slots then
+ * hold stay within hbm_budget (see unc_workspace_bytes).  FIFO, with a later
+ * request allowed to pass a head that does not fit yet for at most bypass_ms.
+ * Results are those of unc_compute on the same inputs. */
+typedef struct unc_server unc_server;
+typedef struct unc_server_config {
+  int device;          /* HIP device ordinal                                          */
+  int slots;           /* requests in flight together, 1..64                          */
+  size_t hbm_budget;   /* device bytes all slots may hold together                    */
+  double bypass_ms;    /* how long later requests may pass a head that does not fit   */
+  int flags;           /* reserved, 0                                                 */
+} unc_server_config;
+typedef struct unc_server_stats {
+  long long queued, completed, admitted_out_of_order;
+  size_t held_bytes, peak_held_bytes;
+  int max_concurrent;
+} unc_server_stats;
+/* defaults: device 0, 4 slots, 64 GiB, 50 ms */
+void unc_server_default_config(unc_server_config *cfg);
+int unc_server_create(const unc_server_config *cfg, unc_server **out);
+/* UNC_ERR_OOM when the request alone exceeds hbm_budget */
+int unc_server_submit(unc_server *srv, const unc_options *opts, int num_rows, int num_cols,
+                      const int *rows, const int *cols, const double *values,
+                      double *cam_unc, double *pt_unc, unc_stats *stats,
+                      long long *ticket);
+/* blocks until the ticket's request finished; status = its unc_compute code,
+ * latency = submit to finish, queue = submit to start (milliseconds) */
+int unc_server_wait(unc_server *srv, long long ticket, int *status, double *latency_ms,
+                    double *queue_ms);
+int unc_server_query(unc_server *srv, unc_server_stats *out);
+/* finishes every queued request, then frees the slots */
+void unc_server_destroy(unc_server *srv);
 
 #ifdef __cplusplus
 }

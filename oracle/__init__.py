@@ -69,6 +69,9 @@ def lib():
         L.orc_inv_equilibrated.argtypes = [ctypes.c_int, _f64p, _f64p]
         L.orc_residual_dd.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p]
         L.orc_gemm.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_front.argtypes = [ctypes.POINTER(Opts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                _i32p, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p,
+                                _f64p]
         L.orc_factorial32.argtypes = [ctypes.c_int]
         L.orc_factorial32.restype = ctypes.c_int
         L.orc_auto_lambda.argtypes = [ctypes.c_int]
@@ -208,14 +211,31 @@ def auto_lambda(N: int) -> float:
     return lib().orc_auto_lambda(N)
 
 
+def front(jac, fix=None, nthreads: int = 0):
+    """fixPts + assembly + Schur only: (Z [row, col], d) -- the scaled Schur
+    complement the TE path inverts and the column scales."""
+    nc, cp, npts = jac.num_cams, jac.cam_params, jac.num_points
+    N = nc * cp
+    Z = np.zeros(N * N)
+    d = np.zeros(N + 3 * (npts - 3))
+    fx = np.array(fix if fix is not None else jac.fix_array(), dtype=np.int32)
+    o = Opts(INV_REFINED, nthreads, -1.0)
+    rc = lib().orc_front(ctypes.byref(o), nc, cp, npts, fx, jac.num_rows, jac.num_cols,
+                         np.ascontiguousarray(jac.rows, np.int32),
+                         np.ascontiguousarray(jac.cols, np.int32),
+                         np.ascontiguousarray(jac.vals, np.float64), Z, d)
+    if rc:
+        raise OracleError(rc, "orc_front")
+    return Z.reshape(N, N).T, d
+
+
 def zplus_reference(jac, fix=None, nthreads: int = 0):
-    """FP64 gauge-fixed covariance: X = pinv_7(Z_s) (eigen-decomposition of the
-    oracle's Schur complement, the 7 smallest eigenvalues dropped), camera
+    """FP64 gauge-fixed covariance: X = pinv_7(Z_s) (LAPACK eigen-decomposition
+    of the oracle's Schur complement, the 7 smallest eigenvalues dropped), camera
     output = upper triangles of D X D per camera, points propagated from X's
     diagonal blocks with the same formula as the TAYLOR path.
     Returns dict(cam, pt, X, eig) -- the reference for the Z^+ precision modes."""
-    r = pipeline(jac, nthreads=nthreads, fix=fix, debug=True)
-    Z, d = r["Z"], r["d"]
+    Z, d = front(jac, fix=fix, nthreads=nthreads)
     w, Q = np.linalg.eigh(Z)
     inv = np.zeros_like(w)
     inv[7:] = 1.0 / w[7:]
@@ -237,3 +257,40 @@ def zplus_reference(jac, fix=None, nthreads: int = 0):
     if rc:
         raise OracleError(rc, "orc_points_from_X")
     return dict(cam=cam, pt=pt.reshape(-1, 6), X=X, eig=w)
+
+
+def zplus_sampled(jac, cams, fix=None, nthreads: int = 0):
+    """Camera outputs of D pinv_7(Z_s) D for the listed cameras only, for scenes
+    too large for a dense eigen-decomposition.  Uses the identity
+    pinv_7(Z) = (Z + N N^T)^-1 - N N^T with N the eigenvectors of the 7 smallest
+    eigenvalues, found by shift-invert Lanczos (ARPACK) on a LAPACK Cholesky of
+    Z + sigma I; the sampled columns come from a Cholesky solve with Z + N N^T.
+    Returns (cam rows for `cams`, the 7 smallest eigenvalues)."""
+    import scipy.linalg as sla
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    Z, d = front(jac, fix=fix, nthreads=nthreads)
+    n = Z.shape[0]
+    cp = jac.cam_params
+    sigma = 1e-10 * np.trace(Z) / n
+    A = np.array(Z, order="F")
+    A[np.diag_indices(n)] += sigma
+    cf = sla.cho_factor(A, lower=True, overwrite_a=True, check_finite=False)
+    op = LinearOperator((n, n), matvec=lambda v: sla.cho_solve(cf, v, check_finite=False),
+                        dtype=np.float64)
+    w, Nb = eigsh(Z, k=7, sigma=-sigma, OPinv=op, which="LM", tol=1e-14,
+                  v0=np.linspace(1.0, 2.0, n))
+    del cf, A
+    G = np.array(Z, order="F")
+    G += Nb @ Nb.T
+    cg = sla.cho_factor(G, lower=True, overwrite_a=True, check_finite=False)
+    idx = np.concatenate([np.arange(a * cp, (a + 1) * cp) for a in cams])
+    E = np.zeros((n, len(idx)))
+    E[idx, np.arange(len(idx))] = 1.0
+    cols = sla.cho_solve(cg, E, check_finite=False) - Nb @ Nb[idx].T
+    iu = np.triu_indices(cp)
+    out = np.empty((len(cams), cp * (cp + 1) // 2))
+    for q, a in enumerate(cams):
+        s = slice(a * cp, (a + 1) * cp)
+        B = cols[s, q * cp:(q + 1) * cp] * d[s][:, None] * d[s][None, :]
+        out[q] = (0.5 * (B + B.T))[iu]
+    return out, np.sort(w)

@@ -598,6 +598,37 @@ int orc_pipeline(const orc_opts *opts, int ncams, int cp, int npts,
   return rc;
 }
 
+int orc_front(const orc_opts *opts, int ncams, int cp, int npts, const int fix_in[3],
+              int nrows, int ncols, const int *rows, const int *cols, const double *vals,
+              double *Z_out, double *d_out) {
+  set_threads(opts);
+  int rc = validate_original(ncams, cp, npts, nrows, ncols, rows);
+  if (rc) return rc;
+  if (!fix_in || fix_in[0] < 0 || fix_in[1] < 0 || fix_in[2] < 0) return ORC_ERR_NO_FIXED_POINTS;
+  int fix[3] = {fix_in[0], fix_in[1], fix_in[2]};
+  for (int i = 0; i < 3; ++i)
+    if (fix[i] >= npts) return ORC_ERR_BAD_INPUT;
+  if (fix[0] == fix[1] || fix[1] == fix[2] || fix[0] == fix[2]) return ORC_ERR_BAD_INPUT;
+  int nnz = rows[nrows];
+  int *r2 = (int *)malloc(sizeof(int) * (size_t)(nrows + 1));
+  int *c2 = (int *)malloc(sizeof(int) * (size_t)nnz);
+  double *v2 = (double *)malloc(sizeof(double) * (size_t)nnz);
+  if (!r2 || !c2 || !v2) { free(r2); free(c2); free(v2); return ORC_ERR_OOM; }
+  int nr2, nc2;
+  rc = orc_fixpts(nrows, ncols, rows, cols, vals, cp, ncams, fix, &nr2, &nc2, r2, c2, v2);
+  if (rc) { free(r2); free(c2); free(v2); return rc; }
+  jstruct s;
+  rc = jstruct_build(&s, cp, ncams, npts - 3, nr2, c2, v2);
+  if (rc) { free(r2); free(c2); free(v2); return rc; }
+  blocks bl;
+  rc = assemble(&s, &bl);
+  if (rc) { jstruct_free(&s); free(r2); free(c2); free(v2); return rc; }
+  schur(&s, &bl, Z_out);
+  if (d_out) memcpy(d_out, bl.d, sizeof(double) * (size_t)(s.N + 3 * s.npc));
+  blocks_free(&bl); jstruct_free(&s); free(r2); free(c2); free(v2);
+  return ORC_OK;
+}
+
 int orc_points_from_X(int ncams, int cp, int npts, const int fix_in[3],
                       int nrows_fixed, const int *cols_fixed,
                       const double *vals_fixed, const double *X, double *pt_out) {

@@ -86,6 +86,12 @@ int orc_pipeline(const orc_opts *opts, int ncams, int cp, int npts,
                  const int *cols, const double *vals, double *cam_out,
                  double *pt_out, orc_stats *stats, orc_debug *dbg);
 
+/* Front half of the pipeline only (fixPts, assembly, Schur): the scaled Schur
+ * complement Z (column-major N*N) and the column scales d (N + 3*npc). */
+int orc_front(const orc_opts *opts, int ncams, int cp, int npts, const int fix_in[3],
+              int nrows, int ncols, const int *rows, const int *cols, const double *vals,
+              double *Z_out, double *d_out);
+
 /* Taylor chain alone on a given scaled Z (column-major N*N); X out. */
 int orc_taylor(const orc_opts *opts, int N, const double *Z, double lambda,
                double *X0_out, double *X_out, orc_stats *stats);

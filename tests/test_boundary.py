@@ -94,6 +94,19 @@ def test_no_gpu_is_a_clean_error():
     assert e.value.code == u.UNC_ERR_DEVICE
 
 
+def test_server_without_gpu_is_a_clean_error():
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(u.UncError) as e:
+        u.Server(slots=2)
+    assert e.value.code == u.UNC_ERR_DEVICE
+
+
 def test_jacob_roundtrip(tmp_path, cube):
     p = tmp_path / "c.jacob"
     jacob.write_jacob(str(p), cube)

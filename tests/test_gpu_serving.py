@@ -1,0 +1,64 @@
+"""GPU: the request server -- results in a concurrent mix equal the results of
+the same requests alone (bit for bit: same kernels, deterministic reductions),
+HBM-footprint admission stays within its budget, oversize requests are refused."""
+import numpy as np
+import pytest
+
+import uncertaintyte_amd as u
+from uncertaintyte_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_scenes():
+    return [serve.request_scene(c, p, 100 + i)
+            for i, (c, p) in enumerate([(20, 40), (35, 60), (24, 120), (60, 30), (41, 80),
+                                        (22, 200), (80, 25), (30, 90)])]
+
+
+@pytest.mark.parametrize("prec", [u.PREC_MIXED, u.PREC_FP64])
+def test_mix_equals_alone(small_scenes, prec):
+    print(f"serving mix vs alone precision={prec}", flush=True)
+    alone = [u.compute_covariances(j, precision=prec) for j in small_scenes]
+    srv = u.Server(slots=4)
+    tickets = [srv.submit(j, precision=prec) for j in small_scenes for _ in range(2)]
+    outs = [srv.wait(t)[0] for t in tickets]
+    st = srv.stats()
+    srv.close()
+    for k, r in enumerate(outs):
+        a = alone[k // 2]
+        assert np.array_equal(r.cam, a.cam) and np.array_equal(r.pt, a.pt)
+    assert st.completed == len(tickets) and st.max_concurrent >= 2
+
+
+def test_admission_respects_budget(small_scenes):
+    fp = [u.workspace_bytes(u.make_options(j, u.PREC_MIXED), j.num_rows) for j in small_scenes]
+    budget = int(1.6 * max(fp)) + (64 << 20)
+    print(f"admission budget {budget}", flush=True)
+    srv = u.Server(slots=4, hbm_budget=budget, bypass_ms=5.0)
+    tickets = [srv.submit(j, precision=u.PREC_MIXED) for j in small_scenes]
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    srv.close()
+    assert st.completed == len(small_scenes)
+    assert 0 < st.peak_held_bytes <= budget
+
+
+def test_oversize_request_refused(small_scenes):
+    srv = u.Server(slots=2, hbm_budget=1 << 20)
+    with pytest.raises(u.UncError) as e:
+        srv.submit(small_scenes[0], precision=u.PREC_MIXED)
+    srv.close()
+    assert e.value.code == u.UNC_ERR_OOM
+
+
+def test_poisson_load_smoke(small_scenes):
+    srv = u.Server(slots=4)
+    order = np.arange(16) % len(small_scenes)
+    res = serve.run_load(srv, small_scenes, order, rate=200.0, precision=u.PREC_MIXED)
+    srv.close()
+    s = res.summary()
+    print(s, flush=True)
+    assert s["requests"] == 16 and s["p99_ms"] >= s["p50_ms"] > 0

@@ -93,3 +93,17 @@ def test_zplus_repeatable_and_context_reuse(daliborka):
     assert np.array_equal(a.cam, c.cam) and np.array_equal(a.pt, c.pt)
     assert b.cam.shape[0] == 10
     ctx.close()
+
+
+def test_zplus_at_scale_sampled_cameras():
+    """walk1500 (N = 13491): 188 sampled cameras against the ARPACK + LAPACK
+    Cholesky construction of D pinv_7(Z_s) D (oracle.zplus_sampled)."""
+    j = scene.make_workload("walk1500")
+    print("zplus walk1500: GPU", flush=True)
+    r = u.compute_covariances(j, precision=u.PREC_MIXED)
+    cams = list(range(0, j.num_cams, 8))
+    print("zplus walk1500: sampled oracle", flush=True)
+    ref, w = oracle.zplus_sampled(j, cams, nthreads=16)
+    rc = block_rel(r.cam[cams], ref)
+    print(f"  walk1500: cam max {rc.max():.2e} median {np.median(rc):.2e}", flush=True)
+    assert rc.max() <= 1e-5

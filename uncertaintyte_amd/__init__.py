@@ -114,6 +114,17 @@ def lib():
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
     L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
+    L.unc_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.unc_server_default_config.restype = None
+    L.unc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+    L.unc_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_int,
+                                    ctypes.c_int, vp, vp, vp, vp, vp, ctypes.POINTER(Stats),
+                                    ctypes.POINTER(ctypes.c_longlong)]
+    L.unc_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.unc_server_query.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+    L.unc_server_destroy.argtypes = [ctypes.c_void_p]
+    L.unc_server_destroy.restype = None
     L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
@@ -128,6 +139,17 @@ def error_string(code: int) -> str:
 def _check(code: int, what: str):
     if code != UNC_OK:
         raise UncError(code, f"{what}: {error_string(code)}")
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("bypass_ms", ctypes.c_double), ("flags", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("admitted_out_of_order", ctypes.c_longlong), ("held_bytes", ctypes.c_size_t),
+                ("peak_held_bytes", ctypes.c_size_t), ("max_concurrent", ctypes.c_int)]
 
 
 def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
@@ -332,3 +354,67 @@ class Uncertainty:
     def getPointsUncEigenValues(self) -> np.ndarray:
         ev = np.linalg.eigvals(self.getPointsUncMatrices()).real
         return -np.sort(-ev, axis=1)
+
+
+class Server:
+    """Request loop (unc_server_*): submit() returns a ticket at once, wait()
+    returns that request's Result plus its latency and queueing time (ms).
+    The Jacobian arrays are kept referenced until the ticket is waited for."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 64 << 30,
+                 bypass_ms: float = 50.0):
+        L = lib()
+        cfg = ServerConfig()
+        L.unc_server_default_config(ctypes.byref(cfg))
+        cfg.device, cfg.slots, cfg.hbm_budget, cfg.bypass_ms = device, slots, hbm_budget, bypass_ms
+        self.handle = ctypes.c_void_p()
+        _check(L.unc_server_create(ctypes.byref(cfg), ctypes.byref(self.handle)),
+               "unc_server_create")
+        self._live = {}
+
+    def submit(self, jac, precision: int = PREC_MIXED, refine=True, fix=None) -> int:
+        o = make_options(jac, precision, refine, 0, fix, None)
+        cp = jac.cam_params
+        cam = np.zeros(jac.num_cams * (cp * (cp + 1) // 2), np.float64)
+        pt = np.zeros(jac.num_points * 6, np.float64)
+        rows = np.ascontiguousarray(jac.rows, np.int32)
+        cols = np.ascontiguousarray(jac.cols, np.int32)
+        vals = np.ascontiguousarray(jac.vals, np.float64)
+        st = Stats()
+        t = ctypes.c_longlong()
+        _check(lib().unc_server_submit(self.handle, ctypes.byref(o), jac.num_rows, jac.num_cols,
+                                       _ptr(rows), _ptr(cols), _ptr(vals), _ptr(cam), _ptr(pt),
+                                       ctypes.byref(st), ctypes.byref(t)), "unc_server_submit")
+        self._live[t.value] = (jac, o, rows, cols, vals, cam, pt, st)
+        return t.value
+
+    def wait(self, ticket: int):
+        status = ctypes.c_int()
+        lat = ctypes.c_double()
+        q = ctypes.c_double()
+        _check(lib().unc_server_wait(self.handle, ticket, ctypes.byref(status), ctypes.byref(lat),
+                                     ctypes.byref(q)), "unc_server_wait")
+        jac, o, rows, cols, vals, cam, pt, st = self._live.pop(ticket)
+        _check(status.value, "request")
+        cp = jac.cam_params
+        res = Result(cam.reshape(jac.num_cams, cp * (cp + 1) // 2), pt.reshape(jac.num_points, 6),
+                     st.lambda_, st.num_terms, list(st.cycle_change[:st.num_terms]),
+                     tuple(st.fixed_pts), st, None)
+        return res, lat.value, q.value
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib().unc_server_query(self.handle, ctypes.byref(s)), "unc_server_query")
+        return s
+
+    def close(self):
+        if self.handle:
+            lib().unc_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

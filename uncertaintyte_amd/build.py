@@ -25,7 +25,7 @@ HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_
 # MFMA accumulators in VGPR form: the loop-carried accumulators of the GEMM
 # then stay in place instead of being copied between AGPRs and VGPRs each slab.
 PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
-CPP_SOURCES = ["uncertainty_api.cpp"]
+CPP_SOURCES = ["uncertainty_api.cpp", "server.cpp"]
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")
 

@@ -26,128 +26,83 @@ namespace unc {
 
 // ---------------------------------------------------------------------------
 // 64x64 leaf: Cholesky A = L L^T and X = L^-1, both lower triangular with zero
-// upper parts.  256 threads as a 16 x 16 grid, thread (ti, tj) keeps rows
-// ti + 16u and columns tj + 16v (u, v < 4) in registers; each pivot step
-// publishes one column through LDS.  A's lower triangle is read (the upper
-// one may hold stale values).
+// upper parts, by one wave.
+//   Cholesky, left-looking: lane i owns row i of L in registers; at step j it
+//   forms s = A(i,j) - L(i,0:j) . L(j,0:j) with row j read back from LDS,
+//   the pivot comes from lane j by a shuffle.
+//   Inverse, column-parallel forward substitution: lane j owns column j of X,
+//   L is read from LDS by broadcast.
+// Every loop is unrolled, so the rows live in registers.  A's lower triangle
+// is read (the upper one may hold stale values).
 // ---------------------------------------------------------------------------
 constexpr int CL_B = 64;
 
-__global__ __launch_bounds__(256) void k_chol_inv64(int b, double *A, int lda, double *Xo, int ldx,
-                                                    unsigned int *err) {
-  __shared__ double colp[CL_B], rowp[CL_B];
-  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-  double m[4][4];
+__global__ __launch_bounds__(64) void k_chol_inv64(int b, double *A, int lda, double *Xo, int ldx,
+                                                   unsigned int *err) {
+  __shared__ double Ls[CL_B][CL_B + 1];
+  const int i = threadIdx.x;
+  double row[CL_B];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = ti + 16 * u, j = tj + 16 * v;
-      const int r = i >= j ? i : j, c = i >= j ? j : i;  // lower-triangle element
-      m[u][v] = (i < b && j < b) ? A[(size_t)r + (size_t)c * lda] : (i == j ? 1.0 : 0.0);
-    }
+  for (int k = 0; k < CL_B; ++k)
+    row[k] = (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0) : (i == k ? 1.0 : 0.0);
   bool bad = false;
-  // Cholesky, right-looking: after step p column p holds L(:, p)
-  for (int p = 0; p < b; ++p) {
-    const int pu = p >> 4, pl = p & 15;
-    if (tj == pl) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        double x = 0.0;
+  for (int j = 0; j < CL_B; ++j) {
+    double s0 = row[j], s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
-        colp[ti + 16 * u] = x;
-      }
+    for (int k = 0; k + 3 < j; k += 4) {
+      s0 = fma(-row[k], Ls[j][k], s0);
+      s1 = fma(-row[k + 1], Ls[j][k + 1], s1);
+      s2 = fma(-row[k + 2], Ls[j][k + 2], s2);
+      s3 = fma(-row[k + 3], Ls[j][k + 3], s3);
     }
-    __syncthreads();
-    const double piv = colp[p];
+#pragma unroll
+    for (int k = j & ~3; k < j; ++k) s0 = fma(-row[k], Ls[j][k], s0);
+    const double s = (s0 + s1) + (s2 + s3);
+    const double piv = __shfl(s, j);
     if (!(piv > 0.0)) bad = true;
-    const double s = sqrt(piv), is = 1.0 / s;
-    double ci[4], cj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) cj[v] = colp[tj + 16 * v];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = ti + 16 * u;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int j = tj + 16 * v;
-        double x = m[u][v];
-        if (j == p) x = (i > p) ? ci[u] * is : (i == p ? s : 0.0);
-        else if (i == p) x = (j < p) ? x : 0.0;
-        else if (i > p && j > p) x = x - (ci[u] * cj[v]) / piv;
-        m[u][v] = x;
-      }
-    }
+    const double d = sqrt(piv);
+    const double l = (i > j) ? s / d : (i == j ? d : 0.0);
+    row[j] = l;
+    Ls[i][j] = l;
     __syncthreads();
   }
+  if (i < b) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = ti + 16 * u, j = tj + 16 * v;
-      if (i < b && j < b) A[(size_t)i + (size_t)j * lda] = m[u][v];
-    }
-  // in-register Gauss-Jordan inverse of the lower-triangular factor
-  for (int p = 0; p < b; ++p) {
-    const int pu = p >> 4, pl = p & 15;
-    if (tj == pl) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        double x = 0.0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) x = (v == pu) ? m[u][v] : x;
-        colp[ti + 16 * u] = x;
-      }
-    }
-    if (ti == pl) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double x = 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) x = (u == pu) ? m[u][v] : x;
-        rowp[tj + 16 * v] = x;
-      }
-    }
-    __syncthreads();
-    const double ip = 1.0 / colp[p];
-    double ci[4], rj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) ci[u] = colp[ti + 16 * u];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) rj[v] = rowp[tj + 16 * v] * ip;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = ti + 16 * u;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int j = tj + 16 * v;
-        double x;
-        if (i == p && j == p) x = ip;
-        else if (i == p) x = rj[v];
-        else if (j == p) x = -ci[u] * ip;
-        else x = m[u][v] - ci[u] * rj[v];
-        m[u][v] = x;
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < CL_B; ++k)
+      if (k < b) A[(size_t)i + (size_t)k * lda] = row[k];
   }
-  if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+  // X = L^-1: lane j solves L x = e_j
+  const int j = i;
+  double x[CL_B];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int r = 0; r < CL_B; ++r) {
+    double s0 = (r == j) ? 1.0 : 0.0, s1 = 0.0;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = ti + 16 * u, j = tj + 16 * v;
-      if (i < b && j < b) Xo[(size_t)i + (size_t)j * ldx] = (i >= j) ? m[u][v] : 0.0;
+    for (int k = 0; k + 1 < r; k += 2) {
+      s0 = fma(-Ls[r][k], x[k], s0);
+      s1 = fma(-Ls[r][k + 1], x[k + 1], s1);
     }
+    if (r & 1) s0 = fma(-Ls[r][r - 1], x[r - 1], s0);
+    x[r] = (r >= j) ? (s0 + s1) / Ls[r][r] : 0.0;
+  }
+  __syncthreads();
+  // transpose through LDS so the column-major stores are coalesced
+#pragma unroll
+  for (int r = 0; r < CL_B; ++r) Ls[r][j] = x[r];
+  __syncthreads();
+  if (i < b) {
+#pragma unroll
+    for (int c = 0; c < CL_B; ++c)
+      if (c < b) Xo[(size_t)i + (size_t)c * ldx] = (i >= c) ? Ls[i][c] : 0.0;
+  }
+  if (bad && i == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st) {
   if (b < 1 || b > CL_B) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chol_inv64, dim3(1), dim3(256), 0, st, b, A, lda, X, ldx, err);
+  hipLaunchKernelGGL(k_chol_inv64, dim3(1), dim3(64), 0, st, b, A, lda, X, ldx, err);
   return hipGetLastError();
 }
 
@@ -348,74 +303,88 @@ hipError_t gram(int n, int k1, const double *V, int k2, const double *W, double 
   return hipGetLastError();
 }
 
-// T = L^-T for G = L L^T (k x k, row-major): V T has orthonormal columns
-__global__ void k_small_orth(int k, const double *G, double *T, unsigned int *err) {
+// T = L^-T for G = L L^T (K x K, row-major): V T has orthonormal columns.
+// One thread; K is a template parameter so the factors stay in registers.
+template <int K>
+__global__ void k_small_orth(const double *G, double *T, unsigned int *err) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double L[16][16] = {}, Li[16][16] = {};
+  double L[K][K], Li[K][K];
   bool bad = false;
-  for (int j = 0; j < k; ++j) {
-    double s = G[j * k + j];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    double s = G[j * K + j];
+#pragma unroll
     for (int q = 0; q < j; ++q) s -= L[j][q] * L[j][q];
     if (!(s > 0.0)) {
       bad = true;
       s = 1e-300;
     }
     L[j][j] = sqrt(s);
-    for (int i = j + 1; i < k; ++i) {
-      double t = G[i * k + j];
+#pragma unroll
+    for (int i = j + 1; i < K; ++i) {
+      double t = G[i * K + j];
+#pragma unroll
       for (int q = 0; q < j; ++q) t -= L[i][q] * L[j][q];
       L[i][j] = t / L[j][j];
     }
   }
-  for (int j = 0; j < k; ++j) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
     Li[j][j] = 1.0 / L[j][j];
-    for (int i = j + 1; i < k; ++i) {
+#pragma unroll
+    for (int i = j + 1; i < K; ++i) {
       double s = 0.0;
+#pragma unroll
       for (int q = j; q < i; ++q) s += L[i][q] * Li[q][j];
       Li[i][j] = -s / L[i][i];
     }
   }
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < k; ++j) T[i * k + j] = Li[j][i];
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+#pragma unroll
+    for (int j = 0; j < K; ++j) T[i * K + j] = (j >= i) ? Li[j][i] : 0.0;
   if (bad) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
-// S = (C + G)^-1 with C = diag(I_h, -I_h), k = 2h (row-major); Gauss-Jordan
-// with partial pivoting
-__global__ void k_small_woodbury(int k, const double *G, double *S, unsigned int *err) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double a[16][32];
+// S = (C + G)^-1 with C = diag(I_h, -I_h), k = 2h <= 16 (row-major):
+// Gauss-Jordan with partial pivoting on [C + G | I] held in LDS, thread per
+// element of the augmented matrix.
+__global__ __launch_bounds__(512) void k_small_woodbury(int k, const double *G, double *S,
+                                                       unsigned int *err) {
+  __shared__ double a[16][33];
+  __shared__ int best_s;
+  const int tid = threadIdx.x, i = tid >> 5, j = tid & 31;
   const int h = k / 2;
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < 2 * k; ++j)
-      a[i][j] = j < k ? G[i * k + j] + (i == j ? (i < h ? 1.0 : -1.0) : 0.0) : (j - k == i ? 1.0 : 0.0);
+  const bool act = i < k && j < 2 * k;
+  if (act)
+    a[i][j] = j < k ? G[i * k + j] + (i == j ? (i < h ? 1.0 : -1.0) : 0.0) : (j - k == i ? 1.0 : 0.0);
+  __syncthreads();
   bool bad = false;
   for (int p = 0; p < k; ++p) {
-    int best = p;
-    for (int i = p + 1; i < k; ++i)
-      if (fabs(a[i][p]) > fabs(a[best][p])) best = i;
-    if (best != p)
-      for (int j = 0; j < 2 * k; ++j) {
-        double t = a[p][j];
-        a[p][j] = a[best][j];
-        a[best][j] = t;
-      }
-    if (!(fabs(a[p][p]) > 0.0)) {
-      bad = true;
-      break;
+    if (tid == 0) {
+      int best = p;
+      for (int r = p + 1; r < k; ++r)
+        if (fabs(a[r][p]) > fabs(a[best][p])) best = r;
+      best_s = best;
     }
-    const double ip = 1.0 / a[p][p];
-    for (int j = 0; j < 2 * k; ++j) a[p][j] *= ip;
-    for (int i = 0; i < k; ++i) {
-      if (i == p) continue;
-      const double f = a[i][p];
-      if (f != 0.0)
-        for (int j = 0; j < 2 * k; ++j) a[i][j] -= f * a[p][j];
+    __syncthreads();
+    const int best = best_s;
+    if (best != p && tid < 2 * k) {
+      const double t0 = a[p][tid];
+      a[p][tid] = a[best][tid];
+      a[best][tid] = t0;
     }
+    __syncthreads();
+    const double piv = a[p][p];
+    if (!(fabs(piv) > 0.0)) bad = true;
+    const double ip = 1.0 / piv;
+    const double f = act ? a[i][p] : 0.0, rp = act ? a[p][j] * ip : 0.0;
+    __syncthreads();
+    if (act) a[i][j] = (i == p) ? rp : a[i][j] - f * rp;
+    __syncthreads();
   }
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < k; ++j) S[i * k + j] = 0.5 * (a[i][k + j] + a[j][k + i]);
-  if (bad) atomicOr(err, ERRBIT_GJ_PIVOT);
+  if (act && j < k) S[i * k + j] = 0.5 * (a[i][k + j] + a[j][k + i]);
+  if (bad && tid == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
 // out(n x k2) = alpha A + beta B T (A n x k2, B n x k1, T k1 x k2 row-major,
@@ -439,13 +408,13 @@ __global__ void k_small_apply(int n, int k1, int k2, double alpha, const double 
 }
 
 hipError_t small_orth(int k, const double *G, double *T, unsigned int *err, hipStream_t st) {
-  if (k < 1 || k > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_small_orth, dim3(1), dim3(64), 0, st, k, G, T, err);
+  if (k != 7) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_orth<7>, dim3(1), dim3(64), 0, st, G, T, err);
   return hipGetLastError();
 }
 hipError_t small_woodbury(int k, const double *G, double *S, unsigned int *err, hipStream_t st) {
   if (k < 2 || k > 16 || (k & 1)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_small_woodbury, dim3(1), dim3(64), 0, st, k, G, S, err);
+  hipLaunchKernelGGL(k_small_woodbury, dim3(1), dim3(512), 0, st, k, G, S, err);
   return hipGetLastError();
 }
 hipError_t small_apply(int n, int k1, int k2, double alpha, const double *A, double beta,
@@ -475,21 +444,21 @@ hipError_t fill_hash(size_t cnt, double *V, unsigned long long seed, hipStream_t
   return hipGetLastError();
 }
 
-// out = A + V V^T (n x n; V n x k column-major)
-__global__ void k_rank_copy(int n, int k, const double *A, const double *V, double *out) {
-  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < (size_t)n * n;
-       x += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(x % n), c = (int)(x / n);
-    double s = 0.0;
-    for (int q = 0; q < k; ++q) s = fma(V[r + (size_t)q * n], V[c + (size_t)q * n], s);
-    out[x] = A[x] + s;
-  }
+// out = A + V V^T (n x n; V n x 7 column-major); block (x, c) covers 256 rows
+// of column c
+__global__ void k_rank_copy(int n, const double *A, const double *V, double *out) {
+  const int c = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) s = fma(V[r + (size_t)q * n], V[c + (size_t)q * n], s);
+  const size_t x = (size_t)r + (size_t)c * n;
+  out[x] = A[x] + s;
 }
 
 hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st) {
-  size_t g = cdiv((size_t)n * n, 256);
-  hipLaunchKernelGGL(k_rank_copy, dim3((unsigned int)(g > 16384 ? 16384 : g)), dim3(256), 0, st, n,
-                     k, A, V, out);
+  if (k != 7 || n > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_rank_copy, dim3(cdiv(n, 256), n), dim3(256), 0, st, n, A, V, out);
   return hipGetLastError();
 }
 

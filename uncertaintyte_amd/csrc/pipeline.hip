@@ -527,6 +527,34 @@ unc_context *default_context(int device, int *rc) {
   return g_default[device];
 }
 
+}  // namespace
+
+// ---- internal helpers for the request server (server.cpp)
+size_t unc_ctx_held_bytes(unc_context *ctx) { return ctx ? ctx->ws_bytes + ctx->io_bytes : 0; }
+size_t unc_ctx_ws_bytes(unc_context *ctx) { return ctx ? ctx->ws_bytes : 0; }
+size_t unc_ctx_io_bytes(unc_context *ctx) { return ctx ? ctx->io_bytes : 0; }
+
+void unc_ctx_release(unc_context *ctx) {
+  if (!ctx) return;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->io) (void)hipFree(ctx->io);
+  ctx->ws = ctx->io = nullptr;
+  ctx->ws_bytes = ctx->io_bytes = 0;
+}
+
+size_t unc_io_bytes_for(const unc_options *o, int num_rows, size_t nnz) {
+  const int cp = o->cam_params;
+  const size_t per = (size_t)cp * (cp + 1) / 2;
+  auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+  return r256(sizeof(int) * ((size_t)num_rows + 1)) + r256(sizeof(int) * nnz) +
+         r256(sizeof(double) * nnz) + r256(sizeof(double) * (size_t)o->num_cams * per) +
+         r256(sizeof(double) * (size_t)o->num_points * 6);
+}
+
+namespace {
+
 int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
                  const int *rows, const int *cols, const double *values, double *cam_unc,
                  double *pt_unc, unc_stats *stats, unc_debug *dbg) {
@@ -665,7 +693,7 @@ const char *unc_error_string(int code) {
     case UNC_OK: return "ok";
     case UNC_ERR_BAD_INPUT: return "bad input (Jacobian layout, sizes or ids)";
     case UNC_ERR_NO_FIXED_POINTS: return "no fixed points given (the reference crashes here)";
-    case UNC_ERR_SINGULAR: return "singular input (zero Jacobian column or singular 3x3 point block)";
+    case UNC_ERR_SINGULAR: return "singular input (zero Jacobian column, singular 3x3 point block, or a null space beyond the 7-dim gauge)";
     case UNC_ERR_OOM: return "out of device memory";
     case UNC_ERR_DEVICE: return "HIP runtime error";
     case UNC_ERR_UNSUPPORTED: return "unsupported option or layout";
