@@ -6,12 +6,22 @@ J ~8.2e5 x 3.0e5 with 9.8e6 nonzeros, dense Z 1800 x 1800).  A step is one
 complete solve (all cameras + all points) with the Jacobian already resident
 in HBM; outputs stay on the device.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-zplus] [--no-serving]
 
 Prints one JSON line.  `roofline` describes the dominant kernel (the FP64
 MFMA GEMM of the Taylor chain) from HIP events recorded by the library around
 each launch inside the timed region; `cpu_baseline` times the CPU oracle (C,
 OpenMP) on one solve of the same scene.
+
+Two further measurements ride on the same line:
+  zplus    the gauge-fixed Z+ path (UNC_PREC_MIXED) against the plain FP64 TE
+           path on walk1500 (1500 cameras on a random walk, 1M points, track
+           Poisson(5) min 2, 10% short-baseline points; N = 13491), both with
+           the Jacobian resident in HBM, plus the camera-block error of the Z+
+           output against D pinv_7(Z_s) D on 188 sampled cameras (oracle);
+  serving  512 requests (cameras log-uniform 20..800, points per camera
+           25..400, 64 distinct seeded scenes) arriving as a Poisson process
+           into the request server (4 slots), p50/p99 latency and solves/s.
 """
 from __future__ import annotations
 
@@ -27,6 +37,95 @@ sys.path.insert(0, ROOT)
 METRIC = "cov solves/s, mixed vs FP64 Z⁺ speedup at ≤1e-5 err; HBM GB/s; serving p50/p99"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+WORKLOADS = {
+    "rings200": "rings200: 200 cameras on 4 rings, 100k points, track Poisson(4) min 2",
+    "walk1500": "walk1500: 1500 cameras on a random walk, 1M points, track Poisson(5) min 2, "
+                "10% short-baseline points",
+    "walk4000": "walk4000: 4000 cameras on a random walk, 3M points, track Poisson(6) min 2",
+    "toy": "toy: 10 cameras on a ring, 1000 points",
+}
+PATHS = {"fp64": "FP64 plain TE path", "mixed": "gauge-fixed Z+ path (UNC_PREC_MIXED)",
+         "zplus_fp64": "gauge-fixed Z+ path (UNC_PREC_ZPLUS_FP64)"}
+
+
+def _device_solver(u, torch, dev, J, precision):
+    rows = torch.from_numpy(J.rows).to(dev)
+    cols = torch.from_numpy(J.cols).to(dev)
+    vals = torch.from_numpy(J.vals).to(dev)
+    per = J.cam_params * (J.cam_params + 1) // 2
+    cam = torch.zeros(J.num_cams * per, dtype=torch.float64, device=dev)
+    pt = torch.zeros(J.num_points * 6, dtype=torch.float64, device=dev)
+    ctx = u.Context(0)
+    opts = u.make_options(J, precision)
+
+    def call():
+        return u.compute_device(opts, J.num_rows, J.num_cols, rows.data_ptr(), cols.data_ptr(),
+                                vals.data_ptr(), cam.data_ptr(), pt.data_ptr(), ctx=ctx)
+    return call, ctx, cam, pt
+
+
+def _timed(call, torch, reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = call()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3, st
+
+
+def measure_zplus(u, scene, torch, dev, check=True):
+    """Z+ path vs plain FP64 TE path on walk1500, inputs resident in HBM."""
+    import numpy as np
+    J = scene.make_workload("walk1500")
+    out = {"workload": "walk1500: 1500 cameras on a random walk, 1M points, track Poisson(5) "
+                       "min 2, 10% short-baseline points", "N": J.num_cams * J.cam_params,
+           "observations": J.num_obs}
+    zcall, zctx, zcam, _ = _device_solver(u, torch, dev, J, u.PREC_MIXED)
+    zcall()
+    z_ms, zst = _timed(zcall, torch, 3)
+    zcam_h = zcam.cpu().numpy().reshape(J.num_cams, -1)
+    zctx.close()
+    fcall, fctx, _, _ = _device_solver(u, torch, dev, J, u.PREC_FP64)
+    fcall()
+    f_ms, fst = _timed(fcall, torch, 1)
+    fctx.close()
+    out.update(zplus_ms=round(z_ms, 2), fp64_te_ms=round(f_ms, 2),
+               speedup=round(f_ms / z_ms, 2), zplus_inverse_ms=round(zst.stage_ms[6], 2),
+               fp64_inverse_plus_taylor_ms=round(fst.stage_ms[6] + fst.stage_ms[7], 2),
+               arithmetic="FP64 Cholesky selected inversion (MFMA FP64 GEMMs)")
+    if check:
+        import oracle
+        cams = list(range(0, J.num_cams, 8))
+        ref, _ = oracle.zplus_sampled(J, cams, nthreads=min(os.cpu_count() or 1, 16))
+        a = zcam_h[cams]
+        rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out.update(cam_err_max=float(rel.max()), cam_err_median=float(np.median(rel)),
+                   err_sample=f"{len(cams)} cameras vs D pinv_7(Z_s) D (ARPACK + LAPACK oracle)",
+                   contract=1e-5)
+    return out
+
+
+def measure_serving(u, rate=150.0, n_requests=512):
+    from uncertaintyte_amd import serve
+    scenes, order = serve.make_requests(n_requests, 64, workers=min(os.cpu_count() or 1, 16))
+    srv = u.Server(slots=4)
+    # warm state: every distinct scene once, so the slots' cached buffers and the
+    # kernels' code objects are in place before the timed load
+    for tk in [srv.submit(s, precision=u.PREC_MIXED) for s in scenes]:
+        try:
+            srv.wait(tk)
+        except u.UncError:
+            pass
+    res = serve.run_load(srv, scenes, order, rate=rate, precision=u.PREC_MIXED)
+    st = srv.stats()
+    srv.close()
+    out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.summary().items()}
+    out.update(rate_per_s=rate, slots=4, precision="Z+ (UNC_PREC_MIXED)",
+               peak_held_GiB=round(st.peak_held_bytes / 2 ** 30, 2),
+               max_concurrent=st.max_concurrent,
+               workload="512 requests, cameras log-uniform 20..800, points/camera 25..400, "
+                        "64 distinct seeded walk scenes, Poisson arrivals")
+    return out
 
 
 def main() -> int:
@@ -35,8 +134,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rings200")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "zplus_fp64"],
+                    help="path of the main measurement (the headline value is fp64)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-zplus", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the Z+ accuracy check")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--rate", type=float, default=150.0, help="serving arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the pipeline is single-GPU",
@@ -61,7 +166,8 @@ def main() -> int:
     pt = torch.zeros(J.num_points * 6, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     ctx = u.Context(0)
-    opts = u.make_options(J)
+    prec = {"fp64": u.PREC_FP64, "mixed": u.PREC_MIXED, "zplus_fp64": u.PREC_ZPLUS_FP64}[args.precision]
+    opts = u.make_options(J, prec)
     call = lambda: u.compute_device(opts, J.num_rows, J.num_cols, rows.data_ptr(), cols.data_ptr(),
                                     vals.data_ptr(), cam.data_ptr(), pt.data_ptr(), ctx=ctx)
     for _ in range(args.warmup):
@@ -103,13 +209,18 @@ def main() -> int:
                "sample": f"1 full solve of the same {args.workload} scene ({cs:.1f} s), C oracle, "
                          f"OpenMP {threads} threads"}
 
+    # serving before the Z+ accuracy check: the CPU oracle's BLAS threads would
+    # otherwise compete with the request threads
+    serving = None if args.no_serving else measure_serving(u, rate=args.rate)
+    zplus = None if args.no_zplus else measure_zplus(u, scene, torch, dev, check=not args.no_check)
+
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded RADIAL-camera scene, analytic Jacobian)",
-        "config": {"workload": "rings200: 200 cameras on 4 rings, 100k points, track Poisson(4) min 2, "
-                               "FP64 plain TE path", "cameras": nC, "points": nP, "observations": nO,
+        "config": {"workload": WORKLOADS.get(args.workload, args.workload) + ", " + PATHS[args.precision],
+                   "cameras": nC, "points": nP, "observations": nO,
                    "jacobian_nnz": nnz, "N": N, "taylor_terms": terms, "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -122,6 +233,8 @@ def main() -> int:
                 "schur_GBps": round(schur_bytes / (stage[5] * 1e-3) / 1e9, 1) if stage[5] > 0 else None,
                 "peak_GBps": PEAK_HBM_GBS},
         "scene_gen_s": round(gen_s, 2),
+        "zplus": zplus,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

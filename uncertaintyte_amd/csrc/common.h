@@ -76,10 +76,26 @@ hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in
                               int *vals_in, int *cam_list, int *cam_off, void *temp,
                               size_t temp_bytes, hipStream_t st);
 hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st);
-hipError_t launch_covis(const Structure &s, unsigned int *covis, int *kcnt, int *kmax,
-                        hipStream_t st);
-hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, const unsigned int *covis,
-                        const int *kcnt, int kmax, hipStream_t st, int max_lds_bytes);
+// Schur complement by sorted camera-pair lists (k_schur.hip, k_sort.hip)
+hipError_t launch_pair_count(const Structure &s, int *cnt, hipStream_t st);
+hipError_t launch_cam_pair_start(const Structure &s, const int *pofs, const int *cnt, int *cps,
+                                 hipStream_t st);
+hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base, const int *pofs,
+                            unsigned int *keys, int *vals, int *o1s, int *o2s, hipStream_t st);
+hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
+                               const unsigned int *ukeys, const int *counts, const int *offs,
+                               const int *nruns, const int *order, const int *o1s,
+                               const int *o2s, hipStream_t st);
+size_t pair_sort_temp_bytes(int n, int key_bits);
+hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out, const int *vals_in,
+                     int *vals_out, int n, int key_bits, void *temp, size_t temp_bytes,
+                     hipStream_t st);
+size_t runs_temp_bytes(int n);
+hipError_t runs(const unsigned int *keys, int n, unsigned int *unique, int *counts, int *nruns,
+                void *temp, size_t temp_bytes, hipStream_t st);
+size_t scan_temp_bytes(int n);
+hipError_t exclusive_scan_int(const int *in, int *out, int n, void *temp, size_t temp_bytes,
+                              hipStream_t st);
 hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *X,
                              double *cam_out, hipStream_t st);
 hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, hipStream_t st);

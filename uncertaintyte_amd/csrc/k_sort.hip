@@ -9,6 +9,8 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
 
@@ -59,6 +61,57 @@ hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in
   hipLaunchKernelGGL(k_segment_offsets, dim3(cdiv(nobs + 1, 256)), dim3(256), 0, st, keys_out, nobs,
                      nc, cam_off);
   return hipGetLastError();
+}
+
+// ---- pair grouping for the Schur complement (k_schur.hip)
+// stable sort of (block key, pair index); then runs of equal keys; exclusive
+// scans of int counts.  Temp sizes are for n elements.
+size_t pair_sort_temp_bytes(int n, int key_bits) {
+  size_t b = 0;
+  if (rocprim::radix_sort_pairs(nullptr, b, (unsigned int *)nullptr, (unsigned int *)nullptr,
+                                (int *)nullptr, (int *)nullptr, (unsigned int)n, 0, key_bits) !=
+      hipSuccess)
+    return 0;
+  return b;
+}
+
+hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out, const int *vals_in,
+                     int *vals_out, int n, int key_bits, void *temp, size_t temp_bytes,
+                     hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  size_t tb = temp_bytes;
+  return rocprim::radix_sort_pairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (unsigned int)n,
+                                   0, key_bits, st);
+}
+
+size_t runs_temp_bytes(int n) {
+  size_t b = 0;
+  if (rocprim::run_length_encode(nullptr, b, (const unsigned int *)nullptr, (unsigned int)n,
+                                 (unsigned int *)nullptr, (int *)nullptr, (int *)nullptr) !=
+      hipSuccess)
+    return 0;
+  return b;
+}
+
+hipError_t runs(const unsigned int *keys, int n, unsigned int *unique, int *counts, int *nruns,
+                void *temp, size_t temp_bytes, hipStream_t st) {
+  size_t tb = temp_bytes;
+  return rocprim::run_length_encode(temp, tb, keys, (unsigned int)n, unique, counts, nruns, st);
+}
+
+size_t scan_temp_bytes(int n) {
+  size_t b = 0;
+  if (rocprim::exclusive_scan(nullptr, b, (const int *)nullptr, (int *)nullptr, 0, (size_t)n,
+                              rocprim::plus<int>()) != hipSuccess)
+    return 0;
+  return b;
+}
+
+hipError_t exclusive_scan_int(const int *in, int *out, int n, void *temp, size_t temp_bytes,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  size_t tb = temp_bytes;
+  return rocprim::exclusive_scan(temp, tb, in, out, 0, (size_t)n, rocprim::plus<int>(), st);
 }
 
 }  // namespace unc

@@ -45,7 +45,8 @@ struct Sizes {
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
-  B_ERR, B_SCAN, B_COVIS, B_KCNT, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
+  B_ERR, B_SCAN, B_PCNT, B_POFS, B_CPS, B_PK0, B_PK1, B_PV0, B_PV1, B_PO1, B_PO2, B_RUNK, B_RUNC,
+  B_RUNO, B_NRUN, B_PTMP, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
   B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
 };
 
@@ -57,6 +58,16 @@ struct Layout {
 constexpr int GJ_BLOCK = 64;
 size_t zplus_work_doubles(int n);
 
+size_t pair_capacity(int nobs) { return std::max<size_t>((size_t)4 * (size_t)nobs, (size_t)1 << 20); }
+
+size_t pair_temp_bytes(int nobs) {
+  const int cap = (int)pair_capacity(nobs);
+  size_t b = pair_sort_temp_bytes(cap, 32);
+  b = std::max(b, runs_temp_bytes(cap));
+  b = std::max(b, scan_temp_bytes(std::max(cap, nobs)));
+  return b + 256;
+}
+
 Layout make_layout(const Sizes &z, bool refine) {
   size_t sz[B_COUNT];
   sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
@@ -66,8 +77,15 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_PT_CNT] = sz[B_PT_CURSOR] = sz[B_PT_HAS] = sizeof(int) * (size_t)z.npc;
   sz[B_PT_OFF] = sizeof(int) * ((size_t)z.npc + 1);
   sz[B_ERR] = 64;
-  sz[B_COVIS] = sizeof(unsigned int) * (size_t)z.nc * (((size_t)z.nc + 31) / 32);
-  sz[B_KCNT] = sizeof(int) * (size_t)z.nc;
+  // camera-pair lists of the Schur complement: one camera's pairs never exceed
+  // the observations of its points, so pair_capacity(nobs) >= any camera
+  const size_t cap = pair_capacity(z.nobs);
+  sz[B_PCNT] = sz[B_POFS] = sizeof(int) * (size_t)z.nobs;
+  sz[B_CPS] = sizeof(int) * ((size_t)z.nc + 1);
+  sz[B_PK0] = sz[B_PK1] = sz[B_RUNK] = sizeof(unsigned int) * cap;
+  sz[B_PV0] = sz[B_PV1] = sz[B_PO1] = sz[B_PO2] = sz[B_RUNC] = sz[B_RUNO] = sizeof(int) * cap;
+  sz[B_NRUN] = sizeof(int);
+  sz[B_PTMP] = pair_temp_bytes(z.nobs);
   sz[B_SCAN] = sizeof(int) * ((size_t)std::max(z.npc, z.nc) / 1024 + 2);
   sz[B_US] = sz[B_XD] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
   sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
@@ -136,12 +154,15 @@ int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[
   return UNC_OK;
 }
 
-int ensure(char **buf, size_t *have, size_t need) {
+// Grow a cached device buffer.  Stream-ordered (hipMallocAsync/hipFreeAsync on
+// the context's stream), so a context that grows does not stall the other
+// streams of the device the way hipFree's implicit synchronisation would.
+int ensure(char **buf, size_t *have, size_t need, hipStream_t s) {
   if (*have >= need) return UNC_OK;
-  if (*buf) (void)hipFree(*buf);
+  if (*buf) (void)hipFreeAsync(*buf, s);
   *buf = nullptr;
   *have = 0;
-  if (hipMalloc((void **)buf, need) != hipSuccess) {
+  if (hipMallocAsync((void **)buf, need, s) != hipSuccess) {
     (void)hipGetLastError();
     *buf = nullptr;
     return UNC_ERR_OOM;
@@ -290,7 +311,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (rc) return rc;
   const bool refine = o->refine != 0;
   Layout L = make_layout(z, refine);
-  rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total);
+  rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total, s);
   if (rc) return rc;
   char *w = ctx->ws;
   auto ptr = [&](int b) { return (void *)(w + L.off[b]); };
@@ -329,23 +350,50 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
                         (int *)ptr(B_SORT_V), S.cam_list, S.cam_off, ptr(B_SORT_TMP),
                         camera_sort_temp_bytes(z.nobs, z.nc) + 256, s));
   HC(hipEventRecord(E[3], s));
-  // ---- (a) + (b), and the co-visibility rows the Schur kernel is sized by
+  // ---- (a) + (b)
   HC(launch_assemble(S, Bk, s));
-  unsigned int *covis = (unsigned int *)ptr(B_COVIS);
-  int *kcnt = (int *)ptr(B_KCNT);
-  int *kmax_d = (int *)(S.err + 1);
-  HC(launch_covis(S, covis, kcnt, kmax_d, s));
-  unsigned int hbuf[2] = {0u, 0u};
-  HC(hipMemcpyAsync(hbuf, S.err, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  herr = hbuf[0];
-  if (herr) return map_err(herr);
-  const int kmax = (int)hbuf[1];
   HC(hipEventRecord(E[4], s));
   HC(hipEventRecord(E[5], s));
-  // ---- (c) Schur into dA
+  // ---- (c) camera-pair lists: counts, positions, per-camera starts
+  int *pcnt = (int *)ptr(B_PCNT), *pofs = (int *)ptr(B_POFS), *cps = (int *)ptr(B_CPS);
+  void *ptmp = ptr(B_PTMP);
+  const size_t ptmp_bytes = pair_temp_bytes(z.nobs);
+  HC(launch_pair_count(S, pcnt, s));
+  HC(exclusive_scan_int(pcnt, pofs, z.nobs, ptmp, ptmp_bytes, s));
+  HC(launch_cam_pair_start(S, pofs, pcnt, cps, s));
+  std::vector<int> h_cps(z.nc + 1), h_off(z.nc + 1);
+  HC(hipMemcpyAsync(h_cps.data(), cps, sizeof(int) * (z.nc + 1), hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(h_off.data(), S.cam_off, sizeof(int) * (z.nc + 1), hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr) return map_err(herr);
+  // ---- (c) Schur into dA, camera ranges whose pairs fit the list capacity
   HC(hipMemsetAsync(dA, 0, sizeof(double) * nn, s));
-  HC(launch_schur(S, Bk, dA, covis, kcnt, kmax, s, ctx->max_lds));
+  {
+    const long long cap = (long long)pair_capacity(z.nobs);
+    unsigned int *k0 = (unsigned int *)ptr(B_PK0), *k1 = (unsigned int *)ptr(B_PK1);
+    int *v0 = (int *)ptr(B_PV0), *v1 = (int *)ptr(B_PV1);
+    int *o1s = (int *)ptr(B_PO1), *o2s = (int *)ptr(B_PO2);
+    unsigned int *rk = (unsigned int *)ptr(B_RUNK);
+    int *rc_ = (int *)ptr(B_RUNC), *ro = (int *)ptr(B_RUNO), *nrun = (int *)ptr(B_NRUN);
+    int a0 = 0;
+    while (a0 < z.nc) {
+      int a1 = a0 + 1;
+      if ((long long)h_cps[a1] - h_cps[a0] > cap) return UNC_ERR_UNSUPPORTED;
+      while (a1 < z.nc && (long long)h_cps[a1 + 1] - h_cps[a0] <= cap) ++a1;
+      const int np_ = h_cps[a1] - h_cps[a0];
+      if (np_ > 0) {
+        int kb = 1;
+        while (kb < 32 && ((unsigned long long)1 << kb) < (unsigned long long)(a1 - a0) * z.nc) ++kb;
+        HC(launch_pair_emit(S, h_off[a0], h_off[a1], a0, h_cps[a0], pofs, k0, v0, o1s, o2s, s));
+        HC(pair_sort(k0, k1, v0, v1, np_, kb, ptmp, ptmp_bytes, s));
+        HC(runs(k1, np_, rk, rc_, nrun, ptmp, ptmp_bytes, s));
+        HC(exclusive_scan_int(rc_, ro, np_, ptmp, ptmp_bytes, s));
+        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, o1s, o2s, s));
+      }
+      a0 = a1;
+    }
+  }
   HC(hipEventRecord(E[6], s));
   if (dbg && dbg->Z) HC(hipMemcpyAsync(dbg->Z, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   if (o->precision != UNC_PREC_FP64) {
@@ -538,8 +586,9 @@ void unc_ctx_release(unc_context *ctx) {
   if (!ctx) return;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  if (ctx->ws) (void)hipFree(ctx->ws);
-  if (ctx->io) (void)hipFree(ctx->io);
+  if (ctx->ws) (void)hipFreeAsync(ctx->ws, ctx->stream);
+  if (ctx->io) (void)hipFreeAsync(ctx->io, ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
   ctx->ws = ctx->io = nullptr;
   ctx->ws_bytes = ctx->io_bytes = 0;
 }
@@ -576,7 +625,7 @@ int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int nu
   const size_t b_vals = ((sizeof(double) * nnz + 255) / 256) * 256;
   const size_t b_cam = ((sizeof(double) * cam_n + 255) / 256) * 256;
   const size_t b_pt = ((sizeof(double) * pt_n + 255) / 256) * 256;
-  rc = ensure(&ctx->io, &ctx->io_bytes, b_rows + b_cols + b_vals + b_cam + b_pt);
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_rows + b_cols + b_vals + b_cam + b_pt, ctx->stream);
   if (rc) return rc;
   int *d_rows = (int *)ctx->io;
   int *d_cols = (int *)(ctx->io + b_rows);
@@ -646,8 +695,9 @@ void unc_context_destroy(unc_context *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &e : c->stage_ev) (void)hipEventDestroy(e);
   for (auto &e : c->gemm_ev) (void)hipEventDestroy(e);
-  if (c->ws) (void)hipFree(c->ws);
-  if (c->io) (void)hipFree(c->io);
+  if (c->ws) (void)hipFreeAsync(c->ws, c->stream);
+  if (c->io) (void)hipFreeAsync(c->io, c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
