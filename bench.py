@@ -209,6 +209,12 @@ def main() -> int:
                "sample": f"1 full solve of the same {args.workload} scene ({cs:.1f} s), C oracle, "
                          f"OpenMP {threads} threads"}
 
+    # the main measurement's context and tensors are released first: the
+    # request server's streams then have the device's hardware queues to
+    # themselves
+    ctx.close()
+    del rows, cols, vals, cam, pt
+    torch.cuda.empty_cache()
     # serving before the Z+ accuracy check: the CPU oracle's BLAS threads would
     # otherwise compete with the request threads
     serving = None if args.no_serving else measure_serving(u, rate=args.rate)
@@ -237,7 +243,6 @@ def main() -> int:
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
     return 0
 
 

@@ -81,15 +81,15 @@ hipError_t launch_pair_count(const Structure &s, int *cnt, hipStream_t st);
 hipError_t launch_cam_pair_start(const Structure &s, const int *pofs, const int *cnt, int *cps,
                                  hipStream_t st);
 hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base, const int *pofs,
-                            unsigned int *keys, int *vals, int *o1s, int *o2s, hipStream_t st);
+                            unsigned int *keys, unsigned long long *vals, hipStream_t st);
 hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
                                const unsigned int *ukeys, const int *counts, const int *offs,
-                               const int *nruns, const int *order, const int *o1s,
-                               const int *o2s, hipStream_t st);
+                               const int *nruns, const unsigned long long *pairs,
+                               hipStream_t st);
 size_t pair_sort_temp_bytes(int n, int key_bits);
-hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out, const int *vals_in,
-                     int *vals_out, int n, int key_bits, void *temp, size_t temp_bytes,
-                     hipStream_t st);
+hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out,
+                     const unsigned long long *vals_in, unsigned long long *vals_out, int n,
+                     int key_bits, void *temp, size_t temp_bytes, hipStream_t st);
 size_t runs_temp_bytes(int n);
 hipError_t runs(const unsigned int *keys, int n, unsigned int *unique, int *counts, int *nruns,
                 void *temp, size_t temp_bytes, hipStream_t st);

@@ -8,8 +8,9 @@
 // 9x9 block S_ab = sum over pairs (o1 of a, o2 of b, same point) of
 // Y_o1 W_o2^T.  The pairs are listed in the order (a, o1 in row order, o2 in
 // point order) -- position = exclusive scan of per-observation pair counts --
-// grouped by block with a stable radix sort on the key (a, b), and cut into runs
-// of equal key.  One workgroup per run stages the pairs' Y and W rows in LDS and
+// grouped by block with a stable radix sort on the key (a, b) carrying the
+// packed (o1, o2), and cut into runs of equal key.  One workgroup per run stages
+// the pairs' Y and W rows in LDS and
 // thread (i, j) forms element (i, j) as one sequential fma chain in list order.
 // That is the CPU oracle's chain, so Z is bit-identical to it and does not
 // depend on scheduling (no atomics on values).  Upper blocks are written with
@@ -42,9 +43,10 @@ __global__ void k_cam_pair_start(Structure s, const int *pofs, const int *cnt, i
   cps[a] = (t < s.nobs) ? pofs[t] : total;
 }
 
-// pairs of positions [t0, t1) (cameras [a0, a1)), written at pofs[t] - base
+// pairs of positions [t0, t1) (cameras [a0, a1)), written at pofs[t] - base:
+// key = (a - a0) nc + b, value = o1 << 32 | o2
 __global__ void k_pair_emit(Structure s, int t0, int t1, int a0, int base, const int *pofs,
-                            unsigned int *keys, int *vals, int *o1s, int *o2s) {
+                            unsigned int *keys, unsigned long long *vals) {
   const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= t1) return;
   const int o1 = s.cam_list[t];
@@ -56,23 +58,25 @@ __global__ void k_pair_emit(Structure s, int t0, int t1, int a0, int base, const
     const int b = s.obs_cam[o2];
     if (b < a) continue;
     keys[w] = row + (unsigned int)b;
-    vals[w] = w;
-    o1s[w] = o1;
-    o2s[w] = o2;
+    vals[w] = ((unsigned long long)(unsigned int)o1 << 32) | (unsigned int)o2;
     ++w;
   }
 }
 
-constexpr int SR_THREADS = 128, SR_CHUNK = 32, SR_GRID = 2048;
+constexpr int SR_THREADS = 128, SR_CHUNK = 64, SR_GRID = 2048;
+constexpr int SR_ROW = 27;  // cp * 3 for cp <= 9
 
 // one run (block (a, b)) per workgroup iteration; thread e owns element (i, j)
-// (upper triangle only on diagonal blocks)
+// (upper triangle only on diagonal blocks).  Per chunk of 64 pairs: the pairs'
+// (o1, o2) are staged first, then every thread issues its share of the 2 x 64
+// Y / W rows as independent loads.
 __global__ __launch_bounds__(SR_THREADS) void k_schur_reduce(Structure s, Blocks bl, double *Z,
                                                              int a0, const unsigned int *ukeys,
                                                              const int *counts, const int *offs,
-                                                             const int *nruns, const int *order,
-                                                             const int *o1s, const int *o2s) {
-  __shared__ double ys[SR_CHUNK][27], ws[SR_CHUNK][27];
+                                                             const int *nruns,
+                                                             const unsigned long long *pairs) {
+  __shared__ double ys[SR_CHUNK][SR_ROW], ws[SR_CHUNK][SR_ROW];
+  __shared__ int po1[SR_CHUNK], po2[SR_CHUNK];
   const int tid = threadIdx.x, cp = s.cp, nc = s.nc, N = s.N, r3 = cp * 3;
   const int nr = *nruns;
   for (int seg = blockIdx.x; seg < nr; seg += gridDim.x) {
@@ -101,11 +105,20 @@ __global__ __launch_bounds__(SR_THREADS) void k_schur_reduce(Structure s, Blocks
     for (int c0 = 0; c0 < cnt; c0 += SR_CHUNK) {
       const int m = min(SR_CHUNK, cnt - c0);
       __syncthreads();
-      for (int x = tid; x < m * 2 * r3; x += SR_THREADS) {
-        const int q = x / (2 * r3), r = x % (2 * r3);
-        const int idx = order[off + c0 + q];
-        if (r < r3) ys[q][r] = bl.Y[(size_t)o1s[idx] * r3 + r];
-        else ws[q][r - r3] = bl.Ws[(size_t)o2s[idx] * r3 + (r - r3)];
+      if (tid < m) {
+        const unsigned long long v = pairs[off + c0 + tid];
+        po1[tid] = (int)(v >> 32);
+        po2[tid] = (int)(v & 0xffffffffull);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < (SR_CHUNK * SR_ROW + SR_THREADS - 1) / SR_THREADS; ++k) {
+        const int x = tid + k * SR_THREADS;
+        const int q = x / SR_ROW, r = x % SR_ROW;
+        if (q < m && r < r3) {
+          ys[q][r] = bl.Y[(size_t)po1[q] * r3 + r];
+          ws[q][r] = bl.Ws[(size_t)po2[q] * r3 + r];
+        }
       }
       __syncthreads();
       if (act) {
@@ -118,15 +131,9 @@ __global__ __launch_bounds__(SR_THREADS) void k_schur_reduce(Structure s, Blocks
     }
     if (act) {
       const size_t r = (size_t)a * cp + i, c = (size_t)b * cp + j;
-      if (diag) {
-        const double v = bl.Us[((size_t)a * cp + i) * cp + j] - acc;
-        Z[r + c * N] = v;
-        Z[c + r * N] = v;
-      } else {
-        const double v = -acc;
-        Z[r + c * N] = v;
-        Z[c + r * N] = v;
-      }
+      const double v = diag ? bl.Us[((size_t)a * cp + i) * cp + j] - acc : -acc;
+      Z[r + c * N] = v;
+      Z[c + r * N] = v;
     }
   }
 }
@@ -145,20 +152,20 @@ hipError_t launch_cam_pair_start(const Structure &s, const int *pofs, const int 
 }
 
 hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base, const int *pofs,
-                            unsigned int *keys, int *vals, int *o1s, int *o2s, hipStream_t st) {
+                            unsigned int *keys, unsigned long long *vals, hipStream_t st) {
   if (t1 <= t0) return hipSuccess;
   hipLaunchKernelGGL(k_pair_emit, dim3(cdiv(t1 - t0, 256)), dim3(256), 0, st, s, t0, t1, a0, base,
-                     pofs, keys, vals, o1s, o2s);
+                     pofs, keys, vals);
   return hipGetLastError();
 }
 
 hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
                                const unsigned int *ukeys, const int *counts, const int *offs,
-                               const int *nruns, const int *order, const int *o1s,
-                               const int *o2s, hipStream_t st) {
+                               const int *nruns, const unsigned long long *pairs,
+                               hipStream_t st) {
   if (s.cp > 9) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_schur_reduce, dim3(SR_GRID), dim3(SR_THREADS), 0, st, s, b, Z, a0, ukeys,
-                     counts, offs, nruns, order, o1s, o2s);
+                     counts, offs, nruns, pairs);
   return hipGetLastError();
 }
 

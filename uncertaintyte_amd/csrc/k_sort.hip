@@ -64,20 +64,20 @@ hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in
 }
 
 // ---- pair grouping for the Schur complement (k_schur.hip)
-// stable sort of (block key, pair index); then runs of equal keys; exclusive
+// stable sort of (block key, packed pair); then runs of equal keys; exclusive
 // scans of int counts.  Temp sizes are for n elements.
 size_t pair_sort_temp_bytes(int n, int key_bits) {
   size_t b = 0;
   if (rocprim::radix_sort_pairs(nullptr, b, (unsigned int *)nullptr, (unsigned int *)nullptr,
-                                (int *)nullptr, (int *)nullptr, (unsigned int)n, 0, key_bits) !=
-      hipSuccess)
+                                (unsigned long long *)nullptr, (unsigned long long *)nullptr,
+                                (unsigned int)n, 0, key_bits) != hipSuccess)
     return 0;
   return b;
 }
 
-hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out, const int *vals_in,
-                     int *vals_out, int n, int key_bits, void *temp, size_t temp_bytes,
-                     hipStream_t st) {
+hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out,
+                     const unsigned long long *vals_in, unsigned long long *vals_out, int n,
+                     int key_bits, void *temp, size_t temp_bytes, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   size_t tb = temp_bytes;
   return rocprim::radix_sort_pairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (unsigned int)n,

@@ -45,7 +45,7 @@ struct Sizes {
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
-  B_ERR, B_SCAN, B_PCNT, B_POFS, B_CPS, B_PK0, B_PK1, B_PV0, B_PV1, B_PO1, B_PO2, B_RUNK, B_RUNC,
+  B_ERR, B_SCAN, B_PCNT, B_POFS, B_CPS, B_PK0, B_PK1, B_PV0, B_PV1, B_RUNK, B_RUNC,
   B_RUNO, B_NRUN, B_PTMP, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
   B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
 };
@@ -83,7 +83,8 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_PCNT] = sz[B_POFS] = sizeof(int) * (size_t)z.nobs;
   sz[B_CPS] = sizeof(int) * ((size_t)z.nc + 1);
   sz[B_PK0] = sz[B_PK1] = sz[B_RUNK] = sizeof(unsigned int) * cap;
-  sz[B_PV0] = sz[B_PV1] = sz[B_PO1] = sz[B_PO2] = sz[B_RUNC] = sz[B_RUNO] = sizeof(int) * cap;
+  sz[B_PV0] = sz[B_PV1] = sizeof(unsigned long long) * cap;
+  sz[B_RUNC] = sz[B_RUNO] = sizeof(int) * cap;
   sz[B_NRUN] = sizeof(int);
   sz[B_PTMP] = pair_temp_bytes(z.nobs);
   sz[B_SCAN] = sizeof(int) * ((size_t)std::max(z.npc, z.nc) / 1024 + 2);
@@ -372,8 +373,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   {
     const long long cap = (long long)pair_capacity(z.nobs);
     unsigned int *k0 = (unsigned int *)ptr(B_PK0), *k1 = (unsigned int *)ptr(B_PK1);
-    int *v0 = (int *)ptr(B_PV0), *v1 = (int *)ptr(B_PV1);
-    int *o1s = (int *)ptr(B_PO1), *o2s = (int *)ptr(B_PO2);
+    unsigned long long *v0 = (unsigned long long *)ptr(B_PV0), *v1 = (unsigned long long *)ptr(B_PV1);
     unsigned int *rk = (unsigned int *)ptr(B_RUNK);
     int *rc_ = (int *)ptr(B_RUNC), *ro = (int *)ptr(B_RUNO), *nrun = (int *)ptr(B_NRUN);
     int a0 = 0;
@@ -385,11 +385,11 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
       if (np_ > 0) {
         int kb = 1;
         while (kb < 32 && ((unsigned long long)1 << kb) < (unsigned long long)(a1 - a0) * z.nc) ++kb;
-        HC(launch_pair_emit(S, h_off[a0], h_off[a1], a0, h_cps[a0], pofs, k0, v0, o1s, o2s, s));
+        HC(launch_pair_emit(S, h_off[a0], h_off[a1], a0, h_cps[a0], pofs, k0, v0, s));
         HC(pair_sort(k0, k1, v0, v1, np_, kb, ptmp, ptmp_bytes, s));
         HC(runs(k1, np_, rk, rc_, nrun, ptmp, ptmp_bytes, s));
         HC(exclusive_scan_int(rc_, ro, np_, ptmp, ptmp_bytes, s));
-        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, o1s, o2s, s));
+        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, s));
       }
       a0 = a1;
     }
