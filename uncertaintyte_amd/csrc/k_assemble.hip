@@ -258,10 +258,13 @@ __device__ inline void inv3x3_ref(const double v[9], double o[9]) {
 }
 
 // ---------------------------------------------------------------------------
-// point blocks: one thread per compact point; V_p, d_p, V_s^-1, then the
-// per-observation W_s = D_c (J_c^T J_p) D_p and Y = W_s V_s^-1
+// point blocks, two passes:
+//  * one thread per compact point: V_p over its observations in list order,
+//    d_p, V_s^-1;
+//  * one thread per observation in row order (coalesced J rows, W_s and Y
+//    streamed out): W_s = D_c (J_c^T J_p) D_p and Y = W_s V_s^-1.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_pt_assemble(Structure s, Blocks b) {
+__global__ __launch_bounds__(128) void k_pt_blocks(Structure s, Blocks b) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= s.npc) return;
   const int cp = s.cp, rl = cp + 3, N = s.N;
@@ -306,35 +309,73 @@ __global__ __launch_bounds__(128) void k_pt_assemble(Structure s, Blocks b) {
   }
   if (bad) atomicOr(s.err, ERRBIT_SINGULAR_V);
   b.pt_has[p] = 1;
-  for (int t = t0; t < t1; ++t) {
-    int o = s.pt_list[t];
-    int a = s.obs_cam[o];
-    const double *r0 = s.vals + (size_t)o * 2 * rl;
-    const double *r1 = r0 + rl;
-    const double *dc = b.d + a * cp;
-    double *Ws = b.Ws + (size_t)o * cp * 3;
-    double *Y = b.Y + (size_t)o * cp * 3;
-    for (int i = 0; i < cp; ++i) {
-      double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
-      double w[3];
-      for (int k = 0; k < 3; ++k) {
-        double wv = fma(ci1, r1[cp + k], ci0 * r0[cp + k]);
-        w[k] = (wv * dci) * dd[k];
-        Ws[i * 3 + k] = w[k];
-      }
-      for (int m = 0; m < 3; ++m) {
-        double y = w[0] * iv[0 * 3 + m];
-        y = fma(w[1], iv[1 * 3 + m], y);
-        y = fma(w[2], iv[2 * 3 + m], y);
-        Y[i * 3 + m] = y;
-      }
+}
+
+template <int CP>
+__global__ __launch_bounds__(256) void k_obs_blocks(Structure s, Blocks b) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= s.nobs) return;
+  const int a = s.obs_cam[o], p = s.obs_pt[o];
+  if (a < 0 || p < 0) return;
+  constexpr int RL = CP + 3;
+  const double *r0 = s.vals + (size_t)o * 2 * RL;
+  const double *r1 = r0 + RL;
+  const double *dc = b.d + a * CP;
+  const double *dpp = b.d + s.N + 3 * p;
+  const double *ivp = b.Vi + (size_t)p * 9;
+  double dd[3], iv[9], jp0[3], jp1[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dd[k] = dpp[k];
+    jp0[k] = r0[CP + k];
+    jp1[k] = r1[CP + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) iv[k] = ivp[k];
+  double *Ws = b.Ws + (size_t)o * CP * 3;
+  double *Y = b.Y + (size_t)o * CP * 3;
+#pragma unroll
+  for (int i = 0; i < CP; ++i) {
+    const double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
+    double w[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double wv = fma(ci1, jp1[k], ci0 * jp0[k]);
+      w[k] = (wv * dci) * dd[k];
+      Ws[i * 3 + k] = w[k];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double y = w[0] * iv[0 * 3 + m];
+      y = fma(w[1], iv[1 * 3 + m], y);
+      y = fma(w[2], iv[2 * 3 + m], y);
+      Y[i * 3 + m] = y;
     }
   }
 }
 
+template <int CP>
+static void launch_obs_blocks_cp(const Structure &s, const Blocks &b, hipStream_t st) {
+  hipLaunchKernelGGL(k_obs_blocks<CP>, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s, b);
+}
+
 hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st) {
   hipLaunchKernelGGL(k_cam_assemble, dim3(s.nc), dim3(64), 0, st, s, b);
-  if (s.npc > 0) hipLaunchKernelGGL(k_pt_assemble, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b);
+  if (s.npc > 0) hipLaunchKernelGGL(k_pt_blocks, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b);
+  if (s.nobs > 0) {
+    switch (s.cp) {
+      case 1: launch_obs_blocks_cp<1>(s, b, st); break;
+      case 2: launch_obs_blocks_cp<2>(s, b, st); break;
+      case 3: launch_obs_blocks_cp<3>(s, b, st); break;
+      case 4: launch_obs_blocks_cp<4>(s, b, st); break;
+      case 5: launch_obs_blocks_cp<5>(s, b, st); break;
+      case 6: launch_obs_blocks_cp<6>(s, b, st); break;
+      case 7: launch_obs_blocks_cp<7>(s, b, st); break;
+      case 8: launch_obs_blocks_cp<8>(s, b, st); break;
+      case 9: launch_obs_blocks_cp<9>(s, b, st); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
   return hipGetLastError();
 }
 

@@ -91,7 +91,9 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_US] = sz[B_XD] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
   sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
   sz[B_VI] = sizeof(double) * 9 * (size_t)z.npc;
-  sz[B_WS] = sz[B_Y] = sizeof(double) * 3 * (size_t)z.cp * z.nobs;
+  sz[B_WS] = sizeof(double) * 3 * (size_t)z.cp * z.nobs;
+  // Y, then the point step's per-observation 3x3 terms
+  sz[B_Y] = sizeof(double) * (size_t)std::max(3 * z.cp, 9) * z.nobs;
   sz[B_CTRL] = sizeof(TeCtrl);
   sz[B_P] = sizeof(double) * GJ_BLOCK * GJ_BLOCK;
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
