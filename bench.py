@@ -105,6 +105,49 @@ def measure_zplus(u, scene, torch, dev, check=True):
     return out
 
 
+def measure_large(u, scene, torch, dev, fp64=False, check=True):
+    """walk4000 (4000 cameras, 3M points, track Poisson(6) min 2; N ~ 36000):
+    the Z+ path, optionally the plain FP64 TE path, and the Z+ camera error on
+    200 sampled cameras against the oracle."""
+    import numpy as np
+
+    def log(msg):
+        print(f"[large] {msg}", file=sys.stderr, flush=True)
+    t0 = time.time()
+    J = scene.make_workload("walk4000")
+    log(f"scene generated ({time.time() - t0:.0f} s)")
+    out = {"workload": "walk4000: 4000 cameras on a random walk, 3M points, track Poisson(6) min 2",
+           "N": J.num_cams * J.cam_params, "observations": J.num_obs,
+           "scene_gen_s": round(time.time() - t0, 1),
+           "workspace_GiB": round(u.workspace_bytes(u.make_options(J, u.PREC_MIXED), J.num_rows)
+                                  / 2 ** 30, 2)}
+    zcall, zctx, zcam, _ = _device_solver(u, torch, dev, J, u.PREC_MIXED)
+    zcall()
+    z_ms, zst = _timed(zcall, torch, 2)
+    zcam_h = zcam.cpu().numpy().reshape(J.num_cams, -1)
+    zctx.close()
+    log(f"Z+ path {z_ms:.0f} ms per solve")
+    out.update(zplus_ms=round(z_ms, 1), zplus_stages_ms={k: round(float(zst.stage_ms[i]), 2)
+               for i, k in enumerate(["upload", "decode", "point_lists", "assemble", "r", "schur",
+                                      "inverse", "taylor", "cam_out", "point_out", "d", "total"])
+               if zst.stage_ms[i] > 0})
+    if fp64:
+        fcall, fctx, _, _ = _device_solver(u, torch, dev, J, u.PREC_FP64)
+        f_ms, fst = _timed(fcall, torch, 1)
+        fctx.close()
+        log(f"plain FP64 path {f_ms:.0f} ms")
+        out.update(fp64_te_ms=round(f_ms, 1), speedup=round(f_ms / z_ms, 2),
+                   fp64_taylor_terms=fst.num_terms)
+    if check:
+        import oracle
+        cams = list(range(0, J.num_cams, 20))
+        ref, _ = oracle.zplus_sampled(J, cams, nthreads=min(os.cpu_count() or 1, 16), log=log)
+        rel = np.linalg.norm(zcam_h[cams] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out.update(cam_err_max=float(rel.max()), cam_err_median=float(np.median(rel)),
+                   err_sample=f"{len(cams)} cameras vs D pinv_7(Z_s) D (ARPACK + LAPACK oracle)")
+    return out
+
+
 def measure_serving(u, rate=150.0, n_requests=512):
     from uncertaintyte_amd import serve
     scenes, order = serve.make_requests(n_requests, 64, workers=min(os.cpu_count() or 1, 16))
@@ -142,6 +185,10 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true", help="skip the Z+ accuracy check")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--rate", type=float, default=150.0, help="serving arrival rate (1/s)")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 4000-camera / 3M-point scene (N = 36000), minutes")
+    ap.add_argument("--large-fp64", action="store_true",
+                    help="with --large, also time the plain FP64 TE path on it (about a minute)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the pipeline is single-GPU",
@@ -219,6 +266,8 @@ def main() -> int:
     # otherwise compete with the request threads
     serving = None if args.no_serving else measure_serving(u, rate=args.rate)
     zplus = None if args.no_zplus else measure_zplus(u, scene, torch, dev, check=not args.no_check)
+    large = measure_large(u, scene, torch, dev, fp64=args.large_fp64,
+                          check=not args.no_check) if args.large else None
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
@@ -241,6 +290,7 @@ def main() -> int:
         "scene_gen_s": round(gen_s, 2),
         "zplus": zplus,
         "serving": serving,
+        "large": large,
     }
     print(json.dumps(line), flush=True)
     return 0

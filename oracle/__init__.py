@@ -259,30 +259,36 @@ def zplus_reference(jac, fix=None, nthreads: int = 0):
     return dict(cam=cam, pt=pt.reshape(-1, 6), X=X, eig=w)
 
 
-def zplus_sampled(jac, cams, fix=None, nthreads: int = 0):
+def zplus_sampled(jac, cams, fix=None, nthreads: int = 0, log=None):
     """Camera outputs of D pinv_7(Z_s) D for the listed cameras only, for scenes
     too large for a dense eigen-decomposition.  Uses the identity
     pinv_7(Z) = (Z + N N^T)^-1 - N N^T with N the eigenvectors of the 7 smallest
     eigenvalues, found by shift-invert Lanczos (ARPACK) on a LAPACK Cholesky of
     Z + sigma I; the sampled columns come from a Cholesky solve with Z + N N^T.
-    Returns (cam rows for `cams`, the 7 smallest eigenvalues)."""
+    Returns (cam rows for `cams`, the 7 smallest eigenvalues).  `log(msg)` is
+    called between the phases (long scenes take minutes)."""
     import scipy.linalg as sla
     from scipy.sparse.linalg import LinearOperator, eigsh
+    log = log or (lambda msg: None)
     Z, d = front(jac, fix=fix, nthreads=nthreads)
+    log("oracle: Schur complement done")
     n = Z.shape[0]
     cp = jac.cam_params
     sigma = 1e-10 * np.trace(Z) / n
     A = np.array(Z, order="F")
     A[np.diag_indices(n)] += sigma
     cf = sla.cho_factor(A, lower=True, overwrite_a=True, check_finite=False)
+    log("oracle: shifted Cholesky done")
     op = LinearOperator((n, n), matvec=lambda v: sla.cho_solve(cf, v, check_finite=False),
                         dtype=np.float64)
     w, Nb = eigsh(Z, k=7, sigma=-sigma, OPinv=op, which="LM", tol=1e-14,
                   v0=np.linspace(1.0, 2.0, n))
     del cf, A
+    log("oracle: gauge basis done")
     G = np.array(Z, order="F")
     G += Nb @ Nb.T
     cg = sla.cho_factor(G, lower=True, overwrite_a=True, check_finite=False)
+    log("oracle: gauge-fixed Cholesky done")
     idx = np.concatenate([np.arange(a * cp, (a + 1) * cp) for a in cams])
     E = np.zeros((n, len(idx)))
     E[idx, np.arange(len(idx))] = 1.0

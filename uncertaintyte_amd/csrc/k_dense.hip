@@ -271,6 +271,182 @@ __global__ __launch_bounds__(G_THREADS, 1) void k_gemm_f64_p(int M, int N, int K
   }
 }
 
+// 8-wave variant: 128x128 tile, waves of 64x32 (two waves per SIMD), K slabs
+// of BK staged through LDS with the next slab prefetched into registers.
+template <int BK>
+__global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, double alpha,
+                                                      const double *__restrict__ A, int lda,
+                                                      const double *__restrict__ B, int ldb,
+                                                      double beta, double *__restrict__ C,
+                                                      int ldc) {
+  __shared__ double As[BK][GB_M + G_PAD];
+  __shared__ double Bs[BK][GB_N + G_PAD];
+  constexpr int T = 512;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
+  const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
+  const int bid = blockIdx.x;
+  const int group = 8;
+  const int per_group = group * tiles_n;
+  const int first_m = (bid / per_group) * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int tm = first_m + (bid % per_group) % gsize;
+  const int tn = (bid % per_group) / gsize;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+  double4_t acc[4][2];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = BK * 128 / T;
+  double ra[NL], rb[NL];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      int idx = tid + T * r;
+      int k = idx >> 7, m = idx & 127;
+      int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      int n = idx / BK, kk = idx % BK;
+      int gn = n0 + n, gk2 = k0 + kk;
+      rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+    }
+  };
+  load_tiles(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      int idx = tid + T * r;
+      As[idx >> 7][idx & 127] = ra[r];
+      Bs[idx % BK][idx / BK] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + BK < K) load_tiles(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[4], bf[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+        if (row < M && col < N) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+}
+
+// 8-wave variant with conflict-free LDS rows (stride 144 doubles: rows shift by
+// 32 banks) and an interior fast path: tiles fully inside M x N load full K
+// slabs without per-element bounds checks.
+template <int BK>
+__global__ __launch_bounds__(512, 1) void k_gemm_f64w8p(int M, int N, int K, double alpha,
+                                                       const double *__restrict__ A, int lda,
+                                                       const double *__restrict__ B, int ldb,
+                                                       double beta, double *__restrict__ C,
+                                                       int ldc) {
+  constexpr int LS = 144;
+  __shared__ double As[BK][LS];
+  __shared__ double Bs[BK][LS];
+  constexpr int T = 512;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
+  const int bid = blockIdx.x;
+  const int group = 8;
+  const int per_group = group * tiles_n;
+  const int first_m = (bid / per_group) * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int tm = first_m + (bid % per_group) % gsize;
+  const int tn = (bid % per_group) / gsize;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+  const bool interior = (m0 + GB_M <= M) && (n0 + GB_N <= N);
+  double4_t acc[4][2];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = BK * 128 / T;
+  double ra[NL], rb[NL];
+  auto load_tiles = [&](int k0) {
+    if (interior && k0 + BK <= K) {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int idx = tid + T * r;
+        ra[r] = A[(size_t)(m0 + (idx & 127)) + (size_t)(k0 + (idx >> 7)) * lda];
+        rb[r] = B[(size_t)(k0 + idx % BK) + (size_t)(n0 + idx / BK) * ldb];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int idx = tid + T * r;
+        const int gm = m0 + (idx & 127), gk = k0 + (idx >> 7);
+        ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+        const int gn = n0 + idx / BK, gk2 = k0 + idx % BK;
+        rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      }
+    }
+  };
+  load_tiles(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + T * r;
+      As[idx >> 7][idx & 127] = ra[r];
+      Bs[idx % BK][idx / BK] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + BK < K) load_tiles(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[4], bf[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+        if (row < M && col < N) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+}
+
 static int g_gemm_variant = 0;
 
 void set_gemm_variant(int v) { g_gemm_variant = v; }
@@ -301,6 +477,26 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                          lda, B, ldb, beta, C, ldc, absmax, active);
       break;
     }
+    case 4:
+      if (absmax || active) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_gemm_f64w8<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
+                         ldb, beta, C, ldc);
+      break;
+    case 5:
+      if (absmax || active) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_gemm_f64w8<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
+                         ldb, beta, C, ldc);
+      break;
+    case 6:
+      if (absmax || active) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_gemm_f64w8p<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
+                         B, ldb, beta, C, ldc);
+      break;
+    case 7:
+      if (absmax || active) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_gemm_f64w8p<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
+                         B, ldb, beta, C, ldc);
+      break;
     default:  // production: K slabs of 32
       hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
                          A, lda, B, ldb, beta, C, ldc, absmax, active, 0);

@@ -808,7 +808,7 @@ __global__ void k_fill_pattern(size_t n, double *A, double scale) {
 }
 
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 7) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
