@@ -19,6 +19,8 @@ namespace unc {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr int GB_M = 128, GB_N = 128, G_THREADS = 256;
+// products with fewer 128 x 128 tiles than this take the 64 x 64 kernel
+constexpr int SMALL_GEMM_TILES = 96;
 constexpr int G_PAD = 4;
 
 template <int GB_K, int MINB, bool TRANS_B = false>
@@ -456,6 +458,8 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                     const int *active, hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
   const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
+  if (g_gemm_variant == 0 && !absmax && !active && tiles < SMALL_GEMM_TILES)
+    return gemm_f64_ex(M, N, K, alpha, A, lda, B, ldb, false, beta, C, ldc, 0, st);
   switch (g_gemm_variant) {
     case 1:
       hipLaunchKernelGGL((k_gemm_f64<16, 2>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
@@ -504,6 +508,96 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
   return hipGetLastError();
 }
 
+// Small-tile variant for the latency-bound products at the bottom of the
+// Cholesky recursion and the panel products of the Gauss-Jordan inverse:
+// 64 x 64 tiles, 4 waves of 32 x 32, K slabs of 32.  Same operand flags.
+constexpr int GS_M = 64, GS_N = 64, GS_K = 32;
+
+template <bool TRANS_B>
+__global__ __launch_bounds__(256) void k_gemm_f64s(int M, int N, int K, double alpha,
+                                                  const double *__restrict__ A, int lda,
+                                                  const double *__restrict__ B, int ldb,
+                                                  double beta, double *__restrict__ C, int ldc,
+                                                  int tri) {
+  __shared__ double As[GS_K][GS_M + G_PAD];
+  __shared__ double Bs[GS_K][GS_N + G_PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int tiles_m = (M + GS_M - 1) / GS_M;
+  const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+  const int m0 = tm * GS_M, n0 = tn * GS_N;
+  if ((tri & GEMM_C_LOWER) && n0 >= m0 + GS_M) return;
+  int kbeg = 0, kend = K;
+  if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GS_M);
+  if (tri & GEMM_B_LOWER) kbeg = n0;
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GS_N);
+  double4_t acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = GS_K * 64 / 256;
+  double ra[NL], rb[NL];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + 256 * r;
+      const int k = idx >> 6, m = idx & 63;
+      const int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < kend) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      if (TRANS_B) {
+        const int gn = n0 + m;
+        rb[r] = (gn < N && gk < kend) ? B[(size_t)gn + (size_t)gk * ldb] : 0.0;
+      } else {
+        const int n = idx / GS_K, kk = idx % GS_K;
+        const int gn = n0 + n, gk2 = k0 + kk;
+        rb[r] = (gn < N && gk2 < kend) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      }
+    }
+  };
+  load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GS_K) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + 256 * r;
+      As[idx >> 6][idx & 63] = ra[r];
+      if (TRANS_B) Bs[idx >> 6][idx & 63] = rb[r];
+      else Bs[idx % GS_K][idx / GS_K] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + GS_K < kend) load_tiles(k0 + GS_K);
+#pragma unroll
+    for (int ks = 0; ks < GS_K; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = As[kr][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+        if (row < M && col < N) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+}
+
 // C = alpha A op(B) + beta C with op(B) = B or B^T and the triangular flags
 // GEMM_* (lower-triangular A / B / B^T operands, lower-only C tiles)
 hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
@@ -511,6 +605,16 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
                        hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
   const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
+  if (tiles < SMALL_GEMM_TILES) {
+    const int stiles = (int)cdiv(M, GS_M) * (int)cdiv(N, GS_N);
+    if (trans_b)
+      hipLaunchKernelGGL((k_gemm_f64s<true>), dim3(stiles), dim3(256), 0, st, M, N, K, alpha, A, lda,
+                         B, ldb, beta, C, ldc, tri);
+    else
+      hipLaunchKernelGGL((k_gemm_f64s<false>), dim3(stiles), dim3(256), 0, st, M, N, K, alpha, A,
+                         lda, B, ldb, beta, C, ldc, tri);
+    return hipGetLastError();
+  }
   if (trans_b)
     hipLaunchKernelGGL((k_gemm_f64<32, 1, true>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K,
                        alpha, A, lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
