@@ -114,6 +114,7 @@ def lib():
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
     L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
+    L.unc_diag_leaf_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.unc_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
     L.unc_server_default_config.restype = None
     L.unc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]

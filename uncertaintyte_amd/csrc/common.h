@@ -118,6 +118,7 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // gauge-fixed Z^+ path (k_zplus.hip)
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st);
+void set_leaf_variant(int v);  // 0: unrolled registers (default), 1: compact
 hipError_t skinny_tn(int n, int kc, const double *A, int lda, const double *B, double *C,
                      bool lower, hipStream_t st);
 hipError_t skinny_nn(int n, int kc, const double *A, int lda, const double *B, double *C,

@@ -902,6 +902,41 @@ int unc_diag_chol_inverse(int n, const double *A, double *L, double *X) {
   return herr ? map_err(herr) : UNC_OK;
 }
 
+int unc_diag_leaf_bench(int reps, int variant, double *avg_us) {
+  if (reps <= 0 || !avg_us || variant < 0 || variant > 1) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int n = 64;
+  std::vector<double> h((size_t)n * n);
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < n; ++r) h[r + (size_t)c * n] = (r == c) ? n + 1.0 : 1.0 / (1.0 + r + c);
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * n * n + 256));
+  double *dA = (double *)buf, *dW = dA + n * n, *dX = dW + n * n;
+  unsigned int *err = (unsigned int *)(dX + n * n);
+  HC(hipMemcpyAsync(dA, h.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  set_leaf_variant(variant);
+  hipEvent_t e0 = ctx->stage_ev[0], e1 = ctx->stage_ev[1];
+  HC(hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  HC(chol_inv_leaf(n, dW, n, dX, n, err, s));
+  HC(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) {
+    HC(hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    HC(chol_inv_leaf(n, dW, n, dX, n, err, s));
+  }
+  HC(hipEventRecord(e1, s));
+  HC(hipStreamSynchronize(s));
+  set_leaf_variant(0);
+  *avg_us = 1e3 * ev_ms(e0, e1) / reps;
+  HC(hipFree(buf));
+  return UNC_OK;
+}
+
 int unc_diag_residual_dd(int n, const double *A, const double *X, double *R) {
   if (n <= 0) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
