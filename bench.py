@@ -19,6 +19,8 @@ Two further measurements ride on the same line:
            Poisson(5) min 2, 10% short-baseline points; N = 13491), both with
            the Jacobian resident in HBM, plus the camera-block error of the Z+
            output against D pinv_7(Z_s) D on 188 sampled cameras (oracle);
+  accessors the eigen accessors on the device: 8M point blocks (3x3 values
+           + vectors, HBM-bound) and 4000 camera blocks (9x9 values);
   serving  512 requests (cameras log-uniform 20..800, points per camera
            25..400, 64 distinct seeded scenes) arriving as a Poisson process
            into the request server (4 slots), p50/p99 latency and solves/s.
@@ -148,6 +150,60 @@ def measure_large(u, scene, torch, dev, fp64=False, check=True):
     return out
 
 
+def measure_accessors(u, torch, dev, n_points=8_000_000, n_cams=4000, reps=20):
+    """Eigen accessors (cov::Uncertainty, uncertainty.cpp:74-185) on device-
+    resident packed outputs: 3x3 eigenvalues + eigenvectors of n_points point
+    blocks (48 B in, 24 + 72 B out per point) and 9x9 eigenvalues + centre
+    eigenvectors of n_cams camera blocks.  HIP events on the launch stream."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    # SPD blocks B B^T + 1e-3 I, packed (xx, xy, xz, yy, yz, zz); elementwise
+    # products (no batched GEMM) keep the set-up simple
+    B = torch.randn((n_points, 3, 3), dtype=torch.float64, device=dev, generator=g)
+    pk = torch.stack([(B[:, i, :] * B[:, j, :]).sum(-1) + (1e-3 if i == j else 0.0)
+                      for i in range(3) for j in range(i, 3)], 1).contiguous()
+    del B
+    vals = torch.empty((n_points, 3), dtype=torch.float64, device=dev)
+    vecs = torch.empty((n_points, 9), dtype=torch.float64, device=dev)
+    C = torch.randn((n_cams, 9, 12), dtype=torch.float64, device=dev, generator=g)
+    ck = torch.stack([(C[:, i, :] * C[:, j, :]).sum(-1) for i in range(9) for j in range(i, 9)],
+                     1).contiguous()
+    cvals = torch.empty((n_cams, 9), dtype=torch.float64, device=dev)
+    cvecs = torch.empty((n_cams, 9), dtype=torch.float64, device=dev)
+    # a stream of its own (torch's default stream is the null stream, handle 0,
+    # which the library reads as "the context's stream"); events on it see the
+    # kernels
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    pcall = lambda: u.eig_points_device(n_points, pk.data_ptr(), vals.data_ptr(), vecs.data_ptr(),
+                                        stream=s.cuda_stream)
+    ccall = lambda: u.eig_cameras_device(n_cams, 9, ck.data_ptr(), cvals.data_ptr(),
+                                         cvecs.data_ptr(), stream=s.cuda_stream)
+    out = {}
+    for name, call in (("points", pcall), ("cameras", ccall)):
+        call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            call()
+        e1.record(s)
+        torch.cuda.synchronize()
+        out[name + "_ms"] = e0.elapsed_time(e1) / reps
+    bytes_pt = 144 * n_points
+    gbs = bytes_pt / (out["points_ms"] * 1e-3) / 1e9
+    res = {"kernel": "k_eig_points<true,true> (3x3 cyclic Jacobi, one lane per point)",
+           "points": n_points, "points_ms": round(out["points_ms"], 4),
+           "points_per_s": round(n_points / (out["points_ms"] * 1e-3), 0),
+           "algorithmic_bytes_per_point": 144, "achieved_GBps": round(gbs, 1),
+           "peak_GBps": PEAK_HBM_GBS, "frac": round(gbs / PEAK_HBM_GBS, 4),
+           "cameras": n_cams, "cameras_ms": round(out["cameras_ms"], 4),
+           "camera_kernel": "k_eig_cameras<9> (9x9 Jacobi eigenvalues + 3x3 centre vectors)"}
+    del pk, vals, vecs, C, ck, cvals, cvecs
+    torch.cuda.empty_cache()
+    return res
+
+
 def measure_serving(u, rate=150.0, n_requests=512):
     from uncertaintyte_amd import serve
     scenes, order = serve.make_requests(n_requests, 64, workers=min(os.cpu_count() or 1, 16))
@@ -184,6 +240,7 @@ def main() -> int:
     ap.add_argument("--no-zplus", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the Z+ accuracy check")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-accessors", action="store_true")
     ap.add_argument("--rate", type=float, default=150.0, help="serving arrival rate (1/s)")
     ap.add_argument("--large", action="store_true",
                     help="also run the 4000-camera / 3M-point scene (N = 36000), minutes")
@@ -264,6 +321,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     # serving before the Z+ accuracy check: the CPU oracle's BLAS threads would
     # otherwise compete with the request threads
+    accessors = None if args.no_accessors else measure_accessors(u, torch, dev)
     serving = None if args.no_serving else measure_serving(u, rate=args.rate)
     zplus = None if args.no_zplus else measure_zplus(u, scene, torch, dev, check=not args.no_check)
     large = measure_large(u, scene, torch, dev, fp64=args.large_fp64,
@@ -289,6 +347,7 @@ def main() -> int:
                 "peak_GBps": PEAK_HBM_GBS},
         "scene_gen_s": round(gen_s, 2),
         "zplus": zplus,
+        "accessors": accessors,
         "serving": serving,
         "large": large,
     }

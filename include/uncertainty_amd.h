@@ -51,8 +51,9 @@ extern "C" {
 
 /* arithmetic of the Z inverse (the only stage whose precision is selectable) */
 #define UNC_PREC_FP64 0        /* plain FP64 TE path: reference output, <=1e-9 per block  */
-#define UNC_PREC_MIXED 1       /* gauge-fixed Z^+, FP32 MFMA inverses + FP64 residuals,
-                                  <=1e-5 per block vs the FP64 pseudo-inverse          */
+#define UNC_PREC_MIXED 1       /* gauge-fixed Z^+ (contract: <=1e-5 per block vs the FP64
+                                  pseudo-inverse); computed by the FP64 Z^+ kernels, as
+                                  FP32 inverses fail on Z's condition (1e7-1e9)        */
 #define UNC_PREC_ZPLUS_FP64 2  /* gauge-fixed Z^+ with FP64 inverses (the FP64 Z^+)     */
 /* Output semantics: UNC_PREC_FP64 reproduces the reference TAYLOR output
  * (including its (X[j,k] + X[k,j]/2) camera packing).  The two Z^+ modes return
@@ -74,8 +75,8 @@ typedef struct unc_options {
   int precision;      /* UNC_PREC_*                                                   */
   int refine;         /* Newton steps on the inverse; -1 = automatic (default):
                          FP64 path 1 step with a compensated residual on
-                         (Z + lambda I)^-1, MIXED 2 steps and ZPLUS_FP64 1 step on
-                         (Z + N N^T)^-1; 0 disables refinement                    */
+                         (Z + lambda I)^-1; Z^+ modes 1 step on the null-space
+                         basis N; 0 disables refinement                           */
   int device;         /* HIP device ordinal                                           */
   int flags;          /* reserved, 0                                                  */
 } unc_options;
@@ -142,6 +143,34 @@ int unc_compute_debug(unc_context *ctx, const unc_options *opts, int num_rows,
 
 /* Device bytes a solve of this size needs (admission control). */
 unsigned long long unc_workspace_bytes(const unc_options *opts, int num_rows);
+
+/* ---- eigen accessors of cov::Uncertainty on the device
+ * (src/uncertaintyTE/uncertainty.cpp:74-115 and :146-185, declared at
+ * src/uncertaintyTE/uncertainty.h:110-142).  Inputs are the packed outputs
+ * above: cp(cp+1)/2 values per camera, 6 per point.  Any output may be NULL.
+ *   eigvals      cp per camera / 3 per point, descending, as
+ *                getCamerasUncEigenValues / getPointsUncEigenValues return them
+ *                (the unpacked blocks are symmetric, so the real parts the
+ *                reference keeps are the eigenvalues)
+ *   center_vecs  9 per camera: eigenvectors of the camera-centre block, rows
+ *                and columns 3..5 (getCamerasCentersUncEigVec); needs cp >= 6
+ *   eigvecs      9 per point (getPointsUncEigVectors)
+ * Vector k of a block sits at [3k, 3k+3) and has unit length.  Order and sign
+ * differ from the reference, whose order and signs are whatever Eigen's
+ * EigenSolver produces: here the vectors follow the descending eigenvalues and
+ * each has its largest-magnitude component positive.
+ * The host variants copy through ctx's device (ctx NULL: device 0) and return
+ * with the results in place.  The _device variants take device pointers,
+ * enqueue on stream (NULL: ctx's stream) and return without waiting. */
+int unc_eig_cameras(unc_context *ctx, int num_cams, int cam_params, const double *cam_unc,
+                    double *eigvals, double *center_vecs);
+int unc_eig_points(unc_context *ctx, int num_points, const double *pt_unc, double *eigvals,
+                   double *eigvecs);
+int unc_eig_cameras_device(unc_context *ctx, int num_cams, int cam_params,
+                           const double *d_cam_unc, double *d_eigvals, double *d_center_vecs,
+                           void *stream);
+int unc_eig_points_device(unc_context *ctx, int num_points, const double *d_pt_unc,
+                          double *d_eigvals, double *d_eigvecs, void *stream);
 
 /* Kernel-level diagnostics on device 0 (host pointers, column-major); they
  * pin single kernels in the parity tests and have no reference counterpart.

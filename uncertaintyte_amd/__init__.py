@@ -129,6 +129,10 @@ def lib():
     L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
+    L.unc_eig_cameras.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+    L.unc_eig_points.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    L.unc_eig_cameras_device.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    L.unc_eig_points_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -321,15 +325,27 @@ def diag_residual_dd(A: np.ndarray, X: np.ndarray) -> np.ndarray:
 # --------------------------------------------------------------------------
 
 class Uncertainty:
-    """Output container with the reference's unpack and eigen accessors."""
+    """Output container with the reference's unpack and eigen accessors
+    (cov::Uncertainty, src/uncertaintyTE/uncertainty.h:84-143).  The unpack
+    accessors are host-side index work, as in the reference; the eigen
+    accessors run on the device (unc_eig_cameras / unc_eig_points) and raise
+    UncError when no GPU is present."""
 
-    def __init__(self, cam_params: int, cam_unc: np.ndarray, pt_unc: np.ndarray):
+    def __init__(self, cam_params: int, cam_unc: np.ndarray, pt_unc: np.ndarray, ctx=None):
         self.cam_params = cam_params
         self.per = cam_params * (cam_params + 1) // 2
         self.cam = np.asarray(cam_unc, np.float64).reshape(-1, self.per)
         self.pt = np.asarray(pt_unc, np.float64).reshape(-1, 6)
+        self.ctx = ctx
+
+    def getCamerasUncRaw(self) -> np.ndarray:
+        return self.cam.reshape(-1).copy()
+
+    def getPointsUncRaw(self) -> np.ndarray:
+        return self.pt.reshape(-1).copy()
 
     def getCameraUncMatrix(self, i: int) -> np.ndarray:
+        # uncertainty.cpp:48-61: row-major upper triangle mirrored
         cp = self.cam_params
         M = np.zeros((cp, cp))
         iu = np.triu_indices(cp)
@@ -338,23 +354,87 @@ class Uncertainty:
         return M
 
     def getCamerasUncMatrices(self) -> np.ndarray:
-        return np.stack([self.getCameraUncMatrix(i) for i in range(len(self.cam))])
+        cp = self.cam_params
+        M = np.zeros((len(self.cam), cp, cp))
+        iu = np.triu_indices(cp)
+        M[:, iu[0], iu[1]] = self.cam
+        M[:, iu[1], iu[0]] = self.cam
+        return M
 
     def getPointUncMatrix(self, i: int) -> np.ndarray:
+        # uncertainty.cpp:120-134
         a = self.pt[i]
         return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])
 
     def getPointsUncMatrices(self) -> np.ndarray:
-        return np.stack([self.getPointUncMatrix(i) for i in range(len(self.pt))])
+        a = self.pt
+        return np.stack([a[:, 0], a[:, 1], a[:, 2], a[:, 1], a[:, 3], a[:, 4], a[:, 2], a[:, 4],
+                         a[:, 5]], 1).reshape(-1, 3, 3)
 
     def getCamerasUncEigenValues(self) -> np.ndarray:
-        # uncertainty.cpp:74-89: real parts of the (general) eigenvalues, descending
-        ev = np.linalg.eigvals(self.getCamerasUncMatrices()).real
-        return -np.sort(-ev, axis=1)
+        """(num_cams, cp) eigenvalues, descending (uncertainty.cpp:74-89)."""
+        return eig_cameras(self.cam, self.cam_params, values=True, vectors=False, ctx=self.ctx)[0]
+
+    def getCamerasCentersUncEigVec(self) -> np.ndarray:
+        """(num_cams, 9): eigenvectors of the camera-centre block, vector k at
+        [3k:3k+3] (uncertainty.cpp:91-115), ordered by descending eigenvalue."""
+        return eig_cameras(self.cam, self.cam_params, values=False, vectors=True, ctx=self.ctx)[1]
 
     def getPointsUncEigenValues(self) -> np.ndarray:
-        ev = np.linalg.eigvals(self.getPointsUncMatrices()).real
-        return -np.sort(-ev, axis=1)
+        """(num_points, 3) eigenvalues, descending (uncertainty.cpp:146-162)."""
+        return eig_points(self.pt, values=True, vectors=False, ctx=self.ctx)[0]
+
+    def getPointsUncEigVectors(self) -> np.ndarray:
+        """(num_points, 9) eigenvectors, vector k at [3k:3k+3]
+        (uncertainty.cpp:164-185), ordered by descending eigenvalue."""
+        return eig_points(self.pt, values=False, vectors=True, ctx=self.ctx)[1]
+
+
+def _handle(ctx):
+    return ctx.handle if isinstance(ctx, Context) else ctx
+
+
+def eig_cameras(cam_unc: np.ndarray, cam_params: int, values: bool = True, vectors: bool = True,
+                ctx=None):
+    """(eigenvalues (n, cp) descending or None, centre-block eigenvectors (n, 9)
+    or None) of the packed camera covariances, on the device."""
+    per = cam_params * (cam_params + 1) // 2
+    cam = np.ascontiguousarray(cam_unc, np.float64).reshape(-1, per)
+    n = cam.shape[0]
+    ev = np.zeros((n, cam_params)) if values else None
+    vec = np.zeros((n, 9)) if vectors else None
+    _check(lib().unc_eig_cameras(_handle(ctx), n, cam_params, _ptr(cam),
+                                 None if ev is None else _ptr(ev),
+                                 None if vec is None else _ptr(vec)), "unc_eig_cameras")
+    return ev, vec
+
+
+def eig_points(pt_unc: np.ndarray, values: bool = True, vectors: bool = True, ctx=None):
+    """(eigenvalues (n, 3) descending or None, eigenvectors (n, 9) or None) of
+    the packed point covariances, on the device."""
+    pt = np.ascontiguousarray(pt_unc, np.float64).reshape(-1, 6)
+    n = pt.shape[0]
+    ev = np.zeros((n, 3)) if values else None
+    vec = np.zeros((n, 9)) if vectors else None
+    _check(lib().unc_eig_points(_handle(ctx), n, _ptr(pt), None if ev is None else _ptr(ev),
+                                None if vec is None else _ptr(vec)), "unc_eig_points")
+    return ev, vec
+
+
+def eig_cameras_device(num_cams: int, cam_params: int, d_cam: int, d_vals: int = 0,
+                       d_vecs: int = 0, ctx=None, stream: int = 0):
+    """Enqueue the camera eigen kernel on device addresses (e.g. torch .data_ptr()).
+    stream 0 means the context's stream (not the null stream); returns at once."""
+    _check(lib().unc_eig_cameras_device(_handle(ctx), num_cams, cam_params, d_cam, d_vals or None,
+                                        d_vecs or None, stream or None), "unc_eig_cameras_device")
+
+
+def eig_points_device(num_points: int, d_pt: int, d_vals: int = 0, d_vecs: int = 0, ctx=None,
+                      stream: int = 0):
+    """Enqueue the point eigen kernel on device addresses (e.g. torch .data_ptr()).
+    stream 0 means the context's stream (not the null stream); returns at once."""
+    _check(lib().unc_eig_points_device(_handle(ctx), num_points, d_pt, d_vals or None,
+                                       d_vecs or None, stream or None), "unc_eig_points_device")
 
 
 class Server:

@@ -146,5 +146,10 @@ hipError_t sym_update(int n, const double *X0, const double *D, double *out, hip
 hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st);
 hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
+// eigen accessors (k_eig.hip): one lane per block, descending eigenvalues
+hipError_t launch_eig_cameras(const double *cam, int nc, int cp, double *vals, double *cvecs,
+                              hipStream_t st);
+hipError_t launch_eig_points(const double *pts, long long np, double *vals, double *vecs,
+                             hipStream_t st);
 
 }  // namespace unc

@@ -732,6 +732,94 @@ int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows, 
              nullptr, s, false);
 }
 
+int unc_eig_cameras_device(unc_context *ctx, int num_cams, int cam_params,
+                           const double *d_cam_unc, double *d_eigvals, double *d_center_vecs,
+                           void *stream) {
+  if (num_cams < 0 || cam_params < 1 || cam_params > 9) return UNC_ERR_BAD_INPUT;
+  if (d_center_vecs && cam_params < 6) return UNC_ERR_BAD_INPUT;  // block(3, 3, 3, 3)
+  if (num_cams == 0 || (!d_eigvals && !d_center_vecs)) return UNC_OK;
+  if (!d_cam_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HC(unc::launch_eig_cameras(d_cam_unc, num_cams, cam_params, d_eigvals, d_center_vecs, s));
+  return UNC_OK;
+}
+
+int unc_eig_points_device(unc_context *ctx, int num_points, const double *d_pt_unc,
+                          double *d_eigvals, double *d_eigvecs, void *stream) {
+  if (num_points < 0) return UNC_ERR_BAD_INPUT;
+  if (num_points == 0 || (!d_eigvals && !d_eigvecs)) return UNC_OK;
+  if (!d_pt_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HC(unc::launch_eig_points(d_pt_unc, num_points, d_eigvals, d_eigvecs, s));
+  return UNC_OK;
+}
+
+int unc_eig_cameras(unc_context *ctx, int num_cams, int cam_params, const double *cam_unc,
+                    double *eigvals, double *center_vecs) {
+  if (num_cams < 0 || cam_params < 1 || cam_params > 9) return UNC_ERR_BAD_INPUT;
+  if (center_vecs && cam_params < 6) return UNC_ERR_BAD_INPUT;
+  if (num_cams == 0 || (!eigvals && !center_vecs)) return UNC_OK;
+  if (!cam_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+  const size_t n_in = (size_t)num_cams * cam_params * (cam_params + 1) / 2;
+  const size_t n_val = (size_t)num_cams * cam_params, n_vec = (size_t)num_cams * 9;
+  const size_t b_in = r256(8 * n_in), b_val = eigvals ? r256(8 * n_val) : 0;
+  const size_t b_vec = center_vecs ? r256(8 * n_vec) : 0;
+  hipStream_t s = ctx->stream;
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_in + b_val + b_vec, s);
+  if (rc) return rc;
+  double *d_in = (double *)ctx->io;
+  double *d_val = eigvals ? (double *)(ctx->io + b_in) : nullptr;
+  double *d_vec = center_vecs ? (double *)(ctx->io + b_in + b_val) : nullptr;
+  HC(hipMemcpyAsync(d_in, cam_unc, 8 * n_in, hipMemcpyHostToDevice, s));
+  HC(unc::launch_eig_cameras(d_in, num_cams, cam_params, d_val, d_vec, s));
+  if (eigvals) HC(hipMemcpyAsync(eigvals, d_val, 8 * n_val, hipMemcpyDeviceToHost, s));
+  if (center_vecs) HC(hipMemcpyAsync(center_vecs, d_vec, 8 * n_vec, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  return UNC_OK;
+}
+
+int unc_eig_points(unc_context *ctx, int num_points, const double *pt_unc, double *eigvals,
+                   double *eigvecs) {
+  if (num_points < 0) return UNC_ERR_BAD_INPUT;
+  if (num_points == 0 || (!eigvals && !eigvecs)) return UNC_OK;
+  if (!pt_unc) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+  const size_t np = (size_t)num_points;
+  const size_t b_in = r256(48 * np), b_val = eigvals ? r256(24 * np) : 0;
+  const size_t b_vec = eigvecs ? r256(72 * np) : 0;
+  hipStream_t s = ctx->stream;
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_in + b_val + b_vec, s);
+  if (rc) return rc;
+  double *d_in = (double *)ctx->io;
+  double *d_val = eigvals ? (double *)(ctx->io + b_in) : nullptr;
+  double *d_vec = eigvecs ? (double *)(ctx->io + b_in + b_val) : nullptr;
+  HC(hipMemcpyAsync(d_in, pt_unc, 48 * np, hipMemcpyHostToDevice, s));
+  HC(unc::launch_eig_points(d_in, num_points, d_val, d_vec, s));
+  if (eigvals) HC(hipMemcpyAsync(eigvals, d_val, 24 * np, hipMemcpyDeviceToHost, s));
+  if (eigvecs) HC(hipMemcpyAsync(eigvecs, d_vec, 72 * np, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  return UNC_OK;
+}
+
 unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   if (!o || o->num_cams <= 0 || o->cam_params <= 0 || o->num_points < 4 || num_rows <= 0) return 0;
   Sizes z;

@@ -2,68 +2,20 @@
 // 48-61, semantics of computeCovariances, compute.cpp:458-540) over the C ABI.
 #include "../../include/uncertainty.h"
 
-#include <cstring>
-
 #include "../../include/uncertainty_amd.h"
+#include "shim_common.h"
 
 namespace {
 thread_local int g_last_error = UNC_OK;
 }
 
+void unc_set_last_error(int code) { g_last_error = code; }
+
 int getCovariancesLastError() { return g_last_error; }
 
 void getCovariances(cov::Options &options, cov::Statistic &statistic, ceres::CRSMatrix &jacobian,
                     double *h_camUnc, double *h_ptUnc) {
-  // compute.cpp:459: nothing to do without output buffers
-  if (h_camUnc == NULL || h_ptUnc == NULL) {
-    g_last_error = UNC_ERR_BAD_INPUT;
-    return;
-  }
-  unc_options o;
-  unc_default_options(&o);
-  o.algorithm = options._algorithm;
-  o.epsilon = options._epsilon;
-  o.lambda = options._lambda;
-  o.num_cams = options._numCams;
-  o.cam_params = options._camParams;
-  o.num_points = options._numPoints;
-  o.num_obs = options._numObs;
-  o.svd_remove_n = options._svdRemoveN;
-  o.max_iter_te = options._maxIterTE;
-  if (options._pts2fix) {
-    o.pts2fix[0] = options._pts2fix[0];
-    o.pts2fix[1] = options._pts2fix[1];
-    o.pts2fix[2] = options._pts2fix[2];
-  }
-  unc_stats st;
-  std::memset(&st, 0, sizeof(st));
-  int rc = unc_compute(NULL, &o, jacobian.num_rows, jacobian.num_cols, jacobian.rows.data(),
-                       jacobian.cols.data(), jacobian.values.data(), h_camUnc, h_ptUnc, &st);
+  const int rc = shim_compute(options, statistic, jacobian, h_camUnc, h_ptUnc);
   g_last_error = rc;
-  if (rc != UNC_OK) {
-    std::cerr << "getCovariances: " << unc_error_string(rc) << "\n";
-    return;
-  }
-  // the reference sorts the first two ids of the caller's array in place (compute.cpp:285)
-  if (options._pts2fix) {
-    options._pts2fix[0] = st.fixed_pts[0];
-    options._pts2fix[1] = st.fixed_pts[1];
-  }
-  statistic.timeCreateJ = st.time_create_j;
-  statistic.timeFixJ = st.time_fix_j;
-  statistic.timeNormJ = st.time_norm_j;
-  statistic.timeMultiplyJJ = st.time_multiply_jj;
-  statistic.timeSplitJJ = st.time_split_jj;
-  statistic.timeInvV = st.time_inv_v;
-  statistic.timeComposeZ = st.time_compose_z;
-  statistic.timeInvZ = st.time_inv_z;
-  statistic.timeTE = st.time_te;
-  statistic.timePtsUnc = st.time_pts_unc;
-  statistic.timeAll = st.time_all;
-  statistic.lambda = st.lambda;
-  free(statistic.fixedPts);
-  statistic.fixedPts = (int *)malloc(3 * sizeof(int));
-  if (statistic.fixedPts)
-    for (int i = 0; i < 3; ++i) statistic.fixedPts[i] = st.fixed_pts[i];
-  statistic.cycle_change.assign(st.cycle_change, st.cycle_change + st.num_terms);
+  if (rc != UNC_OK) std::cerr << "getCovariances: " << unc_error_string(rc) << "\n";
 }
