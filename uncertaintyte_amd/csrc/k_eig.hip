@@ -16,7 +16,7 @@
 //   V_rp, V_rq <- c V_rp - s V_rq, s V_rp + c V_rq
 // A rotation is applied when |a_pq| > eps sqrt(|a_pp a_qq|) (the
 // relative-accuracy test of Demmel and Veselic, SIAM J. Matrix Anal. Appl. 13,
-// 1992); a sweep with no rotation ends the iteration.  Rotations are
+// 1992); the iteration ends when no pair passes that test.  Rotations are
 // branch-free (t = 0 when skipped, which makes the update an exact identity),
 // so the lanes of a wave stay converged.
 //
@@ -49,52 +49,86 @@ struct Vectors {
   double v[n][n];
 };
 
-// One cyclic sweep over every (p, q), p < q.  Returns the number of rotations.
+// One cyclic sweep over every (p, q), p < q.
 // With d = a_qq - a_pp and e = 2 a_pq, Rutishauser's t = sign(tau) / (|tau| +
 // sqrt(1 + tau^2)), tau = d / e, equals sign(d) e / (|d| + sqrt(d^2 + e^2)):
-// one square root and one division per rotation.  The block is scaled to
+// one square root and one reciprocal per rotation.  The block is scaled to
 // max |a| in [1, 2) beforehand (exactly, by a power of two), so d^2 + e^2
 // cannot overflow, and an a_pq whose square underflows is below 1e-154 of the
 // block's largest entry.
+// The rotation test: |a_pq| > eps sqrt(|a_pp a_qq|), squared.  An a_pq whose
+// square is below DBL_MIN (under 1.5e-154 of the scaled block's largest entry)
+// counts as zero, which keeps every operand of the rotation a normal number.
+__device__ __forceinline__ bool needs_rotation(double apq, double app, double aqq) {
+  return apq * apq > fmax(kJacobiEps * kJacobiEps * fabs(app * aqq), 2.2250738585072014e-308);
+}
+
+// 1/sqrt(x) and 1/x for normal, finite x: the hardware estimates refined by two
+// Newton steps each (quadratic convergence from the estimate's >= 20 bits),
+// without the special-value handling of the library versions; the operands
+// here are never zero, denormal, infinite or NaN when their result is used.
+__device__ __forceinline__ double rsqrt_normal(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  return y;
+}
+
+__device__ __forceinline__ double rcp_normal(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+// True when no pair would rotate: the sweep that would find that is skipped.
+template <int n>
+__device__ __forceinline__ bool converged(SymBlock<n> &A) {
+  bool any = false;
+#pragma unroll
+  for (int p = 0; p < n - 1; ++p)
+#pragma unroll
+    for (int q = p + 1; q < n; ++q) any = any || needs_rotation(A.at(p, q), A.at(p, p), A.at(q, q));
+  return !any;
+}
+
 template <int n, bool VEC>
-__device__ __forceinline__ int jacobi_sweep(SymBlock<n> &A, Vectors<n> &V) {
-  int rotations = 0;
+__device__ __forceinline__ void jacobi_sweep(SymBlock<n> &A, Vectors<n> &V) {
 #pragma unroll
   for (int p = 0; p < n - 1; ++p) {
 #pragma unroll
     for (int q = p + 1; q < n; ++q) {
       const double apq = A.at(p, q);
       const double app = A.at(p, p), aqq = A.at(q, q);
-      // |a_pq| > eps sqrt(|a_pp a_qq|), squared
-      const bool rot = apq * apq > kJacobiEps * kJacobiEps * fabs(app * aqq) && apq != 0.0;
+      const bool rot = needs_rotation(apq, app, aqq);
       const double d = aqq - app, e = 2.0 * apq;
-      const double r = sqrt(d * d + e * e);
-      double t = (d < 0.0 ? -e : e) / (fabs(d) + r);
-      t = rot ? t : 0.0;  // a skipped pair (also d = e = 0) gets the identity
-      const double c = rsqrt(1.0 + t * t);
+      const double h = d * d + e * e;        // >= 4 DBL_MIN when rot
+      const double r = h * rsqrt_normal(h);  // sqrt(h)
+      double t = (d < 0.0 ? -e : e) * rcp_normal(fabs(d) + r);
+      t = rot ? t : 0.0;  // a skipped pair gets the identity (and drops any NaN above)
+      const double c = rsqrt_normal(1.0 + t * t);
       const double s = t * c;
-      A.at(p, p) = app - t * apq;
-      A.at(q, q) = aqq + t * apq;
+      A.at(p, p) = fma(-t, apq, app);
+      A.at(q, q) = fma(t, apq, aqq);
       A.at(p, q) = rot ? 0.0 : apq;
 #pragma unroll
       for (int r2 = 0; r2 < n; ++r2) {
         if (r2 == p || r2 == q) continue;
         const double arp = A.at(r2, p), arq = A.at(r2, q);
-        A.at(r2, p) = c * arp - s * arq;
-        A.at(r2, q) = s * arp + c * arq;
+        A.at(r2, p) = fma(c, arp, -s * arq);
+        A.at(r2, q) = fma(s, arp, c * arq);
       }
       if (VEC) {
 #pragma unroll
         for (int r2 = 0; r2 < n; ++r2) {
           const double vrp = V.v[r2][p], vrq = V.v[r2][q];
-          V.v[r2][p] = c * vrp - s * vrq;
-          V.v[r2][q] = s * vrp + c * vrq;
+          V.v[r2][p] = fma(c, vrp, -s * vrq);
+          V.v[r2][q] = fma(s, vrp, c * vrq);
         }
       }
-      rotations += rot ? 1 : 0;
     }
   }
-  return rotations;
 }
 
 // Diagonalise A in place (eigenvalues on its diagonal, V its eigenvectors when
@@ -115,8 +149,7 @@ __device__ __forceinline__ void sym_eig(SymBlock<n> &A, Vectors<n> &V, double (&
 #pragma unroll
   for (int i = 0; i < n * (n + 1) / 2; ++i) A.a[i] = ldexp(A.a[i], -ex);
   constexpr int kMaxSweeps = n <= 3 ? 12 : 24;
-  for (int sweep = 0; sweep < kMaxSweeps; ++sweep)
-    if (jacobi_sweep<n, VEC>(A, V) == 0) break;
+  for (int sweep = 0; sweep < kMaxSweeps && !converged(A); ++sweep) jacobi_sweep<n, VEC>(A, V);
 #pragma unroll
   for (int i = 0; i < n; ++i) d[i] = ldexp(A.at(i, i), ex);
   // descending bubble network on register values (constant indices)
