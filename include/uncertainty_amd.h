@@ -189,8 +189,10 @@ int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
    kernels of the Z^+ path; L's upper triangle above the 64 x 64 diagonal blocks is
    left as A's (only the lower triangle is the factor), X's upper triangle is 0. */
 int unc_diag_chol_inverse(int n, const double *A, double *L, double *X);
-/* average time of one 64 x 64 Cholesky + inverse leaf (plus a 32 KB copy),
-   variant 0 = unrolled register form (production), 1 = compact */
+/* average time of one 64 x 64 leaf kernel: 0 = panelled Cholesky + L^-1
+   (production, plus a 32 KB copy of its input), 1 = compact and 2 = unrolled
+   register forms of the same (plus the copy), 3 = SPD inverse by Cholesky
+   (the Gauss-Jordan panel inverse), 4 = in-register Gauss-Jordan panel inverse */
 int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
 /* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
  * device-resident operands (throughput probe); variant 0 is the production

@@ -118,7 +118,10 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // gauge-fixed Z^+ path (k_zplus.hip)
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st);
-void set_leaf_variant(int v);  // 0: unrolled registers (default), 1: compact
+void set_leaf_variant(int v);  // 0: panelled, 256 threads (default), 1: compact, 2: unrolled registers
+// P = W^-1 of a b x b SPD block (b <= 64) by Cholesky, L^-1 and X^T X (k_zplus.hip)
+hipError_t spd_inverse64(int b, const double *W, int ldw, double *P, unsigned int *err,
+                         hipStream_t st);
 hipError_t skinny_tn(int n, int kc, const double *A, int lda, const double *B, double *C,
                      bool lower, hipStream_t st);
 hipError_t skinny_nn(int n, int kc, const double *A, int lda, const double *B, double *C,

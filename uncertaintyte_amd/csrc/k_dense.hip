@@ -631,7 +631,7 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // ---------------------------------------------------------------------------
 constexpr int R_T = 64, R_K = 16;
 
-__global__ __launch_bounds__(256) void k_residual_dd(int n, const double *__restrict__ A,
+__global__ __launch_bounds__(256, 4) void k_residual_dd(int n, const double *__restrict__ A,
                                                      const double *__restrict__ X,
                                                      double *__restrict__ R) {
   __shared__ double As[R_K][R_T + 1];
@@ -657,8 +657,10 @@ __global__ __launch_bounds__(256) void k_residual_dd(int n, const double *__rest
       Xs[kk][jj] = (gj < n && gk2 < n) ? X[(size_t)gk2 + (size_t)gj * n] : 0.0;
     }
     __syncthreads();
-    const int kn = min(R_K, n - k0);
-    for (int k = 0; k < kn; ++k) {
+    // k beyond n reads zeros from LDS: their products are exact zeros and leave
+    // (s, c) unchanged, so the chunk loop has a constant trip count
+#pragma unroll
+    for (int k = 0; k < R_K; ++k) {
       double av[4], xv[4];
       for (int p = 0; p < 4; ++p) av[p] = As[k][tr + 16 * p];
       for (int q = 0; q < 4; ++q) xv[q] = Xs[k][tc + 16 * q];

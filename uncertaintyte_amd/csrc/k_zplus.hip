@@ -158,6 +158,176 @@ __global__ __launch_bounds__(64) void k_chol_inv64c(int b, double *A, int lda, d
   if (bad && i == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
+// ---------------------------------------------------------------------------
+// Panelled leaf (the default): 256 threads, the block in LDS, 8-column panels.
+//   Cholesky, right-looking by panels: wave 0 factors the 8 panel columns
+//   (lane = row; the panel rows stay in registers and the pivot row's values
+//   are broadcast with readlane), then all four waves apply the rank-8 update
+//   to the trailing lower triangle.  Two barriers per panel instead of one or
+//   two per column.
+//   X = L^-1 the same way: wave 0 solves the panel's 8 rows for all 64 columns
+//   (lane = column), the four waves apply the rank-8 update to the rows below.
+// The loops over panels stay rolled, so the code is small (no instruction-fetch
+// stalls); every register array is indexed by unrolled constants.
+// ---------------------------------------------------------------------------
+constexpr int CP_W = 8;
+constexpr int CP_THREADS = 256;
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// S: the SPD block (lower triangle valid, upper triangle zero) -> L with zeros
+// above the diagonal; Xs -> L^-1 with zeros above.  Returns true in wave 0 when
+// a pivot was not positive.
+__device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  bool bad = false;
+  for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
+    if (wave == 0) {
+      const int i = lane;
+      double r[CP_W];
+#pragma unroll
+      for (int jj = 0; jj < CP_W; ++jj) r[jj] = S[i][p0 + jj];
+#pragma unroll
+      for (int jj = 0; jj < CP_W; ++jj) {
+        const int j = p0 + jj;
+        double s = r[jj];
+#pragma unroll
+        for (int kk = 0; kk < jj; ++kk) s = fma(-r[kk], readlane_d(r[kk], j), s);
+        const double piv = readlane_d(s, j);
+        if (!(piv > 0.0)) bad = true;
+        const double d = sqrt(piv);
+        const double rd = 1.0 / d;
+        r[jj] = i > j ? s * rd : (i == j ? d : 0.0);
+      }
+#pragma unroll
+      for (int jj = 0; jj < CP_W; ++jj) S[i][p0 + jj] = r[jj];
+    }
+    __syncthreads();
+    const int q0 = p0 + CP_W;
+    const int i = q0 + lane;
+    if (i < CL_B) {
+      double li[CP_W];
+#pragma unroll
+      for (int jj = 0; jj < CP_W; ++jj) li[jj] = S[i][p0 + jj];
+      for (int k = q0 + wave; k <= i; k += 4) {
+        double acc = S[i][k];
+#pragma unroll
+        for (int jj = 0; jj < CP_W; ++jj) acc = fma(-li[jj], S[k][p0 + jj], acc);
+        S[i][k] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int x = t; x < CL_B * CL_B; x += CP_THREADS) {
+    const int r = x >> 6, c = x & 63;
+    Xs[r][c] = r == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
+    if (wave == 0) {
+      const int c = lane;
+      double x[CP_W];
+#pragma unroll
+      for (int rr = 0; rr < CP_W; ++rr) x[rr] = Xs[p0 + rr][c];
+#pragma unroll
+      for (int rr = 0; rr < CP_W; ++rr) {
+        double s = x[rr];
+#pragma unroll
+        for (int kk = 0; kk < rr; ++kk) s = fma(-S[p0 + rr][p0 + kk], x[kk], s);
+        x[rr] = s / S[p0 + rr][p0 + rr];
+      }
+#pragma unroll
+      for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
+    }
+    __syncthreads();
+    const int q0 = p0 + CP_W;
+    if (q0 < CL_B) {
+      const int c = lane;
+      double xk[CP_W];
+#pragma unroll
+      for (int kk = 0; kk < CP_W; ++kk) xk[kk] = Xs[p0 + kk][c];
+      for (int r = q0 + wave; r < CL_B; r += 4) {
+        double acc = Xs[r][c];
+#pragma unroll
+        for (int kk = 0; kk < CP_W; ++kk) acc = fma(-S[r][p0 + kk], xk[kk], acc);
+        Xs[r][c] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  return bad;
+}
+
+// b x b block of A (column-major, lower triangle read) -> S, identity padding
+__device__ __forceinline__ void load_spd64(int b, const double *A, int lda,
+                                           double (*S)[CL_B + 1]) {
+  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
+    const int i = x & 63, k = x >> 6;
+    S[i][k] = (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0)
+                               : (i == k ? 1.0 : 0.0);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(CP_THREADS) void k_chol_inv64p(int b, double *A, int lda,
+                                                            double *Xo, int ldx,
+                                                            unsigned int *err) {
+  __shared__ double S[CL_B][CL_B + 1];
+  __shared__ double Xs[CL_B][CL_B + 1];
+  load_spd64(b, A, lda, S);
+  const bool bad = chol_inv_lds(S, Xs);
+  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
+    const int i = x & 63, k = x >> 6;
+    if (i < b && k < b) {
+      A[(size_t)i + (size_t)k * lda] = S[i][k];
+      Xo[(size_t)i + (size_t)k * ldx] = Xs[i][k];
+    }
+  }
+  if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
+// P = W^-1 = X^T X (X = L^-1) for the b x b SPD diagonal block of the blocked
+// Gauss-Jordan inverse; P is b x b with leading dimension b.  Each thread forms
+// a 4 x 4 set of entries, summing k in the same order for (i, j) and (j, i), so
+// P is exactly symmetric.
+__global__ __launch_bounds__(CP_THREADS) void k_spd_inv64(int b, const double *W, int ldw,
+                                                          double *P, unsigned int *err) {
+  __shared__ double S[CL_B][CL_B + 1];
+  __shared__ double Xs[CL_B][CL_B + 1];
+  load_spd64(b, W, ldw, S);
+  const bool bad = chol_inv_lds(S, Xs);
+  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int k = 0; k < CL_B; ++k) {
+    double xi[4], xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xi[u] = Xs[k][ti + 16 * u];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) xj[v] = Xs[k][tj + 16 * v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(xi[u], xj[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti + 16 * u, j = tj + 16 * v;
+      if (i < b && j < b) P[(size_t)i + (size_t)j * b] = acc[u][v];
+    }
+  if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
 static int g_leaf_variant = 0;
 void set_leaf_variant(int v) { g_leaf_variant = v; }
 
@@ -166,8 +336,17 @@ hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned
   if (b < 1 || b > CL_B) return hipErrorInvalidValue;
   if (g_leaf_variant == 1)
     hipLaunchKernelGGL(k_chol_inv64c, dim3(1), dim3(64), 0, st, b, A, lda, X, ldx, err);
-  else
+  else if (g_leaf_variant == 2)
     hipLaunchKernelGGL(k_chol_inv64, dim3(1), dim3(64), 0, st, b, A, lda, X, ldx, err);
+  else
+    hipLaunchKernelGGL(k_chol_inv64p, dim3(1), dim3(CP_THREADS), 0, st, b, A, lda, X, ldx, err);
+  return hipGetLastError();
+}
+
+hipError_t spd_inverse64(int b, const double *W, int ldw, double *P, unsigned int *err,
+                         hipStream_t st) {
+  if (b < 1 || b > CL_B) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_spd_inv64, dim3(1), dim3(CP_THREADS), 0, st, b, W, ldw, P, err);
   return hipGetLastError();
 }
 

@@ -199,7 +199,7 @@ int gj_inverse(int n, double *W, double *P, double *T, double *R, unsigned int *
   for (int k0 = 0; k0 < n; k0 += GJ_BLOCK) {
     const int kb = std::min(GJ_BLOCK, n - k0);
     double *Wkk = W + k0 + (size_t)k0 * n;
-    HC(gj_small_inverse(kb, Wkk, n, P, err, s));
+    HC(spd_inverse64(kb, Wkk, n, P, err, s));
     // T = W[:,K] P ; R = W[K,:]
     HC(gemm_f64(n, kb, kb, 1.0, W + (size_t)k0 * n, n, P, kb, 0.0, T, n, nullptr, nullptr, s));
     HC(copy_block(kb, n, W + k0, n, R, kb, 1.0, s));
@@ -991,7 +991,7 @@ int unc_diag_chol_inverse(int n, const double *A, double *L, double *X) {
 }
 
 int unc_diag_leaf_bench(int reps, int variant, double *avg_us) {
-  if (reps <= 0 || !avg_us || variant < 0 || variant > 1) return UNC_ERR_BAD_INPUT;
+  if (reps <= 0 || !avg_us || variant < 0 || variant > 4) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
@@ -1008,15 +1008,17 @@ int unc_diag_leaf_bench(int reps, int variant, double *avg_us) {
   unsigned int *err = (unsigned int *)(dX + n * n);
   HC(hipMemcpyAsync(dA, h.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
   HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
-  set_leaf_variant(variant);
+  set_leaf_variant(variant <= 2 ? variant : 0);
   hipEvent_t e0 = ctx->stage_ev[0], e1 = ctx->stage_ev[1];
-  HC(hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-  HC(chol_inv_leaf(n, dW, n, dX, n, err, s));
+  auto one = [&]() -> hipError_t {
+    if (variant == 3) return spd_inverse64(n, dA, n, dX, err, s);
+    if (variant == 4) return gj_small_inverse(n, dA, n, dX, err, s);
+    hipError_t e = hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s);
+    return e != hipSuccess ? e : chol_inv_leaf(n, dW, n, dX, n, err, s);
+  };
+  HC(one());
   HC(hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r) {
-    HC(hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-    HC(chol_inv_leaf(n, dW, n, dX, n, err, s));
-  }
+  for (int r = 0; r < reps; ++r) HC(one());
   HC(hipEventRecord(e1, s));
   HC(hipStreamSynchronize(s));
   set_leaf_variant(0);
