@@ -199,6 +199,9 @@ int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
  * kernel (K slab 32), 1-3 are tuning alternatives (occupancy 2, K slab 16,
  * LDS double buffer). */
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
+/* Average duration (ms) of the n x n compensated residual R = I - A X over reps
+ * launches; variant 0 is the production tiling (64 x 32), 1 the 64 x 64 one. */
+int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
 /* Select the GEMM kernel used by every later call in this process (tuning). */
 int unc_diag_set_gemm_variant(int variant);
 

@@ -127,6 +127,7 @@ def lib():
     L.unc_server_destroy.argtypes = [ctypes.c_void_p]
     L.unc_server_destroy.restype = None
     L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    L.unc_diag_residual_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
     L.unc_eig_cameras.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]

@@ -631,62 +631,84 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // ---------------------------------------------------------------------------
 constexpr int R_T = 64, R_K = 16;
 
-__global__ __launch_bounds__(256, 4) void k_residual_dd(int n, const double *__restrict__ A,
-                                                     const double *__restrict__ X,
-                                                     double *__restrict__ R) {
+// Tile 64 x 16*QN per 256 threads; thread (tr, tc) owns rows tr + 16p (p < 4)
+// and columns tc + 16q (q < QN).  QN = 2 halves the work per wave against
+// QN = 4, so the waves spread evenly over the SIMDs (N = 1800: 6.6 waves per
+// SIMD rather than 3.3).
+template <int QN>
+__global__ __launch_bounds__(256, QN == 4 ? 4 : 6) void k_residual_dd(
+    int n, const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ R) {
+  constexpr int RC = 16 * QN;
   __shared__ double As[R_K][R_T + 1];
-  __shared__ double Xs[R_K][R_T + 1];
+  __shared__ double Xs[R_K][RC + 1];
   const int tid = threadIdx.x;
   const int tr = tid & 15, tc = tid >> 4;
-  const int i0 = blockIdx.x * R_T, j0 = blockIdx.y * R_T;
-  double s[4][4], c[4][4];
+  const int i0 = blockIdx.x * R_T, j0 = blockIdx.y * RC;
+  double s[4][QN], c[4][QN];
+#pragma unroll
   for (int p = 0; p < 4; ++p)
-    for (int q = 0; q < 4; ++q) {
-      int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
       s[p][q] = (gi == gj) ? 1.0 : 0.0;
       c[p][q] = 0.0;
     }
   for (int k0 = 0; k0 < n; k0 += R_K) {
     __syncthreads();
     for (int x = tid; x < R_K * R_T; x += 256) {
-      int m = x & 63, k = x >> 6;
-      int gi = i0 + m, gk = k0 + k;
+      const int m = x & 63, k = x >> 6;
+      const int gi = i0 + m, gk = k0 + k;
       As[k][m] = (gi < n && gk < n) ? A[(size_t)gi + (size_t)gk * n] : 0.0;
-      int kk = x & 15, jj = x >> 4;
-      int gj = j0 + jj, gk2 = k0 + kk;
-      Xs[kk][jj] = (gj < n && gk2 < n) ? X[(size_t)gk2 + (size_t)gj * n] : 0.0;
+    }
+    for (int x = tid; x < R_K * RC; x += 256) {
+      const int kk = x & 15, jj = x >> 4;
+      const int gj = j0 + jj, gk = k0 + kk;
+      Xs[kk][jj] = (gj < n && gk < n) ? X[(size_t)gk + (size_t)gj * n] : 0.0;
     }
     __syncthreads();
     // k beyond n reads zeros from LDS: their products are exact zeros and leave
     // (s, c) unchanged, so the chunk loop has a constant trip count
 #pragma unroll
     for (int k = 0; k < R_K; ++k) {
-      double av[4], xv[4];
+      double av[4], xv[QN];
+#pragma unroll
       for (int p = 0; p < 4; ++p) av[p] = As[k][tr + 16 * p];
-      for (int q = 0; q < 4; ++q) xv[q] = Xs[k][tc + 16 * q];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) xv[q] = Xs[k][tc + 16 * q];
+#pragma unroll
       for (int p = 0; p < 4; ++p)
-        for (int q = 0; q < 4; ++q) {
-          double P = av[p] * xv[q];
-          double E = fma(av[p], xv[q], -P);
-          double pm = -P;
-          double t = s[p][q] + pm;
-          double bb = t - s[p][q];
-          double err = (s[p][q] - (t - bb)) + (pm - bb);
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+          const double P = av[p] * xv[q];
+          const double E = fma(av[p], xv[q], -P);
+          const double pm = -P;
+          const double t = s[p][q] + pm;
+          const double bb = t - s[p][q];
+          const double err = (s[p][q] - (t - bb)) + (pm - bb);
           s[p][q] = t;
           c[p][q] = c[p][q] + (err - E);
         }
     }
   }
+#pragma unroll
   for (int p = 0; p < 4; ++p)
-    for (int q = 0; q < 4; ++q) {
-      int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
       if (gi < n && gj < n) R[(size_t)gi + (size_t)gj * n] = s[p][q] + c[p][q];
     }
 }
 
+static int g_residual_variant = 0;
+void set_residual_variant(int v) { g_residual_variant = v; }
+
 hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st) {
-  dim3 grid(cdiv(n, R_T), cdiv(n, R_T));
-  hipLaunchKernelGGL(k_residual_dd, grid, dim3(256), 0, st, n, A, X, R);
+  if (g_residual_variant == 1)
+    hipLaunchKernelGGL(k_residual_dd<4>, dim3(cdiv(n, R_T), cdiv(n, 64)), dim3(256), 0, st, n, A,
+                       X, R);
+  else
+    hipLaunchKernelGGL(k_residual_dd<2>, dim3(cdiv(n, R_T), cdiv(n, 32)), dim3(256), 0, st, n, A,
+                       X, R);
   return hipGetLastError();
 }
 

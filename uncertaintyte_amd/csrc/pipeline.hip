@@ -923,6 +923,33 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
   return UNC_OK;
 }
 
+int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 1) return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n;
+  double *buf = nullptr;
+  if (hipMalloc((void **)&buf, sizeof(double) * 3 * nn) != hipSuccess) {
+    (void)hipGetLastError();
+    return UNC_ERR_OOM;
+  }
+  hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
+  set_residual_variant(variant);
+  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  HC(hipEventRecord(ctx->gemm_ev[0], s));
+  for (int r = 0; r < reps; ++r) HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  HC(hipEventRecord(ctx->gemm_ev[1], s));
+  HC(hipEventSynchronize(ctx->gemm_ev[1]));
+  *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
+  set_residual_variant(0);
+  HC(hipFree(buf));
+  return UNC_OK;
+}
+
 int unc_diag_set_gemm_variant(int variant) {
   if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
   set_gemm_variant(variant);
