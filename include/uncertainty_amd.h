@@ -172,6 +172,45 @@ int unc_eig_cameras_device(unc_context *ctx, int num_cams, int cam_params,
 int unc_eig_points_device(unc_context *ctx, int num_points, const double *d_pt_unc,
                           double *d_eigvals, double *d_eigvecs, void *stream);
 
+/* ---- reconstruction front-end: COLMAP text model -> CRS Jacobian
+ * Replaces the reference's ColmapIO::read (ColmapIO.cpp:24-146) and
+ * JacobianComposer::scene2Jacobian (JacobianComposer.cpp:84-194, Ceres
+ * autodiff of snavely_reprojection_error.h:148-302) with a C++ reader and an
+ * analytic Jacobian computed on the device.  Camera models (-cam of the
+ * reference CLI) and the camera columns they free: */
+#define UNC_CAM_SIMPLE_PINHOLE 0  /* cp 6: aa, C          (f, k1, k2 held)  */
+#define UNC_CAM_PINHOLE 1         /* cp 7: aa, C, f       (k1, k2 held)     */
+#define UNC_CAM_SIMPLE_RADIAL 2   /* cp 8: aa, C, f, k1   (no k2 term)      */
+#define UNC_CAM_RADIAL 3          /* cp 9: aa, C, f, k1, k2                 */
+typedef struct unc_scene unc_scene;
+/* cameras.txt, images.txt and points3D.txt of the COLMAP text model in dir.
+ * Order (that of the reference's parameter blocks, JacobianComposer.cpp:
+ * 123-157): images by id, those without observations dropped; points by id;
+ * observations image by image, each image's by point id. */
+int unc_scene_read_colmap(const char *dir, unc_scene **out);
+int unc_scene_counts(const unc_scene *sc, int *num_cams, int *num_points, int *num_obs);
+/* cams: num_cams x 9 (aa[3], C[3], f, k1, k2); points: num_points x 3;
+ * obs_cam / obs_pt: num_obs indices.  Any pointer may be NULL. */
+int unc_scene_arrays(const unc_scene *sc, double *cams, double *points, int *obs_cam,
+                     int *obs_pt);
+void unc_scene_destroy(unc_scene *sc);
+/* JacobianComposer::findPts2Fix (JacobianComposer.cpp:63-83): the most distant
+ * of 100 000 random distinct triples (triangle perimeter), first two ids sorted;
+ * the seed is fixed (the reference seeds from std::random_device). */
+int unc_find_pts2fix(int num_points, const double *points3D, int out[3]);
+/* The CRS Jacobian in the .jacob layout: rows[2 num_obs + 1], cols and vals
+ * [2 num_obs (cp + 3)], cp from cam_model.  Host arrays (ctx NULL: device 0);
+ * the _device variant takes device pointers, enqueues on stream (NULL: ctx's
+ * stream) and waits for the kernel, since it reports index errors. */
+int unc_scene_jacobian(unc_context *ctx, int cam_model, int num_cams, int num_points,
+                       int num_obs, const double *cams, const double *points,
+                       const int *obs_cam, const int *obs_pt, int *rows, int *cols,
+                       double *vals);
+int unc_scene_jacobian_device(unc_context *ctx, int cam_model, int num_cams, int num_points,
+                              int num_obs, const double *d_cams, const double *d_points,
+                              const int *d_obs_cam, const int *d_obs_pt, int *d_rows,
+                              int *d_cols, double *d_vals, void *stream);
+
 /* Kernel-level diagnostics on device 0 (host pointers, column-major); they
  * pin single kernels in the parity tests and have no reference counterpart.
  *   unc_diag_gemm_f64     C = alpha A B + beta C with the FP64 MFMA GEMM;

@@ -130,6 +130,16 @@ def lib():
     L.unc_diag_residual_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
+    L.unc_scene_read_colmap.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.unc_scene_counts.argtypes = [vp, ip, ip, ip]
+    L.unc_scene_arrays.argtypes = [vp, vp, vp, vp, vp]
+    L.unc_scene_destroy.argtypes = [vp]
+    L.unc_scene_destroy.restype = None
+    L.unc_find_pts2fix.argtypes = [ctypes.c_int, vp, ip]
+    L.unc_scene_jacobian.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     vp, vp, vp, vp, vp, vp, vp]
+    L.unc_scene_jacobian_device.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
     L.unc_eig_cameras.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
     L.unc_eig_points.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.unc_eig_cameras_device.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
@@ -436,6 +446,68 @@ def eig_points_device(num_points: int, d_pt: int, d_vals: int = 0, d_vecs: int =
     stream 0 means the context's stream (not the null stream); returns at once."""
     _check(lib().unc_eig_points_device(_handle(ctx), num_points, d_pt, d_vals or None,
                                        d_vecs or None, stream or None), "unc_eig_points_device")
+
+
+# --------------------------------------------------------------------------
+# reconstruction front-end (ColmapIO.cpp:24-146, JacobianComposer.cpp:63-194)
+# --------------------------------------------------------------------------
+
+CAM_SIMPLE_PINHOLE, CAM_PINHOLE, CAM_SIMPLE_RADIAL, CAM_RADIAL = range(4)
+CAM_MODELS = {"SIMPLE_PINHOLE": CAM_SIMPLE_PINHOLE, "PINHOLE": CAM_PINHOLE,
+              "SIMPLE_RADIAL": CAM_SIMPLE_RADIAL, "RADIAL": CAM_RADIAL}
+CAM_PARAMS = {CAM_SIMPLE_PINHOLE: 6, CAM_PINHOLE: 7, CAM_SIMPLE_RADIAL: 8, CAM_RADIAL: 9}
+
+
+def read_colmap(path: str):
+    """(cams (n_cams, 9) [aa, C, f, k1, k2], points (n_points, 3), obs_cam,
+    obs_pt) of a COLMAP text model, in the reference's parameter-block order."""
+    L = lib()
+    h = ctypes.c_void_p()
+    _check(L.unc_scene_read_colmap(os.fsencode(path), ctypes.byref(h)), "unc_scene_read_colmap")
+    try:
+        nc, npt, no = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(L.unc_scene_counts(h, ctypes.byref(nc), ctypes.byref(npt), ctypes.byref(no)),
+               "unc_scene_counts")
+        cams = np.zeros((nc.value, 9))
+        pts = np.zeros((npt.value, 3))
+        oc = np.zeros(no.value, np.int32)
+        op = np.zeros(no.value, np.int32)
+        _check(L.unc_scene_arrays(h, _ptr(cams), _ptr(pts), _ptr(oc), _ptr(op)),
+               "unc_scene_arrays")
+    finally:
+        L.unc_scene_destroy(h)
+    return cams, pts, oc, op
+
+
+def find_pts2fix(points: np.ndarray) -> tuple:
+    """JacobianComposer::findPts2Fix with the library's fixed seed."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = (ctypes.c_int * 3)()
+    _check(lib().unc_find_pts2fix(P.shape[0], _ptr(P), out), "unc_find_pts2fix")
+    return tuple(out)
+
+
+def scene_jacobian(cams: np.ndarray, points: np.ndarray, obs_cam: np.ndarray,
+                   obs_pt: np.ndarray, model: int = CAM_RADIAL, fix=None, ctx=None):
+    """jacob.Jacobian of a scene by the device kernel (unc_scene_jacobian);
+    fixed points from find_pts2fix unless given."""
+    from . import jacob as _jacob
+    cams = np.ascontiguousarray(cams, np.float64).reshape(-1, 9)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    oc = np.ascontiguousarray(obs_cam, np.int32)
+    op = np.ascontiguousarray(obs_pt, np.int32)
+    cp = CAM_PARAMS[model]
+    nobs = oc.shape[0]
+    rows = np.zeros(2 * nobs + 1, np.int32)
+    cols = np.zeros(2 * nobs * (cp + 3), np.int32)
+    vals = np.zeros(2 * nobs * (cp + 3))
+    _check(lib().unc_scene_jacobian(_handle(ctx), model, cams.shape[0], pts.shape[0], nobs,
+                                    _ptr(cams), _ptr(pts), _ptr(oc), _ptr(op), _ptr(rows),
+                                    _ptr(cols), _ptr(vals)), "unc_scene_jacobian")
+    if fix is None:
+        fix = find_pts2fix(pts)
+    return _jacob.Jacobian(-1.0, cams.shape[0], cp, pts.shape[0], nobs, tuple(fix), 2 * nobs,
+                           cams.shape[0] * cp + 3 * pts.shape[0], rows, cols, vals)
 
 
 class Server:

@@ -1,19 +1,22 @@
 // cli.cpp -- `uncertainty` command line tool with the reference's flags
-// (src/software/main_uncertainty.cpp:33-93): read a .jacob, compute the
-// covariances on the MI355X, write a .cov.
+// (src/software/main_uncertainty.cpp:33-93): read a .jacob or a COLMAP text
+// model, compute the covariances on the MI355X, write a .cov.
 //
 //   uncertainty -in=scene.jacob [-in_form=JACOBIAN] -out=scene.cov
-//               [-alg=TAYLOR_EXPANSION] [-cam=RADIAL] [-debug] [-legacy_time_units]
+//   uncertainty -in=colmap_dir [-in_form=COLMAP] [-cam=SIMPLE_RADIAL] -out=scene.cov
+//               [-alg=TAYLOR_EXPANSION] [-debug] [-legacy_time_units]
 //
 // As in the reference, a .jacob input always runs TAYLOR_EXPANSION
-// (JacobianIO.cpp:25 ignores -alg).  -legacy_time_units prints stage times the
-// way the reference does (ten times the seconds, compute.cpp:33).
+// (JacobianIO.cpp:25 ignores -alg); a path ending in .jacob is read as one
+// whatever -in_form says.  -legacy_time_units prints stage times the way the
+// reference does (ten times the seconds, compute.cpp:33).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "../../include/uncertainty.h"
 #include "../../include/uncertainty_amd.h"
@@ -112,10 +115,75 @@ bool flag(const char *arg, const char *name, std::string *val) {
   return false;
 }
 
+// COLMAP text model -> Jacobian (ColmapIO::read + JacobianComposer::scene2Jacobian,
+// JacobianComposer.cpp:123-194): cp from the camera model, fixed points by
+// unc_find_pts2fix, the Jacobian by the device kernel.
+bool load_colmap(const std::string &dir, const std::string &cam, const std::string &alg,
+                 ceres::CRSMatrix &J, cov::Options &opt) {
+  static const char *kModels[4] = {"SIMPLE_PINHOLE", "PINHOLE", "SIMPLE_RADIAL", "RADIAL"};
+  int model = -1;
+  for (int m = 0; m < 4; ++m)
+    if (cam == kModels[m]) model = m;
+  if (model < 0) {
+    std::cerr << "unknown camera model " << cam << "\n";
+    return false;
+  }
+  int algorithm = -1;
+  static const char *kAlgs[3] = {"SVD_QR_ITERATION", "SVD_DEVIDE_AND_CONQUER", "TAYLOR_EXPANSION"};
+  for (int a = 0; a < 3; ++a)
+    if (alg == kAlgs[a]) algorithm = a;
+  if (algorithm < 0) {
+    std::cerr << "Unrecognized algorithm: " << alg << "\n";
+    return false;
+  }
+  unc_scene *sc = nullptr;
+  int rc = unc_scene_read_colmap(dir.c_str(), &sc);
+  if (rc != UNC_OK) {
+    std::cerr << "cannot read the COLMAP model in " << dir << ": " << unc_error_string(rc) << "\n";
+    return false;
+  }
+  int nc = 0, np = 0, nobs = 0;
+  unc_scene_counts(sc, &nc, &np, &nobs);
+  std::vector<double> cams((size_t)nc * 9), pts((size_t)np * 3);
+  std::vector<int> oc(nobs), op(nobs);
+  unc_scene_arrays(sc, cams.data(), pts.data(), oc.data(), op.data());
+  unc_scene_destroy(sc);
+  int fix[3];
+  rc = unc_find_pts2fix(np, pts.data(), fix);
+  const int cp = 6 + model;
+  if (rc == UNC_OK) {
+    J.num_rows = 2 * nobs;
+    J.num_cols = nc * cp + 3 * np;
+    J.rows.resize(2 * (size_t)nobs + 1);
+    J.cols.resize(2 * (size_t)nobs * (cp + 3));
+    J.values.resize(J.cols.size());
+    rc = unc_scene_jacobian(NULL, model, nc, np, nobs, cams.data(), pts.data(), oc.data(),
+                            op.data(), J.rows.data(), J.cols.data(), J.values.data());
+  }
+  if (rc != UNC_OK) {
+    std::cerr << "cannot compose the Jacobian: " << unc_error_string(rc) << "\n";
+    return false;
+  }
+  opt._algorithm = algorithm;
+  opt._epsilon = 1e-10;
+  opt._lambda = -1;
+  opt._numCams = nc;
+  opt._camParams = cp;
+  opt._numPoints = np;
+  opt._numObs = nobs;
+  opt._svdRemoveN = 7;
+  opt._maxIterTE = -1;
+  opt._pts2fix = (int *)malloc(3 * sizeof(int));
+  for (int i = 0; i < 3; ++i) opt._pts2fix[i] = fix[i];
+  return true;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
-  std::string in = ".", in_form = "JACOBIAN", out = ".", alg = "TAYLOR_EXPANSION", cam = "RADIAL";
+  // defaults as in the reference (main_uncertainty.cpp:33-49)
+  std::string in = ".", in_form = "COLMAP", out = ".", alg = "TAYLOR_EXPANSION",
+              cam = "SIMPLE_RADIAL";
   std::string v;
   bool legacy_units = false;
   for (int i = 1; i < argc; ++i) {
@@ -132,14 +200,17 @@ int main(int argc, char **argv) {
     }
   }
   const bool is_jacob = in.size() > 6 && in.substr(in.size() - 6) == ".jacob";
-  if (in_form != "JACOBIAN" && !is_jacob) {
-    std::cerr << "only JACOBIAN (.jacob) input is supported by this build\n";
-    return 2;
-  }
   cov::Options opt;
   cov::Statistic stat = cov::Statistic();
   ceres::CRSMatrix J;
-  if (!load_jacobian(in, J, opt)) return 1;
+  if (in_form == "JACOBIAN" || is_jacob) {
+    if (!load_jacobian(in, J, opt)) return 1;
+  } else if (in_form == "COLMAP") {
+    if (!load_colmap(in, cam, alg, J, opt)) return 1;
+  } else {
+    std::cerr << "unsupported input format " << in_form << " (JACOBIAN or COLMAP)\n";
+    return 2;
+  }
   const int per = opt._camParams * (opt._camParams + 1) / 2;
   std::vector<double> camu((size_t)opt._numCams * per, 0.0), ptu((size_t)opt._numPoints * 6, 0.0);
   getCovariances(opt, stat, J, camu.data(), ptu.data());

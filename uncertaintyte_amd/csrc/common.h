@@ -17,6 +17,7 @@ enum : unsigned int {
   ERRBIT_SINGULAR_V = 8u,    // a 3x3 point block has no finite inverse
   ERRBIT_GJ_PIVOT = 32u,     // non-positive pivot in the dense inverse
   ERRBIT_SCHUR_OVERFLOW = 64u,  // a staged Schur batch exceeded its pair buffer
+  ERRBIT_PROJECTION = 128u,     // a point lies in a camera's focal plane (depth 0)
 };
 
 // Control block of the Taylor chain, resident in device memory so the whole
@@ -150,6 +151,12 @@ hipError_t sym_update(int n, const double *X0, const double *D, double *out, hip
 hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st);
 hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
+// scene Jacobian (k_scene.hip)
+int scene_cam_params(int model);  // cp of a UNC_CAM_* model, -1 if unknown
+hipError_t launch_scene_jacobian(int model, int nc, int np, int nobs, const double *cams,
+                                 const double *pts, const int *obs_cam, const int *obs_pt,
+                                 int *rows, int *cols, double *vals, unsigned int *err,
+                                 hipStream_t st);
 // eigen accessors (k_eig.hip): one lane per block, descending eigenvalues
 hipError_t launch_eig_cameras(const double *cam, int nc, int cp, double *vals, double *cvecs,
                               hipStream_t st);

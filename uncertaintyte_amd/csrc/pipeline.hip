@@ -820,6 +820,88 @@ int unc_eig_points(unc_context *ctx, int num_points, const double *pt_unc, doubl
   return UNC_OK;
 }
 
+namespace {
+int scene_jacobian_on(unc_context *ctx, int model, int nc, int np, int nobs, const double *cams,
+                      const double *pts, const int *obs_cam, const int *obs_pt, int *rows,
+                      int *cols, double *vals, hipStream_t s) {
+  // the error word lives at the front of the context's upload buffer
+  int rc = ensure(&ctx->io, &ctx->io_bytes, 256, s);
+  if (rc) return rc;
+  unsigned int *err = (unsigned int *)ctx->io;
+  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  HC(unc::launch_scene_jacobian(model, nc, np, nobs, cams, pts, obs_cam, obs_pt, rows, cols, vals,
+                                err, s));
+  unsigned int herr = 0;
+  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr & unc::ERRBIT_RANGE) return UNC_ERR_BAD_INPUT;
+  if (herr & unc::ERRBIT_PROJECTION) return UNC_ERR_SINGULAR;
+  return UNC_OK;
+}
+}  // namespace
+
+int unc_scene_jacobian_device(unc_context *ctx, int cam_model, int num_cams, int num_points,
+                              int num_obs, const double *d_cams, const double *d_points,
+                              const int *d_obs_cam, const int *d_obs_pt, int *d_rows,
+                              int *d_cols, double *d_vals, void *stream) {
+  if (unc::scene_cam_params(cam_model) < 0) return UNC_ERR_UNSUPPORTED;
+  if (num_cams <= 0 || num_points <= 0 || num_obs <= 0) return UNC_ERR_BAD_INPUT;
+  if (!d_cams || !d_points || !d_obs_cam || !d_obs_pt || !d_rows || !d_cols || !d_vals)
+    return UNC_ERR_BAD_INPUT;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  return scene_jacobian_on(ctx, cam_model, num_cams, num_points, num_obs, d_cams, d_points,
+                           d_obs_cam, d_obs_pt, d_rows, d_cols, d_vals, s);
+}
+
+int unc_scene_jacobian(unc_context *ctx, int cam_model, int num_cams, int num_points,
+                       int num_obs, const double *cams, const double *points,
+                       const int *obs_cam, const int *obs_pt, int *rows, int *cols,
+                       double *vals) {
+  const int cp = unc::scene_cam_params(cam_model);
+  if (cp < 0) return UNC_ERR_UNSUPPORTED;
+  if (num_cams <= 0 || num_points <= 0 || num_obs <= 0) return UNC_ERR_BAD_INPUT;
+  if (!cams || !points || !obs_cam || !obs_pt || !rows || !cols || !vals) return UNC_ERR_BAD_INPUT;
+  if ((long long)2 * num_obs * (cp + 3) > 0x7fffffffLL) return UNC_ERR_UNSUPPORTED;
+  int rc = UNC_OK;
+  if (!ctx) ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+  const size_t nnz = (size_t)2 * num_obs * (cp + 3);
+  const size_t b_err = 256, b_cam = r256(72 * (size_t)num_cams), b_pt = r256(24 * (size_t)num_points);
+  const size_t b_obs = r256(4 * (size_t)num_obs), b_rows = r256(4 * (2 * (size_t)num_obs + 1));
+  const size_t b_cols = r256(4 * nnz), b_vals = r256(8 * nnz);
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_err + b_cam + b_pt + 2 * b_obs + b_rows + b_cols + b_vals, s);
+  if (rc) return rc;
+  char *p = ctx->io + b_err;
+  double *d_cam = (double *)p;
+  double *d_pt = (double *)(p += b_cam);
+  int *d_oc = (int *)(p += b_pt);
+  int *d_op = (int *)(p += b_obs);
+  int *d_rows = (int *)(p += b_obs);
+  int *d_cols = (int *)(p += b_rows);
+  double *d_vals = (double *)(p += b_cols);
+  HC(hipMemcpyAsync(d_cam, cams, 72 * (size_t)num_cams, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(d_pt, points, 24 * (size_t)num_points, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(d_oc, obs_cam, 4 * (size_t)num_obs, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(d_op, obs_pt, 4 * (size_t)num_obs, hipMemcpyHostToDevice, s));
+  rc = scene_jacobian_on(ctx, cam_model, num_cams, num_points, num_obs, d_cam, d_pt, d_oc, d_op,
+                         d_rows, d_cols, d_vals, s);
+  if (rc) return rc;
+  HC(hipMemcpyAsync(rows, d_rows, 4 * (2 * (size_t)num_obs + 1), hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(cols, d_cols, 4 * nnz, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(vals, d_vals, 8 * nnz, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  return UNC_OK;
+}
+
 unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   if (!o || o->num_cams <= 0 || o->cam_params <= 0 || o->num_points < 4 || num_rows <= 0) return 0;
   Sizes z;

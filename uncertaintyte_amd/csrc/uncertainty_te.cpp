@@ -4,52 +4,10 @@
 // unpack (host) and eigen (device) accessors.
 #include "../../include/uncertaintyTE/uncertainty.h"
 
-#include <cfloat>
-#include <random>
-
 #include "../../include/uncertainty_amd.h"
 #include "shim_common.h"
 
 namespace {
-
-// Seed of the fixed-point search (the reference draws one from std::random_device).
-constexpr unsigned kPts2FixSeed = 5489u;
-// Accepted triples, as in the reference (JacobianComposer.cpp:70).
-constexpr int kPts2FixTrials = 100000;
-
-double pt_dist(const double *P, int a, int b) {
-  const double dx = P[3 * a] - P[3 * b], dy = P[3 * a + 1] - P[3 * b + 1],
-               dz = P[3 * a + 2] - P[3 * b + 2];
-  return std::sqrt(dx * dx + dy * dy + dz * dz);
-}
-
-// JacobianComposer::findPts2Fix (JacobianComposer.cpp:63-83): among random
-// distinct triples, keep the one with the largest perimeter; the first two
-// ids are then sorted (the reference sorts only those two).
-int find_pts2fix(int N, const double *P, int out[3]) {
-  if (N < 3 || P == NULL) return UNC_ERR_BAD_INPUT;
-  std::mt19937 gen(kPts2FixSeed);
-  std::uniform_real_distribution<> dis(0, N);
-  double best = DBL_MIN;
-  bool found = false;
-  for (int i = 0; i < kPts2FixTrials;) {
-    const int p1 = (int)std::floor(dis(gen)), p2 = (int)std::floor(dis(gen)),
-              p3 = (int)std::floor(dis(gen));
-    if (p1 == p2 || p2 == p3 || p1 == p3) continue;
-    const double d = pt_dist(P, p1, p2) + pt_dist(P, p1, p3) + pt_dist(P, p2, p3);
-    if (d > best) {
-      best = d;
-      out[0] = p1;
-      out[1] = p2;
-      out[2] = p3;
-      found = true;
-    }
-    ++i;
-  }
-  if (!found) return UNC_ERR_NO_FIXED_POINTS;  // every triple coincides
-  if (out[1] < out[0]) std::swap(out[0], out[1]);
-  return UNC_OK;
-}
 
 [[noreturn]] void throw_unc(const char *what, int rc) {
   unc_set_last_error(rc);
@@ -61,7 +19,8 @@ int find_pts2fix(int N, const double *P, int out[3]) {
 void getCovariances(cov::Options &options, cov::Statistic &statistic, ceres::CRSMatrix &jacobian,
                     double *points3D, cov::Uncertainty &uncertainties) {
   int fix[3] = {-1, -1, -1};
-  int rc = find_pts2fix(options._numPoints, points3D, fix);
+  // JacobianComposer::findPts2Fix (JacobianComposer.cpp:63-83), seeded
+  int rc = unc_find_pts2fix(options._numPoints, points3D, fix);
   if (rc == UNC_OK) {
     free(options._pts2fix);
     options._pts2fix = (int *)malloc(3 * sizeof(int));

@@ -21,6 +21,7 @@ Workloads (sizes as named in the measurement plan):
 from __future__ import annotations
 
 import dataclasses
+import os
 from typing import Optional
 
 import numpy as np
@@ -349,3 +350,64 @@ WORKLOADS = {
 def make_workload(name: str, seed: Optional[int] = None) -> Jacobian:
     sc = WORKLOADS[name]() if seed is None else WORKLOADS[name](seed)
     return scene_jacobian(sc)
+
+
+# --------------------------------------------------------------------------
+# COLMAP text model writer (the input of ColmapIO::read, ColmapIO.cpp:24-146)
+# --------------------------------------------------------------------------
+
+def aa_to_quat(w):
+    """Unit quaternions (qw, qx, qy, qz) of angle-axis vectors (..., 3)."""
+    th = np.linalg.norm(w, axis=-1)
+    ths = np.where(th > 0, th, 1.0)
+    s = np.where(th > 0, np.sin(th / 2) / ths, 0.5)
+    return np.concatenate([np.cos(th / 2)[..., None], w * s[..., None]], -1)
+
+
+def write_colmap(path: str, sc: Scene, model: str = "RADIAL", image_id=None, point_id=None,
+                 principal=(640.0, 480.0), extra_image: bool = False):
+    """Write sc as cameras.txt / images.txt / points3D.txt (one COLMAP camera
+    per image; observations are the projections plus the principal point).
+    image_id / point_id map scene indices to file ids (default 1 + index);
+    images are written in decreasing id and each image's observations in
+    decreasing point id, so a reader has to sort.  extra_image adds an image
+    whose only 2D point has no 3D point (POINT3D_ID -1)."""
+    os.makedirs(path, exist_ok=True)
+    nC, nP = sc.num_cams, sc.num_points
+    iid = np.arange(1, nC + 1) if image_id is None else np.asarray(image_id)
+    pid = np.arange(1, nP + 1) if point_id is None else np.asarray(point_id)
+    cx, cy = principal
+    with open(os.path.join(path, "cameras.txt"), "w") as f:
+        f.write("# Camera list with one line of data per camera:\n")
+        f.write("#   CAMERA_ID, MODEL, WIDTH, HEIGHT, PARAMS[]\n")
+        for c in range(nC):
+            fc, k1, k2 = sc.cams[c, 6:9]
+            params = {"SIMPLE_PINHOLE": [fc, cx, cy], "PINHOLE": [fc, fc, cx, cy],
+                      "SIMPLE_RADIAL": [fc, cx, cy, k1], "RADIAL": [fc, cx, cy, k1, k2]}[model]
+            f.write(f"{c + 1} {model} {int(2 * cx)} {int(2 * cy)} "
+                    + " ".join(repr(float(v)) for v in params) + "\n")
+    uv = project(sc.cams[sc.obs_cam], sc.points[sc.obs_pt])
+    q = aa_to_quat(sc.cams[:, 0:3])
+    R = aa_to_R(sc.cams[:, 0:3])
+    t = -np.einsum("cij,cj->ci", R, sc.cams[:, 3:6])
+    with open(os.path.join(path, "images.txt"), "w") as f:
+        f.write("# Image list with two lines of data per image:\n")
+        f.write("#   IMAGE_ID, QW, QX, QY, QZ, TX, TY, TZ, CAMERA_ID, NAME\n")
+        f.write("#   POINTS2D[] as (X, Y, POINT3D_ID)\n")
+        if extra_image:
+            f.write(f"{int(iid.max()) + 5} 1 0 0 0 0 0 0 1 extra.png\n100.5 200.5 -1\n")
+        for c in sorted(range(nC), key=lambda c: -int(iid[c])):
+            vals = [*q[c], *t[c]]
+            f.write(f"{int(iid[c])} " + " ".join(repr(float(v)) for v in vals)
+                    + f" {c + 1} img{c}.png\n")
+            sel = np.nonzero(sc.obs_cam == c)[0]
+            sel = sel[np.argsort(-pid[sc.obs_pt[sel]], kind="stable")]
+            f.write(" ".join(f"{float(uv[o, 0] + cx)!r} {float(uv[o, 1] + cy)!r} "
+                             f"{int(pid[sc.obs_pt[o]])}" for o in sel) + "\n")
+    with open(os.path.join(path, "points3D.txt"), "w") as f:
+        f.write("# 3D point list with one line of data per point:\n")
+        f.write("#   POINT3D_ID, X, Y, Z, R, G, B, ERROR, TRACK[] as (IMAGE_ID, POINT2D_IDX)\n")
+        for p in sorted(range(nP), key=lambda p: -int(pid[p])):
+            X = sc.points[p]
+            f.write(f"{int(pid[p])} {float(X[0])!r} {float(X[1])!r} {float(X[2])!r} "
+                    "128 128 128 0.5\n")
