@@ -235,8 +235,8 @@ int unc_diag_chol_inverse(int n, const double *A, double *L, double *X);
 int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
 /* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
  * device-resident operands (throughput probe); variant 0 is the production
- * kernel (K slab 32), 1-3 are tuning alternatives (occupancy 2, K slab 16,
- * LDS double buffer). */
+ * kernel (8 waves, K slab 16), 1-7 are tuning alternatives (5 is the 4-wave
+ * kernel with K slab 32). */
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
 /* Average duration (ms) of the n x n compensated residual R = I - A X over reps
  * launches; variant 0 is the production tiling (64 x 32), 1 the 64 x 64 one. */

@@ -36,7 +36,8 @@ def block_rel(a, b):
 
 # ----------------------------------------------------------------- kernels
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (130, 77, 45), (300, 257, 129), (576, 576, 576)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (130, 77, 45), (300, 257, 129), (576, 576, 576),
+                                   (1536, 1280, 640), (1537, 1283, 641), (1408, 1664, 1000)])
 def test_gemm_f64(M, N, K):
     say(f"gemm {M}x{N}x{K}")
     rng = np.random.default_rng(M + N + K)

@@ -6,9 +6,10 @@
 // (compute.cpp:64-71).
 //
 // gemm_f64: C = alpha A B + beta C (column-major, no transposes) on the FP64
-// matrix cores (v_mfma_f64_16x16x4_f64).  128x128 tiles, 4 waves of 64x64,
-// K staged through LDS 16 at a time with the next slab prefetched into
-// registers.  Optional fused epilogue: max|C| folded into one 64-bit atomicMax
+// matrix cores (v_mfma_f64_16x16x4_f64).  128x128 tiles, 8 waves of 64x32
+// (k_gemm_f64w8; the 4-wave k_gemm_f64 with 64x64 waves is kept as a tuning
+// variant), K staged through LDS 16 at a time with the next slab prefetched
+// into registers.  Optional fused epilogue: max|C| folded into one 64-bit atomicMax
 // per workgroup (non-negative doubles order like their bit patterns, so the
 // result is order independent), and an early exit when the Taylor control
 // block says the series has stopped (so the whole chain is enqueued at once).
@@ -273,16 +274,23 @@ __global__ __launch_bounds__(G_THREADS, 1) void k_gemm_f64_p(int M, int N, int K
   }
 }
 
-// 8-wave variant: 128x128 tile, waves of 64x32 (two waves per SIMD), K slabs
-// of BK staged through LDS with the next slab prefetched into registers.
-template <int BK>
+// 8-wave kernel (production): 128x128 tile, 8 waves of 64x32 (two waves per
+// SIMD, so one wave's LDS and barrier waits are covered by the other's MFMAs),
+// K slabs of BK staged through LDS with the next slab prefetched into
+// registers.  Same operands, flags and epilogue as k_gemm_f64, and the same
+// per-element accumulation order (k ascending in steps of 4), so the two give
+// identical results.
+template <int BK, bool TRANS_B = false>
 __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, double alpha,
                                                       const double *__restrict__ A, int lda,
                                                       const double *__restrict__ B, int ldb,
                                                       double beta, double *__restrict__ C,
-                                                      int ldc) {
+                                                      int ldc, unsigned long long *absmax,
+                                                      const int *active, int tri) {
+  if (active && *active == 0) return;
   __shared__ double As[BK][GB_M + G_PAD];
   __shared__ double Bs[BK][GB_N + G_PAD];
+  __shared__ double wmax[8];
   constexpr int T = 512;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
@@ -295,6 +303,11 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
   const int tm = first_m + (bid % per_group) % gsize;
   const int tn = (bid % per_group) / gsize;
   const int m0 = tm * GB_M, n0 = tn * GB_N;
+  if ((tri & GEMM_C_LOWER) && n0 >= m0 + GB_M) return;  // tile strictly above the diagonal
+  int kbeg = 0, kend = K;
+  if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GB_M);
+  if (tri & GEMM_B_LOWER) kbeg = n0;
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GB_N);
   double4_t acc[4][2];
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
@@ -303,26 +316,32 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
-      int idx = tid + T * r;
-      int k = idx >> 7, m = idx & 127;
-      int gm = m0 + m, gk = k0 + k;
-      ra[r] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
-      int n = idx / BK, kk = idx % BK;
-      int gn = n0 + n, gk2 = k0 + kk;
-      rb[r] = (gn < N && gk2 < K) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      const int idx = tid + T * r;
+      const int k = idx >> 7, m = idx & 127;
+      const int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < kend) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      if (TRANS_B) {  // B(k, n) = Bmem[n + k ldb]: 128 contiguous n per k
+        const int gn = n0 + m;
+        rb[r] = (gn < N && gk < kend) ? B[(size_t)gn + (size_t)gk * ldb] : 0.0;
+      } else {
+        const int n = idx / BK, kk = idx % BK;
+        const int gn = n0 + n, gk2 = k0 + kk;
+        rb[r] = (gn < N && gk2 < kend) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      }
     }
   };
-  load_tiles(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
-      int idx = tid + T * r;
+      const int idx = tid + T * r;
       As[idx >> 7][idx & 127] = ra[r];
-      Bs[idx % BK][idx / BK] = rb[r];
+      if (TRANS_B) Bs[idx >> 7][idx & 127] = rb[r];
+      else Bs[idx % BK][idx / BK] = rb[r];
     }
     __syncthreads();
-    if (k0 + BK < K) load_tiles(k0 + BK);
+    if (k0 + BK < kend) load_tiles(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
@@ -338,6 +357,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
   }
+  double lmax = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -351,9 +371,20 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
           double *cp = C + (size_t)row + (size_t)col * ldc;
           if (beta != 0.0) v = v + beta * (*cp);
           *cp = v;
+          lmax = fmax(lmax, fabs(v));
         }
       }
     }
+  if (absmax) {
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if (lane == 0) wmax[wave] = lmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = wmax[0];
+      for (int w = 1; w < 8; ++w) m = fmax(m, wmax[w]);
+      atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+    }
+  }
 }
 
 // 8-wave variant with conflict-free LDS rows (stride 144 doubles: rows shift by
@@ -482,14 +513,12 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
       break;
     }
     case 4:
-      if (absmax || active) return hipErrorInvalidValue;
       hipLaunchKernelGGL((k_gemm_f64w8<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
-                         ldb, beta, C, ldc);
+                         ldb, beta, C, ldc, absmax, active, 0);
       break;
     case 5:
-      if (absmax || active) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_gemm_f64w8<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
-                         ldb, beta, C, ldc);
+      hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
+                         A, lda, B, ldb, beta, C, ldc, absmax, active, 0);
       break;
     case 6:
       if (absmax || active) return hipErrorInvalidValue;
@@ -501,9 +530,9 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
       hipLaunchKernelGGL((k_gemm_f64w8p<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
                          B, ldb, beta, C, ldc);
       break;
-    default:  // production: K slabs of 32
-      hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
-                         A, lda, B, ldb, beta, C, ldc, absmax, active, 0);
+    default:  // production: 8 waves, K slabs of 16
+      hipLaunchKernelGGL((k_gemm_f64w8<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
+                         B, ldb, beta, C, ldc, absmax, active, 0);
   }
   return hipGetLastError();
 }
@@ -616,11 +645,11 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
     return hipGetLastError();
   }
   if (trans_b)
-    hipLaunchKernelGGL((k_gemm_f64<32, 1, true>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K,
-                       alpha, A, lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
+    hipLaunchKernelGGL((k_gemm_f64w8<16, true>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
   else
-    hipLaunchKernelGGL((k_gemm_f64<32, 1, false>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K,
-                       alpha, A, lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
+    hipLaunchKernelGGL((k_gemm_f64w8<16, false>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
   return hipGetLastError();
 }
 
