@@ -227,6 +227,34 @@ def measure_serving(u, rate=150.0, n_requests=512):
     return out
 
 
+def copy_calibration() -> int:
+    """A plain copy of known size: 1 GiB read + 1 GiB written per launch."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 1 << 30, "copy_bytes_written_per_launch": 1 << 30,
+                      "launches": 5}), flush=True)
+    return 0
+
+
+def gemm_traffic():
+    """HBM bytes per N x N x N GEMM launch measured offline with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy calibration); recorded in
+    profiles/gemm_rings200_traffic.md as 'traffic_bytes_per_launch: <int>'."""
+    path = os.path.join(ROOT, "profiles", "gemm_rings200_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,7 +274,12 @@ def main() -> int:
                     help="also run the 4000-camera / 3M-point scene (N = 36000), minutes")
     ap.add_argument("--large-fp64", action="store_true",
                     help="with --large, also time the plain FP64 TE path on it (about a minute)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the pipeline is single-GPU",
               file=sys.stderr)
@@ -335,9 +368,9 @@ def main() -> int:
         "config": {"workload": WORKLOADS.get(args.workload, args.workload) + ", " + PATHS[args.precision],
                    "cameras": nC, "points": nP, "observations": nO,
                    "jacobian_nnz": nnz, "N": N, "taylor_terms": terms, "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64w8 (N x N x N)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": gemm_traffic(),
                      "per_launch_flops": flops, "avg_launch_ms": round(avg_gemm_s * 1e3, 4),
                      "launches": gemm_n},
         "cpu_baseline": cpu,
