@@ -62,14 +62,16 @@ extern "C" {
  * pseudo-inverse that drops the 7 gauge eigenvalues, not with the reference. */
 
 typedef struct unc_options {
-  int algorithm;      /* UNC_ALG_*; the device implements TAYLOR_EXPANSION           */
+  int algorithm;      /* UNC_ALG_*: TAYLOR_EXPANSION, or the two SVD modes (dense
+                         pseudo-inverse of J^T J, at most 8192 parameters)          */
   double epsilon;     /* cov::Options::_epsilon (SVD modes only)                      */
   double lambda;      /* -1: 10^(-1.2653 log10(N) - 2.9415), compute.cpp:123-125    */
   int num_cams;
   int cam_params;     /* cp, 1..9                                                     */
   int num_points;
   int num_obs;        /* informational (header of the .jacob)                         */
-  int svd_remove_n;   /* SVD modes only                                               */
+  int svd_remove_n;   /* SVD modes: drop this many smallest singular values; -1 =
+                         drop those <= epsilon instead (compute.cpp:222-240)        */
   int max_iter_te;    /* unused by the reference; kept for layout parity              */
   int pts2fix[3];     /* fixed point ids, -1 -1 -1 = none                             */
   int precision;      /* UNC_PREC_*                                                   */

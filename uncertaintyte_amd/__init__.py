@@ -38,7 +38,7 @@ UNC_ERR_DEVICE = 5
 UNC_ERR_UNSUPPORTED = 6
 UNC_ERR_NUMERIC = 7
 PREC_FP64, PREC_MIXED, PREC_ZPLUS_FP64 = 0, 1, 2
-TAYLOR_EXPANSION = 2
+SVD_QR_ITERATION, SVD_DEVIDE_AND_CONQUER, TAYLOR_EXPANSION = 0, 1, 2
 
 
 class Options(ctypes.Structure):
@@ -169,10 +169,13 @@ class ServerStats(ctypes.Structure):
 
 
 def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
-                 fix=None, lam: Optional[float] = None) -> Options:
+                 fix=None, lam: Optional[float] = None, algorithm: int = TAYLOR_EXPANSION,
+                 svd_remove_n: int = -1, epsilon: float = 1e-10) -> Options:
     o = Options()
     lib().unc_default_options(ctypes.byref(o))
-    o.algorithm = TAYLOR_EXPANSION
+    o.algorithm = algorithm
+    o.svd_remove_n = svd_remove_n
+    o.epsilon = epsilon
     o.lambda_ = float(jac.lam if lam is None else lam)
     o.num_cams, o.cam_params, o.num_points, o.num_obs = (jac.num_cams, jac.cam_params,
                                                          jac.num_points, jac.num_obs)
@@ -204,10 +207,13 @@ def _ptr(a: np.ndarray):
 
 def compute_covariances(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
                         fix=None, lam: Optional[float] = None, debug: bool = False,
-                        ctx=None) -> Result:
-    """Run the pipeline on a host-resident Jacobian (jacob.Jacobian)."""
+                        ctx=None, algorithm: int = TAYLOR_EXPANSION, svd_remove_n: int = -1,
+                        epsilon: float = 1e-10) -> Result:
+    """Run the pipeline on a host-resident Jacobian (jacob.Jacobian).  algorithm
+    SVD_QR_ITERATION / SVD_DEVIDE_AND_CONQUER selects the dense pseudo-inverse
+    modes (small problems only) with svd_remove_n / epsilon as in cov::Options."""
     L = lib()
-    o = make_options(jac, precision, refine, device, fix, lam)
+    o = make_options(jac, precision, refine, device, fix, lam, algorithm, svd_remove_n, epsilon)
     cp = jac.cam_params
     per = cp * (cp + 1) // 2
     cam = np.zeros(jac.num_cams * per, np.float64)

@@ -68,23 +68,28 @@ bool save_results(const std::string &path, const cov::Options &o, const cov::Sta
   out << "# Number of camera parameters: " << o._camParams << "\n";
   out << "# Number of points in 3D: " << o._numPoints << "\n";
   out << "# Number of observations: " << o._numObs << "\n";
-  out << "# Used algorithm: TAYLOR_EXPANSION\n";
-  if (s.fixedPts != NULL)
-    out << "# Fixed points: " << s.fixedPts[0] << ", " << s.fixedPts[1] << ", " << s.fixedPts[2] << "\n";
-  out << "# Used lambda: " << s.lambda << "\n";
-  out << "# Loading jacobian time: " << s.timeCreateJ * tscale << "s\n";
-  out << "# Normalization of jacobian time: " << s.timeNormJ * tscale << "s\n";
-  out << "# Compose information matrix time: " << s.timeMultiplyJJ * tscale << "s\n";
-  out << "# Split infromation matrix time: " << s.timeSplitJJ * tscale << "s\n";
-  out << "# Inverse V time: " << s.timeInvV * tscale << "s\n";
-  out << "# Compose Z time: " << s.timeComposeZ * tscale << "s\n";
-  out << "# Inverse Z time: " << s.timeInvZ * tscale << "s\n";
-  out << "# Taylor expansion time: " << s.timeTE * tscale << "s\n";
-  out << "# Point uncertainty time: " << s.timePtsUnc * tscale << "s\n";
-  out << "# TE number of iterations: " << s.cycle_change.size() << "\n";
-  out << "# TE cycle change: ";
-  for (size_t i = 0; i < s.cycle_change.size(); ++i) out << s.cycle_change[i] << " ";
-  out << "\n";
+  static const char *kAlg[3] = {"SVD_QR_ITERATION", "SVD_DEVIDE_AND_CONQUER", "TAYLOR_EXPANSION"};
+  const bool te = o._algorithm == TAYLOR_EXPANSION;
+  out << "# Used algorithm: " << (o._algorithm >= 0 && o._algorithm < 3 ? kAlg[o._algorithm] : "not defined")
+      << "\n";
+  if (te) {  // the TE lines only, as the reference writes them (IO.cpp:26-44)
+    if (s.fixedPts != NULL)
+      out << "# Fixed points: " << s.fixedPts[0] << ", " << s.fixedPts[1] << ", " << s.fixedPts[2] << "\n";
+    out << "# Used lambda: " << s.lambda << "\n";
+    out << "# Loading jacobian time: " << s.timeCreateJ * tscale << "s\n";
+    out << "# Normalization of jacobian time: " << s.timeNormJ * tscale << "s\n";
+    out << "# Compose information matrix time: " << s.timeMultiplyJJ * tscale << "s\n";
+    out << "# Split infromation matrix time: " << s.timeSplitJJ * tscale << "s\n";
+    out << "# Inverse V time: " << s.timeInvV * tscale << "s\n";
+    out << "# Compose Z time: " << s.timeComposeZ * tscale << "s\n";
+    out << "# Inverse Z time: " << s.timeInvZ * tscale << "s\n";
+    out << "# Taylor expansion time: " << s.timeTE * tscale << "s\n";
+    out << "# Point uncertainty time: " << s.timePtsUnc * tscale << "s\n";
+    out << "# TE number of iterations: " << s.cycle_change.size() << "\n";
+    out << "# TE cycle change: ";
+    for (size_t i = 0; i < s.cycle_change.size(); ++i) out << s.cycle_change[i] << " ";
+    out << "\n";
+  }
   out << "# Time of the algorithm: " << s.timeAll * tscale << "s\n";
   out << "# -----------------------\n";
   for (int i = 0; i < o._numCams; ++i) {

@@ -151,6 +151,18 @@ hipError_t sym_update(int n, const double *X0, const double *D, double *out, hip
 hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st);
 hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
+// SVD algorithm modes (k_svd.hip)
+hipError_t svd_jt_scatter(int nrows, int npar, const int *rows, const int *cols,
+                          const double *vals, double *Jt, unsigned int *err, hipStream_t st);
+hipError_t svd_symmetrize(int n, double *M, hipStream_t st);
+hipError_t svd_identity(int n, double *V, hipStream_t st);
+hipError_t svd_jacobi_round(int n, int m, int r, const double *Aold, double *Anew, double *V,
+                            int *any_rot, hipStream_t st);
+hipError_t svd_diag(int n, const double *A, double *d, hipStream_t st);
+hipError_t svd_cam_blocks(int nc, int cp, int n, const double *V, const double *w, double *Zc,
+                          double *cam_out, hipStream_t st);
+hipError_t svd_points(int nobs, int cp, int nc, int np, const int *cols, const double *vals,
+                      const double *Zc, double *pt_out, unsigned int *err, hipStream_t st);
 // scene Jacobian (k_scene.hip)
 int scene_cam_params(int model);  // cp of a UNC_CAM_* model, -1 if unknown
 hipError_t launch_scene_jacobian(int model, int nc, int np, int nobs, const double *cams,

@@ -305,9 +305,125 @@ int zplus(int N, int newton_steps, double *dA, double *dB, double *dC, double *d
   return UNC_OK;
 }
 
+// ---- SVD algorithm modes (compute.cpp:483-495, svdInverse :185-249), k_svd.hip
+constexpr long long kSvdMaxParams = 8192;          // dense n x n eigen-decomposition
+constexpr long long kSvdMaxJtBytes = 8LL << 30;    // J^T held densely
+
+struct SvdLayout {
+  size_t jt, a0, a1, v, lam, w, zc, err, flag, total;
+};
+
+SvdLayout svd_layout(long long npar, long long nrows, int nc, int cp) {
+  auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
+  SvdLayout L;
+  size_t o = 0;
+  L.jt = o; o += r256(8 * (size_t)npar * (size_t)nrows);
+  L.a0 = o; o += r256(8 * (size_t)npar * (size_t)npar);
+  L.a1 = o; o += r256(8 * (size_t)npar * (size_t)npar);
+  L.v = o; o += r256(8 * (size_t)npar * (size_t)npar);
+  L.lam = o; o += r256(8 * (size_t)npar);
+  L.w = o; o += r256(8 * (size_t)npar);
+  L.zc = o; o += r256(8 * (size_t)nc * cp * cp);
+  L.err = o; o += 256;
+  L.flag = o; o += 256;
+  L.total = o;
+  return L;
+}
+
+int svd_run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
+            const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
+            hipStream_t s) {
+  const int cp = o->cam_params, nc = o->num_cams, np = o->num_points;
+  if (cp < 1 || cp > 9) return UNC_ERR_UNSUPPORTED;
+  if (nc <= 0 || np <= 0 || nrows <= 0 || (nrows & 1)) return UNC_ERR_BAD_INPUT;
+  const long long N = (long long)nc * cp, npar = N + 3LL * np;
+  if ((long long)ncols != npar) return UNC_ERR_BAD_INPUT;
+  if (npar > kSvdMaxParams || 8LL * npar * nrows > kSvdMaxJtBytes) return UNC_ERR_UNSUPPORTED;
+  const int n = (int)npar, m = n + (n & 1);
+  hipEvent_t *E = ctx->stage_ev;
+  HC(hipEventRecord(E[1], s));
+  const SvdLayout L = svd_layout(npar, nrows, nc, cp);
+  int rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total, s);
+  if (rc) return rc;
+  char *base = ctx->ws;
+  double *Jt = (double *)(base + L.jt), *A0 = (double *)(base + L.a0);
+  double *A1 = (double *)(base + L.a1), *V = (double *)(base + L.v);
+  double *lam = (double *)(base + L.lam), *wt = (double *)(base + L.w);
+  double *Zc = (double *)(base + L.zc);
+  unsigned int *err = (unsigned int *)(base + L.err);
+  int *flag = (int *)(base + L.flag);
+  HC(hipMemsetAsync(err, 0, 256, s));
+  HC(hipMemsetAsync(Jt, 0, 8 * (size_t)npar * (size_t)nrows, s));
+  HC(svd_jt_scatter(nrows, n, d_rows, d_cols, d_vals, Jt, err, s));
+  // M = J^T J (unscaled, every camera and point), svdInverse's Z (compute.cpp:186)
+  HC(gemm_f64_ex(n, n, nrows, 1.0, Jt, n, Jt, n, true, 0.0, A0, n, 0, s));
+  HC(svd_symmetrize(n, A0, s));
+  HC(svd_identity(n, V, s));
+  HC(hipEventRecord(E[5], s));
+  double *Ao = A0, *An = A1;
+  int sweeps = 0;
+  for (; sweeps < 40; ++sweeps) {
+    HC(hipMemsetAsync(flag, 0, sizeof(int), s));
+    for (int r = 0; r < m - 1; ++r) {
+      HC(svd_jacobi_round(n, m, r, Ao, An, V, flag, s));
+      std::swap(Ao, An);
+    }
+    int any = 0;
+    HC(hipMemcpyAsync(&any, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    if (!any) break;
+  }
+  if (sweeps >= 40) return UNC_ERR_NUMERIC;
+  // singular values = |eigenvalues|; the kept ones (svdInverse :222-240) get 1/lambda
+  HC(svd_diag(n, Ao, lam, s));
+  std::vector<double> hl(n), hw(n, 0.0);
+  HC(hipMemcpyAsync(hl.data(), lam, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return std::fabs(hl[a]) > std::fabs(hl[b]); });
+  if (o->svd_remove_n != -1) {
+    const long long keep = std::max(0LL, std::min((long long)n, (long long)n - o->svd_remove_n));
+    for (long long i = 0; i < keep; ++i) hw[order[i]] = 1.0 / hl[order[i]];
+  } else {
+    const double eps = o->epsilon < 0 ? 1e-10 : o->epsilon;
+    for (int i = 0; i < n; ++i)
+      if (std::fabs(hl[i]) > eps) hw[i] = 1.0 / hl[i];
+  }
+  HC(hipMemcpyAsync(wt, hw.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
+  HC(hipEventRecord(E[7], s));
+  HC(svd_cam_blocks(nc, cp, n, V, wt, Zc, d_cam, s));
+  HC(hipEventRecord(E[8], s));
+  HC(svd_points(nrows / 2, cp, nc, np, d_cols, d_vals, Zc, d_pt, err, s));
+  HC(hipEventRecord(E[10], s));
+  unsigned int herr = 0;
+  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  if (herr) return map_err(herr);
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->fixed_pts[0] = st->fixed_pts[1] = st->fixed_pts[2] = -1;
+    st->num_obs_kept = nrows / 2;
+    st->stage_ms[5] = ev_ms(E[1], E[5]);    // J^T J
+    st->stage_ms[6] = ev_ms(E[5], E[7]);    // eigen-decomposition + pseudo-inverse weights
+    st->stage_ms[8] = ev_ms(E[7], E[8]);
+    st->stage_ms[9] = ev_ms(E[8], E[10]);
+    st->stage_ms[11] = ev_ms(E[1], E[10]);
+    st->time_create_j = st->stage_ms[5] * 1e-3;
+    st->time_inv_z = st->stage_ms[6] * 1e-3;
+    st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
+    st->time_all = st->stage_ms[11] * 1e-3;
+  }
+  return UNC_OK;
+}
+
 int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
         const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
         unc_debug *dbg, hipStream_t s, bool host_api_upload_timed) {
+  if (o && (o->algorithm == UNC_ALG_SVD_QR_ITERATION ||
+            o->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER))
+    return svd_run(ctx, o, nrows, ncols, d_rows, d_cols, d_vals, d_cam, d_pt, st, s);
   Sizes z;
   int fix[3];
   int rc = check_options(o, nrows, ncols, &z, fix);
@@ -904,6 +1020,11 @@ int unc_scene_jacobian(unc_context *ctx, int cam_model, int num_cams, int num_po
 
 unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   if (!o || o->num_cams <= 0 || o->cam_params <= 0 || o->num_points < 4 || num_rows <= 0) return 0;
+  if (o->algorithm == UNC_ALG_SVD_QR_ITERATION || o->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER)
+    return (unsigned long long)svd_layout((long long)o->num_cams * o->cam_params +
+                                              3LL * o->num_points,
+                                          num_rows, o->num_cams, o->cam_params)
+        .total;
   Sizes z;
   z.nc = o->num_cams; z.cp = o->cam_params; z.np = o->num_points; z.npc = z.np - 3;
   z.nobs = num_rows / 2; z.N = z.nc * z.cp; z.nn = (size_t)z.N * z.N;
