@@ -224,6 +224,12 @@ int unc_scene_jacobian_device(unc_context *ctx, int cam_model, int num_cams, int
 int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                       const double *B, int ldb, double beta, double *C, int ldc,
                       double *absmax_out);
+/* C = alpha A B by the split-K form of the same GEMM (two workgroups per tile,
+   each over half of K, then an epilogue kernel that adds the halves and folds
+   max|C|); UNC_ERR_UNSUPPORTED where the library would not split this shape
+   (128 x 128 tiles between 96 and 256, K >= 1024). */
+int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
+                            const double *B, int ldb, double *C, int ldc, double *absmax_out);
 int unc_diag_inverse(int n, const double *A, double *X, int refine);
 int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
 /* Cholesky A = L L^T of SPD A (n x n, column-major) and X = L^-1 by the recursive
@@ -238,7 +244,7 @@ int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
 /* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
  * device-resident operands (throughput probe); variant 0 is the production
  * kernel (8 waves, K slab 16), 1-7 are tuning alternatives (5 is the 4-wave
- * kernel with K slab 32). */
+ * kernel with K slab 32), 8 the split-K form of the production kernel. */
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
 /* Average duration (ms) of the n x n compensated residual R = I - A X over reps
  * launches; variant 0 is the production tiling (64 x 32), 1 the 64 x 64 one. */

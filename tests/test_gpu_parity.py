@@ -50,6 +50,31 @@ def test_gemm_f64(M, N, K):
     assert m == np.abs(C).max()
 
 
+@pytest.mark.parametrize("M,N,K", [(1800, 1800, 1800), (1537, 1283, 1031), (1280, 2560, 4099)])
+def test_gemm_f64_split_k(M, N, K):
+    """The split-K form (two workgroups per 128 x 128 tile, one per half of K,
+    then the epilogue kernel adding the halves): the same product as the
+    unsplit kernel to FP64 rounding, ragged edges and a K tail that is not a
+    multiple of the 16-wide slab included, and max|C| from the epilogue."""
+    say(f"gemm split {M}x{N}x{K}")
+    rng = np.random.default_rng(M * 3 + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C, m = u.diag_gemm(A, B, rng.standard_normal((M, N)), alpha=-0.75, split=True)
+    ref = -0.75 * (A @ B)
+    assert np.abs(C - ref).max() <= 1e-13 * np.abs(A).max() * np.abs(B).max() * K
+    assert m == np.abs(C).max()
+    C1, _ = u.diag_gemm(A, B, alpha=-0.75)
+    assert np.abs(C - C1).max() <= 1e-14 * np.abs(A).max() * np.abs(B).max() * K
+
+
+def test_gemm_f64_split_k_declines_small_shapes():
+    A = np.ones((256, 1024))
+    with pytest.raises(u.UncError) as e:
+        u.diag_gemm(A, A.T.copy(), split=True)
+    assert e.value.code == u.UNC_ERR_UNSUPPORTED
+
+
 def test_residual_dd_kernel():
     say("residual_dd")
     rng = np.random.default_rng(7)

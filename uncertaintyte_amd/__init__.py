@@ -111,6 +111,8 @@ def lib():
     L.unc_diag_gemm_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp,
                                     ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp,
                                     ctypes.c_int, dp]
+    L.unc_diag_gemm_f64_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, dp]
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
     L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
@@ -286,7 +288,11 @@ def workspace_bytes(opts: Options, num_rows: int) -> int:
 # kernel diagnostics (device 0, host arrays; numpy [row, col] convention)
 # --------------------------------------------------------------------------
 
-def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0):
+def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0,
+              split: bool = False):
+    """C = alpha A B + beta C on the device; split=True takes the split-K form
+    (beta must be 0; raises UncError UNC_ERR_UNSUPPORTED where the library
+    would not split the shape)."""
     M, K = A.shape
     K2, N = B.shape
     assert K == K2
@@ -294,9 +300,16 @@ def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alph
     Bf = np.asfortranarray(B, dtype=np.float64)
     Cf = np.asfortranarray(np.zeros((M, N)) if C is None else C, dtype=np.float64).copy(order="F")
     m = ctypes.c_double()
-    rc = lib().unc_diag_gemm_f64(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K, beta,
-                                 Cf.ctypes.data, M, ctypes.byref(m))
-    _check(rc, "unc_diag_gemm_f64")
+    if split:
+        if beta != 0.0:
+            raise ValueError("the split-K form computes C = alpha A B")
+        rc = lib().unc_diag_gemm_f64_split(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K,
+                                           Cf.ctypes.data, M, ctypes.byref(m))
+        _check(rc, "unc_diag_gemm_f64_split")
+    else:
+        rc = lib().unc_diag_gemm_f64(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K, beta,
+                                     Cf.ctypes.data, M, ctypes.byref(m))
+        _check(rc, "unc_diag_gemm_f64")
     return np.asarray(Cf), m.value
 
 

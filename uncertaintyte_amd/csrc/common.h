@@ -108,6 +108,13 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                     int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
                     const int *active, hipStream_t st);
 void set_gemm_variant(int v);
+// C = alpha A B with the K range split over two workgroups per tile where that
+// fills idle CUs (gemm_split_applies); work: M x N doubles of scratch, or
+// nullptr for the unsplit kernel.  Same absmax / active contract as gemm_f64.
+bool gemm_split_applies(int M, int N, int K);
+hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
+                          const double *B, int ldb, double *C, int ldc, double *work,
+                          unsigned long long *absmax, const int *active, hipStream_t st);
 // triangular-operand flags of gemm_f64_ex
 constexpr int GEMM_A_LOWER = 1;   // A lower triangular: K range ends at the tile's last row
 constexpr int GEMM_B_LOWER = 2;   // B lower triangular: K range starts at the tile's first column

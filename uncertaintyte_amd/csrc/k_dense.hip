@@ -280,13 +280,20 @@ __global__ __launch_bounds__(G_THREADS, 1) void k_gemm_f64_p(int M, int N, int K
 // registers.  Same operands, flags and epilogue as k_gemm_f64, and the same
 // per-element accumulation order (k ascending in steps of 4), so the two give
 // identical results.
-template <int BK, bool TRANS_B = false>
+//
+// SPLIT: the grid holds two workgroups per tile, each summing one half of the
+// K range (split at a multiple of BK); the first half goes to C and the second
+// to `part1` (M x N, leading dimension M), both unscaled, and
+// k_split_reduce applies the epilogue.  Used where the tiles alone would leave
+// CUs idle (N = 1800 gives 225 tiles for 256 CUs, two resident workgroups each).
+template <int BK, bool TRANS_B = false, bool SPLIT = false>
 __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, double alpha,
                                                       const double *__restrict__ A, int lda,
                                                       const double *__restrict__ B, int ldb,
                                                       double beta, double *__restrict__ C,
                                                       int ldc, unsigned long long *absmax,
-                                                      const int *active, int tri) {
+                                                      const int *active, int tri,
+                                                      double *__restrict__ part1) {
   if (active && *active == 0) return;
   __shared__ double As[BK][GB_M + G_PAD];
   __shared__ double Bs[BK][GB_N + G_PAD];
@@ -295,7 +302,12 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
   const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
-  const int bid = blockIdx.x;
+  int bid = blockIdx.x, part = 0;
+  if (SPLIT) {
+    const int nt = tiles_m * tiles_n;
+    part = bid >= nt;
+    bid -= part * nt;
+  }
   const int group = 8;
   const int per_group = group * tiles_n;
   const int first_m = (bid / per_group) * group;
@@ -308,6 +320,11 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
   if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GB_M);
   if (tri & GEMM_B_LOWER) kbeg = n0;
   if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GB_N);
+  if (SPLIT) {  // tri is 0 here
+    const int half = min(K, ((K + 1) / 2 + BK - 1) / BK * BK);
+    if (part) kbeg = half;
+    else kend = half;
+  }
   double4_t acc[4][2];
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
@@ -357,6 +374,22 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
   }
+  if (SPLIT) {
+    double *dst = part ? part1 : C;
+    const int ld = part ? M : ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+          if (row < M && col < N) dst[(size_t)row + (size_t)col * ld] = acc[i][j][r];
+        }
+      }
+    return;
+  }
   double lmax = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -385,6 +418,65 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
       atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
     }
   }
+}
+
+// Epilogue of the split product: C = alpha (C + W), with the max|C| fold and
+// the early exit of the unsplit kernel.  k_split_reduce_flat takes C stored
+// densely (ldc == M, even size, 16-byte aligned) as one array and moves two
+// double2 per thread per step; k_split_reduce goes column by column.
+__device__ inline void fold_absmax(double lmax, unsigned long long *absmax, double *wmax) {
+  const int tid = threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+  if ((tid & 63) == 0) wmax[tid >> 6] = lmax;
+  __syncthreads();
+  if (tid == 0) {
+    const double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_split_reduce_flat(size_t n4, double alpha,
+                                                           double *__restrict__ C,
+                                                           const double *__restrict__ W,
+                                                           unsigned long long *absmax,
+                                                           const int *active) {
+  if (active && *active == 0) return;
+  __shared__ double wmax[4];
+  double2 *c2 = (double2 *)C;
+  const double2 *w2 = (const double2 *)W;
+  double lmax = 0.0;
+  // n4 = number of 4-element groups
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n4; q += (size_t)gridDim.x * 256) {
+    const double2 a0 = c2[2 * q], a1 = c2[2 * q + 1];
+    const double2 b0 = w2[2 * q], b1 = w2[2 * q + 1];
+    double2 r0, r1;
+    r0.x = alpha * (a0.x + b0.x);
+    r0.y = alpha * (a0.y + b0.y);
+    r1.x = alpha * (a1.x + b1.x);
+    r1.y = alpha * (a1.y + b1.y);
+    c2[2 * q] = r0;
+    c2[2 * q + 1] = r1;
+    lmax = fmax(lmax, fmax(fmax(fabs(r0.x), fabs(r0.y)), fmax(fabs(r1.x), fabs(r1.y))));
+  }
+  if (absmax) fold_absmax(lmax, absmax, wmax);
+}
+
+__global__ __launch_bounds__(256) void k_split_reduce(int M, double alpha, double *__restrict__ C,
+                                                      int ldc, const double *__restrict__ W,
+                                                      unsigned long long *absmax,
+                                                      const int *active) {
+  if (active && *active == 0) return;
+  __shared__ double wmax[4];
+  const int col = blockIdx.x, tid = threadIdx.x;
+  double *c = C + (size_t)col * ldc;
+  const double *w = W + (size_t)col * M;
+  double lmax = 0.0;
+  for (int row = tid; row < M; row += 256) {
+    const double v = alpha * (c[row] + w[row]);
+    c[row] = v;
+    lmax = fmax(lmax, fabs(v));
+  }
+  if (absmax) fold_absmax(lmax, absmax, wmax);
 }
 
 // 8-wave variant with conflict-free LDS rows (stride 144 doubles: rows shift by
@@ -514,7 +606,7 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
     }
     case 4:
       hipLaunchKernelGGL((k_gemm_f64w8<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
-                         ldb, beta, C, ldc, absmax, active, 0);
+                         ldb, beta, C, ldc, absmax, active, 0, nullptr);
       break;
     case 5:
       hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
@@ -532,7 +624,39 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
       break;
     default:  // production: 8 waves, K slabs of 16
       hipLaunchKernelGGL((k_gemm_f64w8<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
-                         B, ldb, beta, C, ldc, absmax, active, 0);
+                         B, ldb, beta, C, ldc, absmax, active, 0, nullptr);
+  }
+  return hipGetLastError();
+}
+
+// Split-K applies while the doubled grid still fits the resident slots (two
+// 512-thread workgroups per CU on 256 CUs) and each half keeps >= 512 of K.
+constexpr int SPLIT_MAX_TILES = 256, SPLIT_MIN_K = 1024;
+
+bool gemm_split_applies(int M, int N, int K) {
+  const long long tiles = cdiv(M, GB_M) * cdiv(N, GB_N);
+  return g_gemm_variant == 0 && tiles >= SMALL_GEMM_TILES && tiles <= SPLIT_MAX_TILES &&
+         K >= SPLIT_MIN_K;
+}
+
+hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
+                          const double *B, int ldb, double *C, int ldc, double *work,
+                          unsigned long long *absmax, const int *active, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (!work || !gemm_split_applies(M, N, K))
+    return gemm_f64(M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, absmax, active, st);
+  const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
+  hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(2 * tiles), dim3(512), 0, st, M, N, K,
+                     1.0, A, lda, B, ldb, 0.0, C, ldc, nullptr, active, 0, work);
+  const size_t mn = (size_t)M * N;
+  if (ldc == M && mn % 4 == 0 && ((uintptr_t)C & 15) == 0 && ((uintptr_t)work & 15) == 0) {
+    const size_t n4 = mn / 4;
+    const int grid = (int)std::min<size_t>(cdiv((long long)n4, 256), 2048);
+    hipLaunchKernelGGL(k_split_reduce_flat, dim3(grid), dim3(256), 0, st, n4, alpha, C, work,
+                       absmax, active);
+  } else {
+    hipLaunchKernelGGL(k_split_reduce, dim3(N), dim3(256), 0, st, M, alpha, C, ldc, work, absmax,
+                       active);
   }
   return hipGetLastError();
 }
@@ -646,10 +770,10 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
   }
   if (trans_b)
     hipLaunchKernelGGL((k_gemm_f64w8<16, true>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr);
   else
     hipLaunchKernelGGL((k_gemm_f64w8<16, false>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri);
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr);
   return hipGetLastError();
 }
 

@@ -574,7 +574,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (refine) {
     HC(residual_dd(N, dA, dB, dC, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64(N, N, N, 1.0, dB, N, dC, N, 0.0, dD, N, nullptr, nullptr, s));
+    HC(gemm_f64_split(N, N, N, 1.0, dB, N, dC, N, dD, N, dE, nullptr, nullptr, s));  // dE: scratch
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     HC(sym_update(N, dB, dD, dE, s));
@@ -584,7 +584,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (dbg && dbg->X0) HC(hipMemcpyAsync(dbg->X0, X0, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   // ---- Taylor chain (compute.cpp:142-160); X in dA (A is dead), T ping-pongs
   double *X = dA;
-  double *Tb0 = refine ? dC : dC, *Tb1 = dD;
+  double *Tb0 = dC, *Tb1 = dD;
+  double *split_work = refine ? dB : dE;  // the buffer X0 does not occupy
   HC(hipMemcpyAsync(X, X0, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
   {
     TeCtrl h;
@@ -600,7 +601,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   }
   const int first_chain_pair = gemm_pairs;
   HC(hipEventRecord(GE[2 * gemm_pairs], s));
-  HC(gemm_f64(N, N, N, 1.0, X0, N, X0, N, 0.0, Tb0, N, &ctrl->maxbits[1], nullptr, s));
+  HC(gemm_f64_split(N, N, N, 1.0, X0, N, X0, N, Tb0, N, split_work, &ctrl->maxbits[1], nullptr, s));
   HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
   ++gemm_pairs;
   double *Tc = Tb0, *Tn = Tb1;
@@ -609,7 +610,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     HC(te_control(ctrl, i, k, s));
     HC(te_axpy(N, X, Tc, ctrl, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64(N, N, N, 1.0, Tc, N, X0, N, 0.0, Tn, N, &ctrl->maxbits[i + 1], &ctrl->active, s));
+    HC(gemm_f64_split(N, N, N, 1.0, Tc, N, X0, N, Tn, N, split_work, &ctrl->maxbits[i + 1],
+                      &ctrl->active, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     std::swap(Tc, Tn);
@@ -1092,6 +1094,36 @@ int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int ld
   return UNC_OK;
 }
 
+int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
+                            const double *B, int ldb, double *C, int ldc, double *absmax_out) {
+  if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < K || ldc < M) return UNC_ERR_BAD_INPUT;
+  if (!gemm_split_applies(M, N, K)) return UNC_ERR_UNSUPPORTED;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N,
+               nw = (size_t)M * N;
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * (na + nb + nc + nw) + 256));
+  double *dA = (double *)buf, *dB = dA + na, *dC = dB + nb, *dW = dC + nc;
+  unsigned long long *dm = (unsigned long long *)(dW + nw);
+  HC(hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
+  HC(gemm_f64_split(M, N, K, alpha, dA, lda, dB, ldb, dC, ldc, dW, dm, nullptr, s));
+  unsigned long long mb = 0;
+  HC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
+  return UNC_OK;
+}
+
 __global__ void k_fill_pattern(size_t n, double *A, double scale) {
   for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n;
        x += (size_t)gridDim.x * blockDim.x)
@@ -1099,7 +1131,7 @@ __global__ void k_fill_pattern(size_t n, double *A, double scale) {
 }
 
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 7) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 8) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
@@ -1108,16 +1140,22 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
   hipStream_t s = ctx->stream;
   const size_t nn = (size_t)n * n;
   double *buf = nullptr;
-  if (hipMalloc((void **)&buf, sizeof(double) * 3 * nn) != hipSuccess) {
+  if (hipMalloc((void **)&buf, sizeof(double) * 4 * nn) != hipSuccess) {
     (void)hipGetLastError();
     return UNC_ERR_OOM;
   }
   hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
-  set_gemm_variant(variant);
-  HC(gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr, nullptr, s));
+  const bool split = variant == 8;
+  set_gemm_variant(split ? 0 : variant);
+  auto one = [&]() {
+    return split ? gemm_f64_split(n, n, n, 1.0, buf, n, buf + nn, n, buf + 2 * nn, n, buf + 3 * nn,
+                                  nullptr, nullptr, s)
+                 : gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr,
+                            nullptr, s);
+  };
+  HC(one());
   HC(hipEventRecord(ctx->gemm_ev[0], s));
-  for (int r = 0; r < reps; ++r)
-    HC(gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr, nullptr, s));
+  for (int r = 0; r < reps; ++r) HC(one());
   HC(hipEventRecord(ctx->gemm_ev[1], s));
   HC(hipEventSynchronize(ctx->gemm_ev[1]));
   *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
