@@ -24,7 +24,8 @@ CLI = os.path.join(OUT, "uncertainty")
 HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "k_zplus.hip", "k_eig.hip", "k_scene.hip", "k_svd.hip", "pipeline.hip"]
 # MFMA accumulators in VGPR form: the loop-carried accumulators of the GEMM
 # then stay in place instead of being copied between AGPRs and VGPRs each slab.
-PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+                  "k_zplus.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CPP_SOURCES = ["uncertainty_api.cpp", "uncertainty_te.cpp", "scene_io.cpp", "server.cpp"]
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")

@@ -130,6 +130,14 @@ void set_leaf_variant(int v);  // 0: panelled, 256 threads (default), 1: compact
 // P = W^-1 of a b x b SPD block (b <= 64) by Cholesky, L^-1 and X^T X (k_zplus.hip)
 hipError_t spd_inverse64(int b, const double *W, int ldw, double *P, unsigned int *err,
                          hipStream_t st);
+// blocked Gauss-Jordan panel kernels (k_zplus.hip): the first half of a panel
+// (T, R and the copy of the next diagonal block), then the trailing update,
+// the panel's rows and columns and the next panel's P in one launch
+hipError_t gj_panel_a(int n, int k0, int kb, const double *W, const double *P, double *T,
+                      double *R, int k1, int kb1, double *Dsrc, hipStream_t st);
+hipError_t gj_update(int n, int k0, int kb, double *W, const double *P, const double *T,
+                     const double *R, int kb1, const double *Dsrc, double *Pn, unsigned int *err,
+                     hipStream_t st);
 hipError_t skinny_tn(int n, int kc, const double *A, int lda, const double *B, double *C,
                      bool lower, hipStream_t st);
 hipError_t skinny_nn(int n, int kc, const double *A, int lda, const double *B, double *C,

@@ -1,0 +1,153 @@
+// gemm_tile.h -- the 128 x 128 FP64 MFMA tile of the 8-wave GEMM as a device
+// function, shared by the GEMM kernels (k_dense.hip) and the fused
+// Gauss-Jordan step (k_zplus.hip).
+#pragma once
+#include "common.h"
+
+namespace unc {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+constexpr int GB_M = 128, GB_N = 128;
+constexpr int G_PAD = 4;
+
+// One 128 x 128 tile of C = alpha op(A) B + beta C for the workgroup's tile
+// index tile_bid (512 threads, 8 waves of 64 x 32; tiles in 8-row groups so
+// neighbouring workgroups share A panels in L2).  tri: triangular-operand flags
+// (GEMM_*); SPLIT: see k_gemm_f64w8.  Rows and columns of C in [skip0, skip1)
+// are neither read nor written, which lets a Gauss-Jordan step leave the
+// panel's band to other workgroups of the same launch.
+template <int BK, bool TRANS_B, bool SPLIT>
+__device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, double alpha,
+                                             const double *__restrict__ A, int lda,
+                                             const double *__restrict__ B, int ldb, double beta,
+                                             double *__restrict__ C, int ldc,
+                                             unsigned long long *absmax, int tri,
+                                             double *__restrict__ part1, int skip0, int skip1) {
+  __shared__ double As[BK][GB_M + G_PAD];
+  __shared__ double Bs[BK][GB_N + G_PAD];
+  __shared__ double wmax[8];
+  constexpr int T = 512;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
+  const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
+  int bid = tile_bid, part = 0;
+  if (SPLIT) {
+    const int nt = tiles_m * tiles_n;
+    part = bid >= nt;
+    bid -= part * nt;
+  }
+  const int group = 8;
+  const int per_group = group * tiles_n;
+  const int first_m = (bid / per_group) * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int tm = first_m + (bid % per_group) % gsize;
+  const int tn = (bid % per_group) / gsize;
+  const int m0 = tm * GB_M, n0 = tn * GB_N;
+  if ((tri & GEMM_C_LOWER) && n0 >= m0 + GB_M) return;  // tile strictly above the diagonal
+  int kbeg = 0, kend = K;
+  if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GB_M);
+  if (tri & GEMM_B_LOWER) kbeg = n0;
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GB_N);
+  if (SPLIT) {  // tri is 0 here
+    const int half = min(K, ((K + 1) / 2 + BK - 1) / BK * BK);
+    if (part) kbeg = half;
+    else kend = half;
+  }
+  double4_t acc[4][2];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = BK * 128 / T;
+  double ra[NL], rb[NL];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + T * r;
+      const int k = idx >> 7, m = idx & 127;
+      const int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < kend) ? A[(size_t)gm + (size_t)gk * lda] : 0.0;
+      if (TRANS_B) {  // B(k, n) = Bmem[n + k ldb]: 128 contiguous n per k
+        const int gn = n0 + m;
+        rb[r] = (gn < N && gk < kend) ? B[(size_t)gn + (size_t)gk * ldb] : 0.0;
+      } else {
+        const int n = idx / BK, kk = idx % BK;
+        const int gn = n0 + n, gk2 = k0 + kk;
+        rb[r] = (gn < N && gk2 < kend) ? B[(size_t)gk2 + (size_t)gn * ldb] : 0.0;
+      }
+    }
+  };
+  load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = tid + T * r;
+      As[idx >> 7][idx & 127] = ra[r];
+      if (TRANS_B) Bs[idx >> 7][idx & 127] = rb[r];
+      else Bs[idx % BK][idx / BK] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + BK < kend) load_tiles(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double af[4], bf[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  if (SPLIT) {
+    double *dst = part ? part1 : C;
+    const int ld = part ? M : ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+          if (row < M && col < N) dst[(size_t)row + (size_t)col * ld] = acc[i][j][r];
+        }
+      }
+    return;
+  }
+  double lmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+      const bool col_ok = col < N && (col < skip0 || col >= skip1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+        if (row < M && col_ok && (row < skip0 || row >= skip1)) {
+          double v = alpha * acc[i][j][r];
+          double *cp = C + (size_t)row + (size_t)col * ldc;
+          if (beta != 0.0) v = v + beta * (*cp);
+          *cp = v;
+          lmax = fmax(lmax, fabs(v));
+        }
+      }
+    }
+  if (absmax) {
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+    if (lane == 0) wmax[wave] = lmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = wmax[0];
+      for (int w = 1; w < 8; ++w) m = fmax(m, wmax[w]);
+      atomicMax(absmax, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+}  // namespace unc

@@ -21,6 +21,7 @@
 //
 // Skinny products with 7 or 14 right-hand columns stream X or Z once from HBM.
 #include "common.h"
+#include "gemm_tile.h"
 
 namespace unc {
 
@@ -182,8 +183,9 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 
 // S: the SPD block (lower triangle valid, upper triangle zero) -> L with zeros
 // above the diagonal; Xs -> L^-1 with zeros above.  Returns true in wave 0 when
-// a pivot was not positive.
-__device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
+// a pivot was not positive.  The first 256 threads work (act); in a larger
+// workgroup the others pass only the barriers.
+__device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], bool act = true) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   bool bad = false;
   for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
@@ -210,7 +212,7 @@ __device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
     __syncthreads();
     const int q0 = p0 + CP_W;
     const int i = q0 + lane;
-    if (i < CL_B) {
+    if (act && i < CL_B) {
       double li[CP_W];
 #pragma unroll
       for (int jj = 0; jj < CP_W; ++jj) li[jj] = S[i][p0 + jj];
@@ -223,10 +225,11 @@ __device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
     }
     __syncthreads();
   }
-  for (int x = t; x < CL_B * CL_B; x += CP_THREADS) {
-    const int r = x >> 6, c = x & 63;
-    Xs[r][c] = r == c ? 1.0 : 0.0;
-  }
+  if (act)
+    for (int x = t; x < CL_B * CL_B; x += CP_THREADS) {
+      const int r = x >> 6, c = x & 63;
+      Xs[r][c] = r == c ? 1.0 : 0.0;
+    }
   __syncthreads();
   for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
     if (wave == 0) {
@@ -246,7 +249,7 @@ __device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
     }
     __syncthreads();
     const int q0 = p0 + CP_W;
-    if (q0 < CL_B) {
+    if (act && q0 < CL_B) {
       const int c = lane;
       double xk[CP_W];
 #pragma unroll
@@ -263,14 +266,34 @@ __device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1]) {
   return bad;
 }
 
+// Dst[i][k] = f(i, k) over the 64 x 64 tile, (i, k) = (x & 63, x >> 6) for
+// x = tid + 256 r: each thread issues its 16 loads before the first LDS store,
+// so the global latency is paid once, not 16 times.  Only the first 256
+// threads take part (act).
+constexpr int STAGE_R = CL_B * CL_B / CP_THREADS;
+template <typename F>
+__device__ __forceinline__ void stage64(double (*Dst)[CL_B + 1], F f, bool act = true) {
+  if (!act) return;
+  double v[STAGE_R];
+#pragma unroll
+  for (int r = 0; r < STAGE_R; ++r) {
+    const int x = threadIdx.x + CP_THREADS * r;
+    v[r] = f(x & 63, x >> 6);
+  }
+#pragma unroll
+  for (int r = 0; r < STAGE_R; ++r) {
+    const int x = threadIdx.x + CP_THREADS * r;
+    Dst[x & 63][x >> 6] = v[r];
+  }
+}
+
 // b x b block of A (column-major, lower triangle read) -> S, identity padding
 __device__ __forceinline__ void load_spd64(int b, const double *A, int lda,
                                            double (*S)[CL_B + 1]) {
-  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
-    const int i = x & 63, k = x >> 6;
-    S[i][k] = (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0)
-                               : (i == k ? 1.0 : 0.0);
-  }
+  stage64(S, [&](int i, int k) {
+    return (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0)
+                            : (i == k ? 1.0 : 0.0);
+  });
   __syncthreads();
 }
 
@@ -291,16 +314,10 @@ __global__ __launch_bounds__(CP_THREADS) void k_chol_inv64p(int b, double *A, in
   if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
-// P = W^-1 = X^T X (X = L^-1) for the b x b SPD diagonal block of the blocked
-// Gauss-Jordan inverse; P is b x b with leading dimension b.  Each thread forms
-// a 4 x 4 set of entries, summing k in the same order for (i, j) and (j, i), so
-// P is exactly symmetric.
-__global__ __launch_bounds__(CP_THREADS) void k_spd_inv64(int b, const double *W, int ldw,
-                                                          double *P, unsigned int *err) {
-  __shared__ double S[CL_B][CL_B + 1];
-  __shared__ double Xs[CL_B][CL_B + 1];
-  load_spd64(b, W, ldw, S);
-  const bool bad = chol_inv_lds(S, Xs);
+// P = X^T X (X = L^-1 in Xs) -> P (b x b, leading dimension b).  Each thread
+// forms a 4 x 4 set of entries, summing k in the same order for (i, j) and
+// (j, i), so P is exactly symmetric.
+__device__ __forceinline__ void store_xtx(int b, double (*Xs)[CL_B + 1], double *P) {
   const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
   double acc[4][4];
 #pragma unroll
@@ -325,7 +342,216 @@ __global__ __launch_bounds__(CP_THREADS) void k_spd_inv64(int b, const double *W
       const int i = ti + 16 * u, j = tj + 16 * v;
       if (i < b && j < b) P[(size_t)i + (size_t)j * b] = acc[u][v];
     }
+}
+
+// 64 x 64 block product acc(4 x 4 per thread) = As[0:64][0:kk] Bs[0:kk][0:64]
+// (thread (ti, tj) of the first 256 owns rows ti + 16u, columns tj + 16v);
+// k ascending.
+__device__ __forceinline__ void lds_product64(int kk, double (*As)[CL_B + 1],
+                                              double (*Bs)[CL_B + 1], double acc[4][4]) {
+  const int ti = threadIdx.x & 15, tj = (threadIdx.x >> 4) & 15;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int k = 0; k < kk; ++k) {
+    double a[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = As[ti + 16 * u][k];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) c[v] = Bs[k][tj + 16 * v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], c[v], acc[u][v]);
+  }
+}
+
+// P = W^-1 for the b x b SPD diagonal block of the blocked Gauss-Jordan
+// inverse (Cholesky, L^-1, X^T X); P is b x b with leading dimension b.
+__global__ __launch_bounds__(CP_THREADS) void k_spd_inv64(int b, const double *W, int ldw,
+                                                          double *P, unsigned int *err) {
+  __shared__ double S[CL_B][CL_B + 1];
+  __shared__ double Xs[CL_B][CL_B + 1];
+  load_spd64(b, W, ldw, S);
+  const bool bad = chol_inv_lds(S, Xs);
+  store_xtx(b, Xs, P);
   if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
+// First half of a Gauss-Jordan panel K = [k0, k0 + kb) on W (n x n, ld n),
+// one workgroup per 64-row / 64-column block g:
+//   T[rows_g, :] = W[rows_g, K] P      (T: n x kb, ld n)
+//   R[:, cols_g] = W[K, cols_g]        (R: kb x n, ld kb)
+// and block 0 copies the next diagonal block W[K1, K1] (kb1 x kb1) to Dsrc
+// (ld 64) when kb1 > 0, before the trailing update changes it.
+__global__ __launch_bounds__(CP_THREADS) void k_gj_panel_a(int n, int k0, int kb,
+                                                           const double *__restrict__ W,
+                                                           const double *__restrict__ P,
+                                                           double *__restrict__ T,
+                                                           double *__restrict__ R, int k1, int kb1,
+                                                           double *__restrict__ Dsrc) {
+  __shared__ double As[CL_B][CL_B + 1];
+  __shared__ double Bs[CL_B][CL_B + 1];
+  const int g0 = blockIdx.x * CL_B;
+  {  // R[:, cols_g] = W[K, cols_g]; block 0: Dsrc = W[K1, K1]
+    double v[STAGE_R], w[STAGE_R];
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      v[r] = (i < kb && g0 + k < n) ? W[(size_t)(k0 + i) + (size_t)(g0 + k) * n] : 0.0;
+      w[r] = (blockIdx.x == 0 && i < kb1 && k < kb1) ? W[(size_t)(k1 + i) + (size_t)(k1 + k) * n]
+                                                     : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      if (i < kb && g0 + k < n) R[(size_t)i + (size_t)(g0 + k) * kb] = v[r];
+      if (blockIdx.x == 0 && i < kb1 && k < kb1) Dsrc[i + k * CL_B] = w[r];
+    }
+  }
+  stage64(As, [&](int i, int k) {
+    return (g0 + i < n && k < kb) ? W[(size_t)(g0 + i) + (size_t)(k0 + k) * n] : 0.0;
+  });
+  stage64(Bs, [&](int i, int k) { return (i < kb && k < kb) ? P[i + k * kb] : 0.0; });
+  __syncthreads();
+  double acc[4][4];
+  lds_product64(kb, As, Bs, acc);
+  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = g0 + ti + 16 * u, j = tj + 16 * v;
+      if (i < n && j < kb) T[(size_t)i + (size_t)j * n] = acc[u][v];
+    }
+}
+
+// Second half of the panel, for the 64-column / 64-row block g:
+//   g = K's block:  W[K, K] = P
+//   otherwise:      W[K, cols_g] = P R[:, cols_g];  W[rows_g, K] = -T[rows_g, :]
+// (k0 is a multiple of 64, so K is exactly block k0 / 64.)  Touches only the
+// panel's rows and columns of W.
+__device__ void gj_finish_block(double (*As)[CL_B + 1], double (*Bs)[CL_B + 1], bool act, int g,
+                                int n, int k0, int kb, double *__restrict__ W,
+                                const double *__restrict__ P, const double *__restrict__ T,
+                                const double *__restrict__ R) {
+  const int g0 = g * CL_B;
+  if (g0 == k0) {
+    if (!act) return;
+    double v[STAGE_R];
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      v[r] = (i < kb && k < kb) ? P[i + k * kb] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      if (i < kb && k < kb) W[(size_t)(k0 + i) + (size_t)(k0 + k) * n] = v[r];
+    }
+    return;
+  }
+  if (act) {  // W[rows_g, K] = -T[rows_g, :]
+    double v[STAGE_R];
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      v[r] = (g0 + i < n && k < kb) ? T[(size_t)(g0 + i) + (size_t)k * n] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      if (g0 + i < n && k < kb) W[(size_t)(g0 + i) + (size_t)(k0 + k) * n] = -v[r];
+    }
+  }
+  stage64(As, [&](int i, int k) { return (i < kb && k < kb) ? P[i + k * kb] : 0.0; }, act);
+  stage64(Bs, [&](int i, int k) {
+    return (i < kb && g0 + k < n) ? R[(size_t)i + (size_t)(g0 + k) * kb] : 0.0;
+  }, act);
+  __syncthreads();
+  if (!act) return;
+  double acc[4][4];
+  lds_product64(kb, As, Bs, acc);
+  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti + 16 * u, j = g0 + tj + 16 * v;
+      if (i < kb && j < n) W[(size_t)(k0 + i) + (size_t)j * n] = acc[u][v];
+    }
+}
+
+// Look-ahead inverse of the next diagonal block: D = Dsrc - T1 R1 (Dsrc: b x b
+// copy of the block taken before this panel's update, ld 64; T1: the b rows
+// of T for the next panel, ld ldt; R1: the b columns of R, ld ldr; kb = |K|),
+// then Pn = D^-1 (b x b, ld b).
+__device__ void gj_next_inverse(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], bool act, int b,
+                                const double *Dsrc, const double *T1, int ldt, const double *R1,
+                                int ldr, int kb, double *Pn, unsigned int *err) {
+  // S = T1 (b x kb), Xs = R1 (kb x b)
+  stage64(S, [&](int i, int k) { return (i < b && k < kb) ? T1[(size_t)i + (size_t)k * ldt] : 0.0; },
+          act);
+  stage64(Xs, [&](int k, int i) { return (k < kb && i < b) ? R1[(size_t)k + (size_t)i * ldr] : 0.0; },
+          act);
+  __syncthreads();
+  double acc[4][4], d[4][4];
+  const int ti = threadIdx.x & 15, tj = (threadIdx.x >> 4) & 15;
+  if (act) {
+    lds_product64(kb, S, Xs, acc);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ti + 16 * u, k = tj + 16 * v;
+        d[u][v] = (i < b && k <= i) ? Dsrc[i + k * CL_B] : 0.0;
+      }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ti + 16 * u, k = tj + 16 * v;
+        S[i][k] = (i < b && k < b) ? (k <= i ? d[u][v] - acc[u][v] : 0.0) : (i == k ? 1.0 : 0.0);
+      }
+  }
+  __syncthreads();
+  const bool bad = chol_inv_lds(S, Xs, act);
+  if (act) store_xtx(b, Xs, Pn);
+  if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
+// One Gauss-Jordan step after k_gj_panel_a, as one launch of three kinds of
+// workgroup that touch disjoint data:
+//   [0, tiles)             W -= T R on the 128 x 128 MFMA tiles, leaving the
+//                          panel's rows and columns K alone
+//   [tiles, tiles + nblk)  gj_finish_block: the panel's rows and columns
+//   tiles + nblk (kb1 > 0) gj_next_inverse: P of the next panel
+// The next panel's P depends only on T, R and the copied block, so it is
+// formed while the trailing update runs instead of after it.
+__global__ __launch_bounds__(512, 1) void k_gj_update(int n, int k0, int kb, double *W,
+                                                      const double *P, const double *T,
+                                                      const double *R, int kb1, const double *Dsrc,
+                                                      double *Pn, unsigned int *err) {
+  __shared__ double S[CL_B][CL_B + 1];
+  __shared__ double Xs[CL_B][CL_B + 1];
+  const int tm = (n + GB_M - 1) / GB_M, tiles = tm * tm, nblk = (n + CL_B - 1) / CL_B;
+  const int bid = blockIdx.x;
+  if (bid < tiles) {
+    gemm_w8_tile<16, false, false>(bid, n, n, kb, -1.0, T, n, R, kb, 1.0, W, n, nullptr, 0,
+                                   nullptr, k0, k0 + kb);
+    return;
+  }
+  const bool act = threadIdx.x < CP_THREADS;
+  if (bid < tiles + nblk) {
+    gj_finish_block(S, Xs, act, bid - tiles, n, k0, kb, W, P, T, R);
+    return;
+  }
+  const int k1 = k0 + kb;
+  gj_next_inverse(S, Xs, act, kb1, Dsrc, T + k1, n, R + (size_t)k1 * kb, kb, kb, Pn, err);
 }
 
 static int g_leaf_variant = 0;
@@ -347,6 +573,24 @@ hipError_t spd_inverse64(int b, const double *W, int ldw, double *P, unsigned in
                          hipStream_t st) {
   if (b < 1 || b > CL_B) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_spd_inv64, dim3(1), dim3(CP_THREADS), 0, st, b, W, ldw, P, err);
+  return hipGetLastError();
+}
+
+hipError_t gj_panel_a(int n, int k0, int kb, const double *W, const double *P, double *T,
+                      double *R, int k1, int kb1, double *Dsrc, hipStream_t st) {
+  if (kb < 1 || kb > CL_B || kb1 < 0 || kb1 > CL_B || (k0 % CL_B) != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gj_panel_a, dim3(cdiv(n, CL_B)), dim3(CP_THREADS), 0, st, n, k0, kb, W, P,
+                     T, R, k1, kb1, Dsrc);
+  return hipGetLastError();
+}
+
+hipError_t gj_update(int n, int k0, int kb, double *W, const double *P, const double *T,
+                     const double *R, int kb1, const double *Dsrc, double *Pn, unsigned int *err,
+                     hipStream_t st) {
+  if (kb < 1 || kb > CL_B || kb1 < 0 || kb1 > CL_B || (k0 % CL_B) != 0) return hipErrorInvalidValue;
+  const int grid = (int)(cdiv(n, GB_M) * cdiv(n, GB_N) + cdiv(n, CL_B)) + (kb1 > 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_gj_update, dim3(grid), dim3(512), 0, st, n, k0, kb, W, P, T, R, kb1, Dsrc,
+                     Pn, err);
   return hipGetLastError();
 }
 

@@ -95,7 +95,7 @@ Layout make_layout(const Sizes &z, bool refine) {
   // Y, then the point step's per-observation 3x3 terms
   sz[B_Y] = sizeof(double) * (size_t)std::max(3 * z.cp, 9) * z.nobs;
   sz[B_CTRL] = sizeof(TeCtrl);
-  sz[B_P] = sizeof(double) * GJ_BLOCK * GJ_BLOCK;
+  sz[B_P] = sizeof(double) * 3 * GJ_BLOCK * GJ_BLOCK;
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
   sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
   sz[B_DE] = sizeof(double) * z.nn;
@@ -193,21 +193,25 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     }                                                                                   \
   } while (0)
 
-// Blocked in-place Gauss-Jordan inverse of the SPD matrix in W (n x n).
-int gj_inverse(int n, double *W, double *P, double *T, double *R, unsigned int *err,
+// Blocked in-place Gauss-Jordan inverse of the SPD matrix in W (n x n) with
+// 64-column panels K: P = W[K,K]^-1, T = W[:,K] P, R = W[K,:], W -= T R, then
+// W[K,:] = P R, W[:,K] = -T, W[K,K] = P.  Two launches per panel: gj_panel_a
+// (T, R and a copy of the next diagonal block W[K1,K1]) and gj_update (the
+// trailing update outside the panel's rows and columns, those rows and
+// columns, and the next panel's P = (W[K1,K1] - T[K1,:] R[:,K1])^-1, which
+// needs nothing the trailing update writes).  P3: 3 x 64 x 64 doubles (two P
+// buffers and the copy); T: n x 64; R: 64 x n.
+int gj_inverse(int n, double *W, double *P3, double *T, double *R, unsigned int *err,
                hipStream_t s) {
-  for (int k0 = 0; k0 < n; k0 += GJ_BLOCK) {
-    const int kb = std::min(GJ_BLOCK, n - k0);
-    double *Wkk = W + k0 + (size_t)k0 * n;
-    HC(spd_inverse64(kb, Wkk, n, P, err, s));
-    // T = W[:,K] P ; R = W[K,:]
-    HC(gemm_f64(n, kb, kb, 1.0, W + (size_t)k0 * n, n, P, kb, 0.0, T, n, nullptr, nullptr, s));
-    HC(copy_block(kb, n, W + k0, n, R, kb, 1.0, s));
-    // W -= T R ; W[K,:] = P R ; W[:,K] = -T ; W[K,K] = P
-    HC(gemm_f64(n, n, kb, -1.0, T, n, R, kb, 1.0, W, n, nullptr, nullptr, s));
-    HC(gemm_f64(kb, n, kb, 1.0, P, kb, R, kb, 0.0, W + k0, n, nullptr, nullptr, s));
-    HC(copy_block(n, kb, T, n, W + (size_t)k0 * n, n, -1.0, s));
-    HC(copy_block(kb, kb, P, kb, Wkk, n, 1.0, s));
+  const int panels = (int)cdiv(n, GJ_BLOCK);
+  double *Pb[2] = {P3, P3 + GJ_BLOCK * GJ_BLOCK};
+  double *Dsrc = P3 + 2 * GJ_BLOCK * GJ_BLOCK;
+  HC(spd_inverse64(std::min(GJ_BLOCK, n), W, n, Pb[0], err, s));
+  for (int p = 0; p < panels; ++p) {
+    const int k0 = p * GJ_BLOCK, kb = std::min(GJ_BLOCK, n - k0);
+    const int k1 = k0 + kb, kb1 = p + 1 < panels ? std::min(GJ_BLOCK, n - k1) : 0;
+    HC(gj_panel_a(n, k0, kb, W, Pb[p & 1], T, R, k1, kb1, Dsrc, s));
+    HC(gj_update(n, k0, kb, W, Pb[p & 1], T, R, kb1, Dsrc, Pb[(p + 1) & 1], err, s));
   }
   return UNC_OK;
 }
@@ -1207,9 +1211,10 @@ int unc_diag_inverse(int n, const double *A, double *X, int refine) {
   hipStream_t s = ctx->stream;
   const size_t nn = (size_t)n * n;
   char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * (5 * nn + (size_t)GJ_BLOCK * (GJ_BLOCK + 2 * n)) + 256));
+  HC(hipMalloc((void **)&buf,
+               sizeof(double) * (5 * nn + (size_t)GJ_BLOCK * (3 * GJ_BLOCK + 2 * n)) + 256));
   double *dA = (double *)buf, *dW = dA + nn, *dR = dW + nn, *dD = dR + nn, *dX = dD + nn;
-  double *P = dX + nn, *T = P + GJ_BLOCK * GJ_BLOCK, *Rp = T + (size_t)GJ_BLOCK * n;
+  double *P = dX + nn, *T = P + 3 * GJ_BLOCK * GJ_BLOCK, *Rp = T + (size_t)GJ_BLOCK * n;
   unsigned int *err = (unsigned int *)(Rp + (size_t)GJ_BLOCK * n);
   HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
   HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
