@@ -247,7 +247,9 @@ int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
  * kernel with K slab 32), 8 the split-K form of the production kernel. */
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
 /* Average duration (ms) of the n x n compensated residual R = I - A X over reps
- * launches; variant 0 is the production tiling (64 x 32), 1 the 64 x 64 one. */
+ * launches; variant 0 is the production choice (64 x 16 tiles while the 64 x 32
+ * grid would fill fewer than three rounds of resident workgroups, else 64 x 32),
+ * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles. */
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
 /* Select the GEMM kernel used by every later call in this process (tuning). */
 int unc_diag_set_gemm_variant(int variant);

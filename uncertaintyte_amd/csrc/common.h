@@ -156,7 +156,9 @@ hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const
 hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,
                           hipStream_t st);
 hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st);
-void set_residual_variant(int v);  // 0: 64 x 32 tiles (default), 1: 64 x 64 tiles
+// 0: 64 x 16 or 64 x 32 tiles by size (default); 1: 64 x 64, 2: 64 x 48,
+// 3: 64 x 16, 4: 64 x 32 tiles
+void set_residual_variant(int v);
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st);
 hipError_t copy_block(int m, int n, const double *src, int lds, double *dst, int ldd, double scale,

@@ -668,7 +668,7 @@ constexpr int R_T = 64, R_K = 16;
 // QN = 4, so the waves spread evenly over the SIMDs (N = 1800: 6.6 waves per
 // SIMD rather than 3.3).
 template <int QN>
-__global__ __launch_bounds__(256, QN == 4 ? 4 : 6) void k_residual_dd(
+__global__ __launch_bounds__(256, QN == 4 ? 4 : QN == 3 ? 5 : QN == 2 ? 6 : 8) void k_residual_dd(
     int n, const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ R) {
   constexpr int RC = 16 * QN;
   __shared__ double As[R_K][R_T + 1];
@@ -735,10 +735,26 @@ static int g_residual_variant = 0;
 void set_residual_variant(int v) { g_residual_variant = v; }
 
 hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st) {
+  // 64 x 32 tiles at six workgroups per CU, unless they would need fewer than
+  // three rounds of the 1536 resident slots: then the last round runs at low
+  // occupancy for a large share of the time, and 64 x 16 tiles (eight per CU)
+  // balance better (N = 1800: 2.14 ms against 2.47 ms).
+  const long long wg32 = (long long)cdiv(n, R_T) * cdiv(n, 32);
+  if (g_residual_variant == 0 && wg32 < 3 * 1536) {
+    hipLaunchKernelGGL(k_residual_dd<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n, A,
+                       X, R);
+    return hipGetLastError();
+  }
   if (g_residual_variant == 1)
     hipLaunchKernelGGL(k_residual_dd<4>, dim3(cdiv(n, R_T), cdiv(n, 64)), dim3(256), 0, st, n, A,
                        X, R);
-  else
+  else if (g_residual_variant == 2)
+    hipLaunchKernelGGL(k_residual_dd<3>, dim3(cdiv(n, R_T), cdiv(n, 48)), dim3(256), 0, st, n, A,
+                       X, R);
+  else if (g_residual_variant == 3)
+    hipLaunchKernelGGL(k_residual_dd<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n, A,
+                       X, R);
+  else  // 0 (large n) and 4: 64 x 32 tiles
     hipLaunchKernelGGL(k_residual_dd<2>, dim3(cdiv(n, R_T), cdiv(n, 32)), dim3(256), 0, st, n, A,
                        X, R);
   return hipGetLastError();

@@ -1169,7 +1169,7 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
 }
 
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 1) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 4) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
