@@ -9,9 +9,11 @@ in HBM; outputs stay on the device.
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-zplus] [--no-serving]
 
 Prints one JSON line.  `roofline` describes the dominant kernel (the FP64
-MFMA GEMM of the Taylor chain) from HIP events recorded by the library around
-each launch inside the timed region; `cpu_baseline` times the CPU oracle (C,
-OpenMP) on one solve of the same scene.
+MFMA product T X0 of the Taylor chain; it is symmetric, so only its lower
+128 x 128 tiles are computed and N^2 (N + 1) flops are counted per launch)
+from HIP events recorded by the library around each launch inside the timed
+region; `cpu_baseline` times the CPU oracle (C, OpenMP) on one solve of the
+same scene.
 
 Two further measurements ride on the same line:
   zplus    the gauge-fixed Z+ path (UNC_PREC_MIXED) against the plain FP64 TE
@@ -242,9 +244,10 @@ def copy_calibration() -> int:
 
 
 def gemm_traffic():
-    """HBM bytes per N x N x N GEMM launch measured offline with rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy calibration); recorded in
-    profiles/gemm_rings200_traffic.md as 'traffic_bytes_per_launch: <int>'."""
+    """HBM bytes per Taylor-chain product (GEMM + epilogue) measured offline with
+    rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE, each scaled by the copy
+    calibration); recorded in profiles/gemm_rings200_traffic.md as
+    'traffic_bytes_per_launch: <int>'."""
     path = os.path.join(ROOT, "profiles", "gemm_rings200_traffic.md")
     try:
         for line in open(path):
@@ -311,6 +314,7 @@ def main() -> int:
         call()
     torch.cuda.synchronize()
     gemm_ms = 0.0
+    gemm_flops = 0.0
     gemm_n = 0
     stage = np.zeros(12)
     terms = 0
@@ -318,6 +322,7 @@ def main() -> int:
     for _ in range(args.steps):
         st = call()
         gemm_ms += st.gemm_ms
+        gemm_flops += st.gemm_flops
         gemm_n += st.gemm_count
         stage += np.array(st.stage_ms[:])
         terms = st.num_terms
@@ -325,7 +330,9 @@ def main() -> int:
     el = time.perf_counter() - t0
     value = args.steps / el
     N = J.num_cams * J.cam_params
-    flops = 2.0 * N ** 3
+    # algorithmic flops per chain product: N^2 (N + 1) for the symmetric form
+    # (lower tiles only), 2 N^3 where the plain product runs
+    flops = gemm_flops / max(gemm_n, 1)
     avg_gemm_s = gemm_ms / max(gemm_n, 1) * 1e-3
     achieved = flops / avg_gemm_s / 1e12 if avg_gemm_s > 0 else 0.0
     stage /= args.steps
@@ -383,7 +390,7 @@ def main() -> int:
         "config": {"workload": WORKLOADS.get(args.workload, args.workload) + ", " + PATHS[args.precision],
                    "cameras": nC, "points": nP, "observations": nO,
                    "jacobian_nnz": nnz, "N": N, "taylor_terms": terms, "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64w8 split-K + k_split_reduce_flat (N x N x N)", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": "Taylor-chain product T X0: k_gemm_f64w8 (lower tiles, 4 K chunks) + k_split_reduce_sym", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": gemm_traffic(),
                      "per_launch_flops": flops, "avg_launch_ms": round(avg_gemm_s * 1e3, 4),

@@ -92,8 +92,9 @@ typedef struct unc_stats {
   int num_terms;           /* accepted Taylor terms = cycle_change size            */
   double cycle_change[20];
   /* device-side measurements */
-  double gemm_ms;          /* summed duration of the dense GEMM launches (HIP events) */
-  double gemm_flops;       /* 2 M N K summed over those launches                      */
+  double gemm_ms;          /* summed duration of the Taylor-chain products (HIP events) */
+  double gemm_flops;       /* their algorithmic flops: 2 N^3 each, N^2 (N + 1) for the  */
+                           /* symmetric form that computes the lower tiles only         */
   int gemm_count;
   int num_obs_kept;        /* observations left after fixPts                          */
   double stage_ms[12];     /* per device stage, see unc_stage_name()                  */
@@ -230,6 +231,14 @@ int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int ld
    (128 x 128 tiles between 96 and 256, K >= 1024). */
 int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
                             const double *B, int ldb, double *C, int ldc, double *absmax_out);
+/* C = alpha A B for n x n column-major A, B whose product is symmetric (powers
+   of one symmetric matrix, as in the Taylor chain): only the 128 x 128 tiles on
+   and below the diagonal are computed, in four K chunks, and the lower triangle
+   is mirrored, so C is exactly symmetric; *absmax_out = max|C|.
+   UNC_ERR_UNSUPPORTED where the library would not use this form (same shapes
+   as unc_diag_gemm_f64_split). */
+int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
+                          double *absmax_out);
 int unc_diag_inverse(int n, const double *A, double *X, int refine);
 int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
 /* Cholesky A = L L^T of SPD A (n x n, column-major) and X = L^-1 by the recursive

@@ -68,10 +68,32 @@ def test_gemm_f64_split_k(M, N, K):
     assert np.abs(C - C1).max() <= 1e-14 * np.abs(A).max() * np.abs(B).max() * K
 
 
+@pytest.mark.parametrize("n", [1800, 1283, 2048])
+def test_gemm_f64_symmetric_product(n):
+    """The Taylor chain's product T X0 of two powers of one symmetric matrix:
+    lower 128 x 128 tiles only, K in four chunks, lower triangle mirrored.  C
+    must be exactly symmetric, equal the full product to FP64 rounding
+    (ragged n included), and carry max|C|."""
+    say(f"gemm sym {n}")
+    rng = np.random.default_rng(n)
+    Q = rng.standard_normal((n, n)) / np.sqrt(n)
+    S = Q + Q.T                         # symmetric
+    S2 = S @ S                          # a power of S: S2 S is symmetric
+    S2 = 0.5 * (S2 + S2.T)
+    C, m = u.diag_gemm_sym(S2, S, alpha=0.5)
+    ref = 0.5 * (S2 @ S)
+    assert np.array_equal(C, C.T)
+    assert np.abs(C - ref).max() <= 1e-13 * np.abs(S2).max() * np.abs(S).max() * n
+    assert m == np.abs(C).max()
+
+
 def test_gemm_f64_split_k_declines_small_shapes():
     A = np.ones((256, 1024))
     with pytest.raises(u.UncError) as e:
         u.diag_gemm(A, A.T.copy(), split=True)
+    assert e.value.code == u.UNC_ERR_UNSUPPORTED
+    with pytest.raises(u.UncError) as e:
+        u.diag_gemm_sym(np.eye(512), np.eye(512))
     assert e.value.code == u.UNC_ERR_UNSUPPORTED
 
 
@@ -238,4 +260,5 @@ def test_device_api_matches_host_api(daliborka):
     r = u.compute_covariances(j)
     assert np.array_equal(cam.cpu().numpy().reshape(r.cam.shape), r.cam)
     assert np.array_equal(pt.cpu().numpy().reshape(r.pt.shape), r.pt)
-    assert st.num_terms == r.n_terms and st.gemm_count == 2 + r.n_terms
+    # the chain's products: T1 and one per accepted term
+    assert st.num_terms == r.n_terms and st.gemm_count == 1 + r.n_terms

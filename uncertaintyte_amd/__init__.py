@@ -113,6 +113,7 @@ def lib():
                                     ctypes.c_int, dp]
     L.unc_diag_gemm_f64_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, dp]
+    L.unc_diag_gemm_f64_sym.argtypes = [ctypes.c_int, ctypes.c_double, vp, vp, vp, dp]
     L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
     L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
     L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
@@ -310,6 +311,20 @@ def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alph
         rc = lib().unc_diag_gemm_f64(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K, beta,
                                      Cf.ctypes.data, M, ctypes.byref(m))
         _check(rc, "unc_diag_gemm_f64")
+    return np.asarray(Cf), m.value
+
+
+def diag_gemm_sym(A: np.ndarray, B: np.ndarray, alpha: float = 1.0):
+    """C = alpha A B by the symmetric-product form (lower tiles, mirrored);
+    UncError UNC_ERR_UNSUPPORTED for shapes the library does not split."""
+    n = A.shape[0]
+    assert A.shape == (n, n) and B.shape == (n, n)
+    Af = np.asfortranarray(A, dtype=np.float64)
+    Bf = np.asfortranarray(B, dtype=np.float64)
+    Cf = np.zeros((n, n), order="F")
+    m = ctypes.c_double()
+    _check(lib().unc_diag_gemm_f64_sym(n, alpha, Af.ctypes.data, Bf.ctypes.data, Cf.ctypes.data,
+                                       ctypes.byref(m)), "unc_diag_gemm_f64_sym")
     return np.asarray(Cf), m.value
 
 

@@ -14,7 +14,9 @@ constexpr int G_PAD = 4;
 // One 128 x 128 tile of C = alpha op(A) B + beta C for the workgroup's tile
 // index tile_bid (512 threads, 8 waves of 64 x 32; tiles in 8-row groups so
 // neighbouring workgroups share A panels in L2).  tri: triangular-operand flags
-// (GEMM_*); SPLIT: see k_gemm_f64w8.  Rows and columns of C in [skip0, skip1)
+// (GEMM_*); SPLIT: the grid holds `parts` workgroups per tile, part p summing
+// the p-th BK-aligned chunk of K into C (p = 0) or part1 + (p - 1) M N
+// (leading dimension M), unscaled.  Rows and columns of C in [skip0, skip1)
 // are neither read nor written, which lets a Gauss-Jordan step leave the
 // panel's band to other workgroups of the same launch.
 template <int BK, bool TRANS_B, bool SPLIT>
@@ -23,7 +25,8 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
                                              const double *__restrict__ B, int ldb, double beta,
                                              double *__restrict__ C, int ldc,
                                              unsigned long long *absmax, int tri,
-                                             double *__restrict__ part1, int skip0, int skip1) {
+                                             double *__restrict__ part1, int parts, int skip0,
+                                             int skip1) {
   __shared__ double As[BK][GB_M + G_PAD];
   __shared__ double Bs[BK][GB_N + G_PAD];
   __shared__ double wmax[8];
@@ -34,7 +37,7 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
   int bid = tile_bid, part = 0;
   if (SPLIT) {
     const int nt = tiles_m * tiles_n;
-    part = bid >= nt;
+    part = bid / nt;
     bid -= part * nt;
   }
   const int group = 8;
@@ -49,10 +52,10 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
   if (tri & GEMM_A_LOWER) kend = min(kend, m0 + GB_M);
   if (tri & GEMM_B_LOWER) kbeg = n0;
   if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + GB_N);
-  if (SPLIT) {  // tri is 0 here
-    const int half = min(K, ((K + 1) / 2 + BK - 1) / BK * BK);
-    if (part) kbeg = half;
-    else kend = half;
+  if (SPLIT) {  // no triangular operand here: K is split into `parts` BK-aligned chunks
+    const int chunk = ((K + parts - 1) / parts + BK - 1) / BK * BK;
+    kbeg = min(K, part * chunk);
+    kend = min(K, (part + 1) * chunk);
   }
   double4_t acc[4][2];
   for (int i = 0; i < 4; ++i)
@@ -104,7 +107,7 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
     }
   }
   if (SPLIT) {
-    double *dst = part ? part1 : C;
+    double *dst = part ? part1 + (size_t)(part - 1) * M * N : C;
     const int ld = part ? M : ldc;
 #pragma unroll
     for (int i = 0; i < 4; ++i)

@@ -293,16 +293,17 @@ __global__ __launch_bounds__(512, 1) void k_gemm_f64w8(int M, int N, int K, doub
                                                       double beta, double *__restrict__ C,
                                                       int ldc, unsigned long long *absmax,
                                                       const int *active, int tri,
-                                                      double *__restrict__ part1) {
+                                                      double *__restrict__ part1, int parts) {
   if (active && *active == 0) return;
   gemm_w8_tile<BK, TRANS_B, SPLIT>(blockIdx.x, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                                   absmax, tri, part1, 0, 0);
+                                   absmax, tri, part1, parts, 0, 0);
 }
 
 // Epilogue of the split product: C = alpha (C + W), with the max|C| fold and
 // the early exit of the unsplit kernel.  k_split_reduce_flat takes C stored
 // densely (ldc == M, even size, 16-byte aligned) as one array and moves two
-// double2 per thread per step; k_split_reduce goes column by column.
+// double2 per thread per step; k_split_reduce goes column by column;
+// k_split_reduce_sym is the symmetric form.
 __device__ inline void fold_absmax(double lmax, unsigned long long *absmax, double *wmax) {
   const int tid = threadIdx.x;
   for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
@@ -336,6 +337,56 @@ __global__ __launch_bounds__(256) void k_split_reduce_flat(size_t n4, double alp
     c2[2 * q] = r0;
     c2[2 * q + 1] = r1;
     lmax = fmax(lmax, fmax(fmax(fabs(r0.x), fabs(r0.y)), fmax(fabs(r1.x), fabs(r1.y))));
+  }
+  if (absmax) fold_absmax(lmax, absmax, wmax);
+}
+
+// Symmetric epilogue for an n x n product computed on the 128 x 128 tiles on
+// and below the diagonal in `parts` K chunks (C and W_1 .. W_{parts-1}, each
+// n x n, ld n): one workgroup per 64 x 64 block (bi >= bj) forms
+// v = alpha (C + W_1 + ...) and writes it to block (bi, bj) and its transpose
+// to block (bj, bi); in diagonal blocks the lower triangle is mirrored, so C is
+// exactly symmetric.  max|C| is folded over the lower triangle.
+__global__ __launch_bounds__(256) void k_split_reduce_sym(int n, double alpha,
+                                                          double *__restrict__ C,
+                                                          const double *__restrict__ W, int parts,
+                                                          unsigned long long *absmax,
+                                                          const int *active) {
+  if (active && *active == 0) return;
+  __shared__ double V[64][65];
+  __shared__ double wmax[4];
+  // block index -> (bi, bj), bi >= bj, in row-major order of the lower triangle
+  const int x = blockIdx.x;
+  int bi = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
+  while (bi * (bi + 1) / 2 > x) --bi;
+  const int bj = x - bi * (bi + 1) / 2;
+  const int i0 = bi * 64, j0 = bj * 64, tid = threadIdx.x;
+  const size_t nn = (size_t)n * n;
+  double lmax = 0.0;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6, gi = i0 + r, gj = j0 + c;
+    double v = 0.0;
+    if (gi < n && gj < n) {
+      const size_t o = (size_t)gi + (size_t)gj * n;
+      v = C[o];
+      for (int p = 1; p < parts; ++p) v += W[o + (size_t)(p - 1) * nn];
+      v *= alpha;
+      if (bi != bj || r >= c) lmax = fmax(lmax, fabs(v));
+    }
+    V[r][c] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6, gi = i0 + r, gj = j0 + c;
+    if (bi == bj) {  // mirror the lower triangle of the diagonal block
+      if (gi < n && gj < n) C[(size_t)gi + (size_t)gj * n] = r >= c ? V[r][c] : V[c][r];
+    } else {
+      if (gi < n && gj < n) C[(size_t)gi + (size_t)gj * n] = V[r][c];
+      // block (bj, bi): element (j0 + r, i0 + c) = V[c][r]
+      const int ti = j0 + r, tj = i0 + c;
+      if (ti < n && tj < n) C[(size_t)ti + (size_t)tj * n] = V[c][r];
+    }
   }
   if (absmax) fold_absmax(lmax, absmax, wmax);
 }
@@ -485,7 +536,7 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
     }
     case 4:
       hipLaunchKernelGGL((k_gemm_f64w8<32>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda, B,
-                         ldb, beta, C, ldc, absmax, active, 0, nullptr);
+                         ldb, beta, C, ldc, absmax, active, 0, nullptr, 1);
       break;
     case 5:
       hipLaunchKernelGGL((k_gemm_f64<32, 1>), dim3(tiles), dim3(G_THREADS), 0, st, M, N, K, alpha,
@@ -503,7 +554,7 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
       break;
     default:  // production: 8 waves, K slabs of 16
       hipLaunchKernelGGL((k_gemm_f64w8<16>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A, lda,
-                         B, ldb, beta, C, ldc, absmax, active, 0, nullptr);
+                         B, ldb, beta, C, ldc, absmax, active, 0, nullptr, 1);
   }
   return hipGetLastError();
 }
@@ -511,11 +562,34 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
 // Split-K applies while the doubled grid still fits the resident slots (two
 // 512-thread workgroups per CU on 256 CUs) and each half keeps >= 512 of K.
 constexpr int SPLIT_MAX_TILES = 256, SPLIT_MIN_K = 1024;
+// The symmetric form computes only the tiles on and below the diagonal and
+// splits K four ways: N = 1800 gives 120 such tiles, 480 workgroups.
+constexpr int SYM_PARTS = 4;
 
 bool gemm_split_applies(int M, int N, int K) {
   const long long tiles = cdiv(M, GB_M) * cdiv(N, GB_N);
   return g_gemm_variant == 0 && tiles >= SMALL_GEMM_TILES && tiles <= SPLIT_MAX_TILES &&
          K >= SPLIT_MIN_K;
+}
+
+size_t gemm_sym_split_work_doubles(int n) {
+  return gemm_split_applies(n, n, n) ? (size_t)(SYM_PARTS - 1) * n * n : 0;
+}
+
+hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
+                        double *work, size_t work_doubles, unsigned long long *absmax,
+                        const int *active, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (!work || work_doubles < gemm_sym_split_work_doubles(n) || !gemm_split_applies(n, n, n))
+    return gemm_f64(n, n, n, alpha, A, n, B, n, 0.0, C, n, absmax, active, st);
+  const int tiles = (int)cdiv(n, GB_M) * (int)cdiv(n, GB_N);
+  hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(SYM_PARTS * tiles), dim3(512), 0, st,
+                     n, n, n, 1.0, A, n, B, n, 0.0, C, n, nullptr, active, GEMM_C_LOWER, work,
+                     SYM_PARTS);
+  const int nb = (int)cdiv(n, 64);
+  hipLaunchKernelGGL(k_split_reduce_sym, dim3(nb * (nb + 1) / 2), dim3(256), 0, st, n, alpha, C,
+                     work, SYM_PARTS, absmax, active);
+  return hipGetLastError();
 }
 
 hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
@@ -526,7 +600,7 @@ hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, in
     return gemm_f64(M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, absmax, active, st);
   const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
   hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(2 * tiles), dim3(512), 0, st, M, N, K,
-                     1.0, A, lda, B, ldb, 0.0, C, ldc, nullptr, active, 0, work);
+                     1.0, A, lda, B, ldb, 0.0, C, ldc, nullptr, active, 0, work, 2);
   const size_t mn = (size_t)M * N;
   if (ldc == M && mn % 4 == 0 && ((uintptr_t)C & 15) == 0 && ((uintptr_t)work & 15) == 0) {
     const size_t n4 = mn / 4;
@@ -649,10 +723,10 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
   }
   if (trans_b)
     hipLaunchKernelGGL((k_gemm_f64w8<16, true>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr);
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr, 1);
   else
     hipLaunchKernelGGL((k_gemm_f64w8<16, false>), dim3(tiles), dim3(512), 0, st, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr);
+                       lda, B, ldb, beta, C, ldc, nullptr, nullptr, tri, nullptr, 1);
   return hipGetLastError();
 }
 

@@ -542,7 +542,7 @@ __global__ __launch_bounds__(512, 1) void k_gj_update(int n, int k0, int kb, dou
   const int bid = blockIdx.x;
   if (bid < tiles) {
     gemm_w8_tile<16, false, false>(bid, n, n, kb, -1.0, T, n, R, kb, 1.0, W, n, nullptr, 0,
-                                   nullptr, k0, k0 + kb);
+                                   nullptr, 1, k0, k0 + kb);
     return;
   }
   const bool act = threadIdx.x < CP_THREADS;

@@ -47,7 +47,7 @@ enum {
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
   B_ERR, B_SCAN, B_PCNT, B_POFS, B_CPS, B_PK0, B_PK1, B_PV0, B_PV1, B_RUNK, B_RUNC,
   B_RUNO, B_NRUN, B_PTMP, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
-  B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_COUNT
+  B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_SPLIT, B_COUNT
 };
 
 struct Layout {
@@ -99,6 +99,7 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
   sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
   sz[B_DE] = sizeof(double) * z.nn;
+  sz[B_SPLIT] = sizeof(double) * gemm_sym_split_work_doubles(z.N);  // Taylor chain K chunks
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
   (void)refine;
   Layout L;
@@ -589,7 +590,9 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   // ---- Taylor chain (compute.cpp:142-160); X in dA (A is dead), T ping-pongs
   double *X = dA;
   double *Tb0 = dC, *Tb1 = dD;
-  double *split_work = refine ? dB : dE;  // the buffer X0 does not occupy
+  // X0^(i+2) = T_i X0 is symmetric: only its lower tiles are computed
+  double *sym_work = (double *)ptr(B_SPLIT);
+  const size_t sym_work_doubles = gemm_sym_split_work_doubles(N);
   HC(hipMemcpyAsync(X, X0, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
   {
     TeCtrl h;
@@ -605,7 +608,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   }
   const int first_chain_pair = gemm_pairs;
   HC(hipEventRecord(GE[2 * gemm_pairs], s));
-  HC(gemm_f64_split(N, N, N, 1.0, X0, N, X0, N, Tb0, N, split_work, &ctrl->maxbits[1], nullptr, s));
+  HC(gemm_f64_sym(N, 1.0, X0, X0, Tb0, sym_work, sym_work_doubles, &ctrl->maxbits[1], nullptr, s));
   HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
   ++gemm_pairs;
   double *Tc = Tb0, *Tn = Tb1;
@@ -614,8 +617,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     HC(te_control(ctrl, i, k, s));
     HC(te_axpy(N, X, Tc, ctrl, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64_split(N, N, N, 1.0, Tc, N, X0, N, Tn, N, split_work, &ctrl->maxbits[i + 1],
-                      &ctrl->active, s));
+    HC(gemm_f64_sym(N, 1.0, Tc, X0, Tn, sym_work, sym_work_doubles, &ctrl->maxbits[i + 1],
+                    &ctrl->active, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     std::swap(Tc, Tn);
@@ -648,15 +651,18 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     for (int i = 0; i < hc.nterms && i < 20; ++i) st->cycle_change[i] = hc.change[i];
     double gms = 0.0;
     int gcount = 0;
-    // executed GEMMs: refinement (if any), T1, and one per accepted term
+    // executed products of the Taylor chain: T1 and one per accepted term
     const int executed = first_chain_pair + 1 + hc.nterms;
-    for (int p = 0; p < executed && p < gemm_pairs; ++p) {
+    for (int p = first_chain_pair; p < executed && p < gemm_pairs; ++p) {
       gms += ev_ms(GE[2 * p], GE[2 * p + 1]);
       ++gcount;
     }
     st->gemm_ms = gms;
     st->gemm_count = gcount;
-    st->gemm_flops = 2.0 * (double)N * (double)N * (double)N * gcount;
+    // a symmetric product computed on its lower tiles needs N (N + 1) / 2
+    // entries of 2 N flops each
+    const double nd = (double)N;
+    st->gemm_flops = (sym_work_doubles > 0 ? nd * nd * (nd + 1.0) : 2.0 * nd * nd * nd) * gcount;
     st->stage_ms[1] = ev_ms(E[1], E[2]);
     st->stage_ms[2] = ev_ms(E[2], E[3]);
     st->stage_ms[3] = ev_ms(E[3], E[4]);
@@ -1121,6 +1127,35 @@ int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, 
   HC(gemm_f64_split(M, N, K, alpha, dA, lda, dB, ldb, dC, ldc, dW, dm, nullptr, s));
   unsigned long long mb = 0;
   HC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+  HC(hipFree(buf));
+  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
+  return UNC_OK;
+}
+
+int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
+                          double *absmax_out) {
+  if (n <= 0 || !A || !B || !C) return UNC_ERR_BAD_INPUT;
+  if (!gemm_split_applies(n, n, n)) return UNC_ERR_UNSUPPORTED;
+  int rc = UNC_OK;
+  unc_context *ctx = default_context(0, &rc);
+  if (!ctx) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nn = (size_t)n * n, nw = gemm_sym_split_work_doubles(n);
+  char *buf = nullptr;
+  HC(hipMalloc((void **)&buf, sizeof(double) * (3 * nn + nw) + 256));
+  double *dA = (double *)buf, *dB = dA + nn, *dC = dB + nn, *dW = dC + nn;
+  unsigned long long *dm = (unsigned long long *)(dW + nw);
+  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  HC(hipMemcpyAsync(dB, B, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  HC(hipMemsetAsync(dC, 0, sizeof(double) * nn, s));
+  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
+  HC(gemm_f64_sym(n, alpha, dA, dB, dC, dW, nw, dm, nullptr, s));
+  unsigned long long mb = 0;
+  HC(hipMemcpyAsync(C, dC, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
   HC(hipFree(buf));
