@@ -579,7 +579,10 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (refine) {
     HC(residual_dd(N, dA, dB, dC, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64_split(N, N, N, 1.0, dB, N, dC, N, dD, N, dE, nullptr, nullptr, s));  // dE: scratch
+    // the full product: X0 from Gauss-Jordan is symmetric only to about
+    // cond * eps, and X0 R carries an antisymmetric part of that size, which a
+    // lower-tile product would fold into the result (dE: scratch)
+    HC(gemm_f64_split(N, N, N, 1.0, dB, N, dC, N, dD, N, dE, nullptr, nullptr, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     HC(sym_update(N, dB, dD, dE, s));
