@@ -114,10 +114,11 @@ void set_gemm_variant(int v);
 bool gemm_split_applies(int M, int N, int K);
 // C = alpha A B for n x n operands (ld n) whose product is symmetric (powers
 // of one symmetric matrix): only the 128 x 128 tiles on and below the
-// diagonal are computed, over four K chunks where gemm_split_applies (work:
+// diagonal are computed, over three K chunks where gemm_split_applies (work:
 // gemm_sym_split_work_doubles(n) doubles), and the lower triangle is mirrored;
 // otherwise the plain product.  Same absmax / active contract as gemm_f64.
 size_t gemm_sym_split_work_doubles(int n);
+void set_sym_parts(int p);  // tuning: K chunks of the symmetric form (default 3, at most 4)
 hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                         double *work, size_t work_doubles, unsigned long long *absmax,
                         const int *active, hipStream_t st);

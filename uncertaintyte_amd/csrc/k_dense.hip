@@ -563,8 +563,12 @@ hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
 // 512-thread workgroups per CU on 256 CUs) and each half keeps >= 512 of K.
 constexpr int SPLIT_MAX_TILES = 256, SPLIT_MIN_K = 1024;
 // The symmetric form computes only the tiles on and below the diagonal and
-// splits K four ways: N = 1800 gives 120 such tiles, 480 workgroups.
+// splits K three ways: N = 1800 gives 120 such tiles, 360 workgroups (rings200
+// solves/s with 2 / 3 / 4 chunks: 97.4 / 108.5 / 103.7).  The scratch is sized
+// for up to SYM_PARTS chunks so the count can be tuned per process.
 constexpr int SYM_PARTS = 4;
+static int g_sym_parts = 3;
+void set_sym_parts(int p) { g_sym_parts = p < 1 ? 1 : (p > SYM_PARTS ? SYM_PARTS : p); }
 
 bool gemm_split_applies(int M, int N, int K) {
   const long long tiles = cdiv(M, GB_M) * cdiv(N, GB_N);
@@ -583,12 +587,13 @@ hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, d
   if (!work || work_doubles < gemm_sym_split_work_doubles(n) || !gemm_split_applies(n, n, n))
     return gemm_f64(n, n, n, alpha, A, n, B, n, 0.0, C, n, absmax, active, st);
   const int tiles = (int)cdiv(n, GB_M) * (int)cdiv(n, GB_N);
-  hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(SYM_PARTS * tiles), dim3(512), 0, st,
+  const int parts = g_sym_parts;
+  hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(parts * tiles), dim3(512), 0, st,
                      n, n, n, 1.0, A, n, B, n, 0.0, C, n, nullptr, active, GEMM_C_LOWER, work,
-                     SYM_PARTS);
+                     parts);
   const int nb = (int)cdiv(n, 64);
   hipLaunchKernelGGL(k_split_reduce_sym, dim3(nb * (nb + 1) / 2), dim3(256), 0, st, n, alpha, C,
-                     work, SYM_PARTS, absmax, active);
+                     work, parts, absmax, active);
   return hipGetLastError();
 }
 

@@ -1173,7 +1173,7 @@ __global__ void k_fill_pattern(size_t n, double *A, double scale) {
 }
 
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 8) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 12) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
@@ -1182,14 +1182,18 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
   hipStream_t s = ctx->stream;
   const size_t nn = (size_t)n * n;
   double *buf = nullptr;
-  if (hipMalloc((void **)&buf, sizeof(double) * 4 * nn) != hipSuccess) {
+  if (hipMalloc((void **)&buf, sizeof(double) * 6 * nn) != hipSuccess) {
     (void)hipGetLastError();
     return UNC_ERR_OOM;
   }
   hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
-  const bool split = variant == 8;
-  set_gemm_variant(split ? 0 : variant);
+  const bool split = variant == 8, sym = variant >= 9;
+  set_gemm_variant(split || sym ? 0 : variant);
+  if (sym) set_sym_parts(variant - 8);  // 9..12: symmetric form with 1..4 K chunks
   auto one = [&]() {
+    if (sym)
+      return gemm_f64_sym(n, 1.0, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, 3 * nn, nullptr,
+                          nullptr, s);
     return split ? gemm_f64_split(n, n, n, 1.0, buf, n, buf + nn, n, buf + 2 * nn, n, buf + 3 * nn,
                                   nullptr, nullptr, s)
                  : gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr,
@@ -1202,6 +1206,7 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
   HC(hipEventSynchronize(ctx->gemm_ev[1]));
   *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
   set_gemm_variant(0);
+  set_sym_parts(3);
   HC(hipFree(buf));
   return UNC_OK;
 }
@@ -1234,6 +1239,10 @@ int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
 }
 
 int unc_diag_set_gemm_variant(int variant) {
+  if (variant >= 21 && variant <= 24) {  // K chunks of the symmetric chain product
+    set_sym_parts(variant - 20);
+    return UNC_OK;
+  }
   if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
   set_gemm_variant(variant);
   return UNC_OK;
