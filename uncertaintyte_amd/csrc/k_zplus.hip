@@ -161,13 +161,15 @@ __global__ __launch_bounds__(64) void k_chol_inv64c(int b, double *A, int lda, d
 
 // ---------------------------------------------------------------------------
 // Panelled leaf (the default): 256 threads, the block in LDS, 8-column panels.
-//   Cholesky, right-looking by panels: wave 0 factors the 8 panel columns
-//   (lane = row; the panel rows stay in registers and the pivot row's values
-//   are broadcast with readlane), then all four waves apply the rank-8 update
-//   to the trailing lower triangle.  Two barriers per panel instead of one or
-//   two per column.
-//   X = L^-1 the same way: wave 0 solves the panel's 8 rows for all 64 columns
-//   (lane = column), the four waves apply the rank-8 update to the rows below.
+//   Cholesky, right-looking by panels with one panel of look-ahead: all four
+//   waves first apply panel p's rank-8 update to the next panel's 8 columns;
+//   then wave 0 factors that panel (lane = row; the panel rows stay in
+//   registers and the pivot row's values are broadcast with readlane) while
+//   waves 1-3 apply panel p's update to the rest of the trailing lower
+//   triangle, so the serial column chain of wave 0 hides the bulk update.
+//   X = L^-1 the same way: all waves update the next panel's 8 rows, then wave
+//   0 solves them for all 64 columns (lane = column) while waves 1-3 update
+//   the rows below.
 // The loops over panels stay rolled, so the code is small (no instruction-fetch
 // stalls); every register array is indexed by unrolled constants.
 // ---------------------------------------------------------------------------
@@ -181,80 +183,122 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Wave 0 (lane = row i): factor the 8 columns [p0, p0 + 8) of S in place,
+// their rows above p0 being zero.  True when a pivot was not positive.
+__device__ __forceinline__ bool chol_panel8(double (*S)[CL_B + 1], int p0, int lane) {
+  const int i = lane;
+  bool bad = false;
+  double r[CP_W];
+#pragma unroll
+  for (int jj = 0; jj < CP_W; ++jj) r[jj] = S[i][p0 + jj];
+#pragma unroll
+  for (int jj = 0; jj < CP_W; ++jj) {
+    const int j = p0 + jj;
+    double s = r[jj];
+#pragma unroll
+    for (int kk = 0; kk < jj; ++kk) s = fma(-r[kk], readlane_d(r[kk], j), s);
+    const double piv = readlane_d(s, j);
+    if (!(piv > 0.0)) bad = true;
+    const double d = sqrt(piv);
+    const double rd = 1.0 / d;
+    r[jj] = i > j ? s * rd : (i == j ? d : 0.0);
+  }
+#pragma unroll
+  for (int jj = 0; jj < CP_W; ++jj) S[i][p0 + jj] = r[jj];
+  return bad;
+}
+
+// Wave 0 (lane = column c): rows [p0, p0 + 8) of X = L^-1 from their updated
+// right-hand sides in Xs and L's diagonal block.
+__device__ __forceinline__ void tri_panel8(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int p0,
+                                           int lane) {
+  const int c = lane;
+  double x[CP_W];
+#pragma unroll
+  for (int rr = 0; rr < CP_W; ++rr) x[rr] = Xs[p0 + rr][c];
+#pragma unroll
+  for (int rr = 0; rr < CP_W; ++rr) {
+    double s = x[rr];
+#pragma unroll
+    for (int kk = 0; kk < rr; ++kk) s = fma(-S[p0 + rr][p0 + kk], x[kk], s);
+    x[rr] = s / S[p0 + rr][p0 + rr];
+  }
+#pragma unroll
+  for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
+}
+
 // S: the SPD block (lower triangle valid, upper triangle zero) -> L with zeros
 // above the diagonal; Xs -> L^-1 with zeros above.  Returns true in wave 0 when
 // a pivot was not positive.  The first 256 threads work (act); in a larger
 // workgroup the others pass only the barriers.
-__device__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], bool act = true) {
+__device__ __forceinline__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1],
+                                             bool act = true) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   bool bad = false;
-  for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
-    if (wave == 0) {
-      const int i = lane;
-      double r[CP_W];
+  // Cholesky; step q0 factors panel q0 after applying panel p0 = q0 - 8
+  for (int q0 = 0; q0 < CL_B; q0 += CP_W) {
+    const int p0 = q0 - CP_W, i = q0 + lane;
+    if (p0 >= 0) {
+      // panel q0's columns k in [q0, q0 + 8), k <= i: wave w takes q0 + w, q0 + w + 4
+      if (act && i < CL_B) {
 #pragma unroll
-      for (int jj = 0; jj < CP_W; ++jj) r[jj] = S[i][p0 + jj];
+        for (int c = 0; c < 2; ++c) {
+          const int k = q0 + wave + 4 * c;
+          if (k <= i) {
+            double acc = S[i][k];
 #pragma unroll
-      for (int jj = 0; jj < CP_W; ++jj) {
-        const int j = p0 + jj;
-        double s = r[jj];
-#pragma unroll
-        for (int kk = 0; kk < jj; ++kk) s = fma(-r[kk], readlane_d(r[kk], j), s);
-        const double piv = readlane_d(s, j);
-        if (!(piv > 0.0)) bad = true;
-        const double d = sqrt(piv);
-        const double rd = 1.0 / d;
-        r[jj] = i > j ? s * rd : (i == j ? d : 0.0);
+            for (int jj = 0; jj < CP_W; ++jj) acc = fma(-S[i][p0 + jj], S[k][p0 + jj], acc);
+            S[i][k] = acc;
+          }
+        }
       }
-#pragma unroll
-      for (int jj = 0; jj < CP_W; ++jj) S[i][p0 + jj] = r[jj];
+      __syncthreads();
     }
-    __syncthreads();
-    const int q0 = p0 + CP_W;
-    const int i = q0 + lane;
-    if (act && i < CL_B) {
-      double li[CP_W];
+    if (wave == 0) {
+      bad = chol_panel8(S, q0, lane) || bad;
+    } else if (act) {
+      if (p0 < 0) {  // first step: Xs = I meanwhile
+        for (int x = t - 64; x < CL_B * CL_B; x += CP_THREADS - 64) {
+          const int r = x >> 6, c = x & 63;
+          Xs[r][c] = r == c ? 1.0 : 0.0;
+        }
+      } else if (i < CL_B) {  // panel p0's update of columns k in [q0 + 8, i]
+        double li[CP_W];
 #pragma unroll
-      for (int jj = 0; jj < CP_W; ++jj) li[jj] = S[i][p0 + jj];
-      for (int k = q0 + wave; k <= i; k += 4) {
-        double acc = S[i][k];
+        for (int jj = 0; jj < CP_W; ++jj) li[jj] = S[i][p0 + jj];
+        for (int k = q0 + CP_W + wave - 1; k <= i; k += 3) {
+          double acc = S[i][k];
 #pragma unroll
-        for (int jj = 0; jj < CP_W; ++jj) acc = fma(-li[jj], S[k][p0 + jj], acc);
-        S[i][k] = acc;
+          for (int jj = 0; jj < CP_W; ++jj) acc = fma(-li[jj], S[k][p0 + jj], acc);
+          S[i][k] = acc;
+        }
       }
     }
     __syncthreads();
   }
-  if (act)
-    for (int x = t; x < CL_B * CL_B; x += CP_THREADS) {
-      const int r = x >> 6, c = x & 63;
-      Xs[r][c] = r == c ? 1.0 : 0.0;
-    }
-  __syncthreads();
-  for (int p0 = 0; p0 < CL_B; p0 += CP_W) {
-    if (wave == 0) {
-      const int c = lane;
-      double x[CP_W];
+  // X = L^-1; step q0 solves panel q0's rows after applying panel p0 = q0 - 8
+  for (int q0 = 0; q0 < CL_B; q0 += CP_W) {
+    const int p0 = q0 - CP_W, c = lane;
+    if (p0 >= 0) {
+      if (act) {  // panel q0's rows r in [q0, q0 + 8): wave w takes q0 + w, q0 + w + 4
 #pragma unroll
-      for (int rr = 0; rr < CP_W; ++rr) x[rr] = Xs[p0 + rr][c];
+        for (int h = 0; h < 2; ++h) {
+          const int r = q0 + wave + 4 * h;
+          double acc = Xs[r][c];
 #pragma unroll
-      for (int rr = 0; rr < CP_W; ++rr) {
-        double s = x[rr];
-#pragma unroll
-        for (int kk = 0; kk < rr; ++kk) s = fma(-S[p0 + rr][p0 + kk], x[kk], s);
-        x[rr] = s / S[p0 + rr][p0 + rr];
+          for (int kk = 0; kk < CP_W; ++kk) acc = fma(-S[r][p0 + kk], Xs[p0 + kk][c], acc);
+          Xs[r][c] = acc;
+        }
       }
-#pragma unroll
-      for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
+      __syncthreads();
     }
-    __syncthreads();
-    const int q0 = p0 + CP_W;
-    if (act && q0 < CL_B) {
-      const int c = lane;
+    if (wave == 0) {
+      tri_panel8(S, Xs, q0, lane);
+    } else if (act && p0 >= 0) {  // panel p0's update of rows r >= q0 + 8
       double xk[CP_W];
 #pragma unroll
       for (int kk = 0; kk < CP_W; ++kk) xk[kk] = Xs[p0 + kk][c];
-      for (int r = q0 + wave; r < CL_B; r += 4) {
+      for (int r = q0 + CP_W + wave - 1; r < CL_B; r += 3) {
         double acc = Xs[r][c];
 #pragma unroll
         for (int kk = 0; kk < CP_W; ++kk) acc = fma(-S[r][p0 + kk], xk[kk], acc);
