@@ -202,4 +202,23 @@ hipError_t launch_eig_cameras(const double *cam, int nc, int cp, double *vals, d
 hipError_t launch_eig_points(const double *pts, long long np, double *vals, double *vecs,
                              hipStream_t st);
 
+// 1/sqrt(x) and 1/x for normal, finite x: the hardware estimates refined by two
+// Newton steps each (quadratic convergence from the estimate's >= 20 bits),
+// without the special-value handling of the library versions; callers use the
+// result only where the operand is never zero, denormal, infinite or NaN.
+__device__ __forceinline__ double rsqrt_normal(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  return y;
+}
+
+__device__ __forceinline__ double rcp_normal(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
 }  // namespace unc

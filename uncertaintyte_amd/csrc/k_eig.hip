@@ -63,25 +63,6 @@ __device__ __forceinline__ bool needs_rotation(double apq, double app, double aq
   return apq * apq > fmax(kJacobiEps * kJacobiEps * fabs(app * aqq), 2.2250738585072014e-308);
 }
 
-// 1/sqrt(x) and 1/x for normal, finite x: the hardware estimates refined by two
-// Newton steps each (quadratic convergence from the estimate's >= 20 bits),
-// without the special-value handling of the library versions; the operands
-// here are never zero, denormal, infinite or NaN when their result is used.
-__device__ __forceinline__ double rsqrt_normal(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double hx = 0.5 * x;
-  y = fma(y, fma(-hx * y, y, 0.5), y);
-  y = fma(y, fma(-hx * y, y, 0.5), y);
-  return y;
-}
-
-__device__ __forceinline__ double rcp_normal(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = fma(y, fma(-x, y, 1.0), y);
-  y = fma(y, fma(-x, y, 1.0), y);
-  return y;
-}
-
 // True when no pair would rotate: the sweep that would find that is skipped.
 template <int n>
 __device__ __forceinline__ bool converged(SymBlock<n> &A) {

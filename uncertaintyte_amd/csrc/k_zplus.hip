@@ -199,9 +199,12 @@ __device__ __forceinline__ bool chol_panel8(double (*S)[CL_B + 1], int p0, int l
     for (int kk = 0; kk < jj; ++kk) s = fma(-r[kk], readlane_d(r[kk], j), s);
     const double piv = readlane_d(s, j);
     if (!(piv > 0.0)) bad = true;
-    const double d = sqrt(piv);
-    const double rd = 1.0 / d;
+    // 1 / L_jj from the refined reciprocal square root; it is kept in the
+    // padding column for the triangular solve
+    const double rd = rsqrt_normal(piv);
+    const double d = piv * rd;
     r[jj] = i > j ? s * rd : (i == j ? d : 0.0);
+    if (i == j) S[j][CL_B] = rd;
   }
 #pragma unroll
   for (int jj = 0; jj < CP_W; ++jj) S[i][p0 + jj] = r[jj];
@@ -221,7 +224,7 @@ __device__ __forceinline__ void tri_panel8(double (*S)[CL_B + 1], double (*Xs)[C
     double s = x[rr];
 #pragma unroll
     for (int kk = 0; kk < rr; ++kk) s = fma(-S[p0 + rr][p0 + kk], x[kk], s);
-    x[rr] = s / S[p0 + rr][p0 + rr];
+    x[rr] = s * S[p0 + rr][CL_B];
   }
 #pragma unroll
   for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
