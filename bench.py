@@ -390,7 +390,7 @@ def main() -> int:
         "config": {"workload": WORKLOADS.get(args.workload, args.workload) + ", " + PATHS[args.precision],
                    "cameras": nC, "points": nP, "observations": nO,
                    "jacobian_nnz": nnz, "N": N, "taylor_terms": terms, "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "Taylor-chain product T X0: k_gemm_f64w8 (lower tiles, 3 K chunks) + k_split_reduce_sym", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": "Taylor-chain product T X0: k_gemm_f64w8 (lower tiles in XCD-contiguous blocks, 4 K chunks) + k_split_reduce_sym", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": gemm_traffic(),
                      "per_launch_flops": flops, "avg_launch_ms": round(avg_gemm_s * 1e3, 4),

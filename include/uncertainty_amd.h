@@ -233,7 +233,7 @@ int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, 
                             const double *B, int ldb, double *C, int ldc, double *absmax_out);
 /* C = alpha A B for n x n column-major A, B whose product is symmetric (powers
    of one symmetric matrix, as in the Taylor chain): only the 128 x 128 tiles on
-   and below the diagonal are computed, in three K chunks, and the lower triangle
+   and below the diagonal are computed, in three or four K chunks, and the lower triangle
    is mirrored, so C is exactly symmetric; *absmax_out = max|C|.
    UNC_ERR_UNSUPPORTED where the library would not use this form (same shapes
    as unc_diag_gemm_f64_split). */
@@ -261,7 +261,9 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
  * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles. */
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
 /* Select the GEMM kernel used by every later call in this process (tuning):
-   0-3 kernel variants; 21-24: 1-4 K chunks for the symmetric chain product. */
+   0-3 kernel variants; 20: automatic and 21-24: 1-4 K chunks for the symmetric
+   chain product; 30 / 31: that product on all tiles / on the lower tiles in
+   XCD-contiguous blocks (the default). */
 int unc_diag_set_gemm_variant(int variant);
 
 const char *unc_error_string(int code);

@@ -71,7 +71,7 @@ def test_gemm_f64_split_k(M, N, K):
 @pytest.mark.parametrize("n", [1800, 1283, 2048])
 def test_gemm_f64_symmetric_product(n):
     """The Taylor chain's product T X0 of two powers of one symmetric matrix:
-    lower 128 x 128 tiles only, K in three chunks, lower triangle mirrored.  C
+    lower 128 x 128 tiles only, K in three or four chunks, lower triangle mirrored.  C
     must be exactly symmetric, equal the full product to FP64 rounding
     (ragged n included), and carry max|C|."""
     say(f"gemm sym {n}")

@@ -114,11 +114,12 @@ void set_gemm_variant(int v);
 bool gemm_split_applies(int M, int N, int K);
 // C = alpha A B for n x n operands (ld n) whose product is symmetric (powers
 // of one symmetric matrix): only the 128 x 128 tiles on and below the
-// diagonal are computed, over three K chunks where gemm_split_applies (work:
+// diagonal are computed, over three or four K chunks where gemm_split_applies (work:
 // gemm_sym_split_work_doubles(n) doubles), and the lower triangle is mirrored;
 // otherwise the plain product.  Same absmax / active contract as gemm_f64.
 size_t gemm_sym_split_work_doubles(int n);
-void set_sym_parts(int p);  // tuning: K chunks of the symmetric form (default 3, at most 4)
+void set_sym_parts(int p);  // tuning: K chunks of the symmetric form (0: automatic, at most 4)
+void set_sym_list(int on);  // tuning: 1 (default) lower tiles only in XCD-contiguous blocks
 hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                         double *work, size_t work_doubles, unsigned long long *absmax,
                         const int *active, hipStream_t st);
@@ -130,6 +131,9 @@ constexpr int GEMM_A_LOWER = 1;   // A lower triangular: K range ends at the til
 constexpr int GEMM_B_LOWER = 2;   // B lower triangular: K range starts at the tile's first column
 constexpr int GEMM_BT_LOWER = 4;  // op(B) = S^T with S lower: K range ends at the tile's last column
 constexpr int GEMM_C_LOWER = 8;   // only tiles on or below the diagonal of C are written
+// with GEMM_C_LOWER on a square C: the grid lists only the lower tiles (times
+// the K parts), in 4 x 4 tile blocks, each XCD taking a contiguous range
+constexpr int GEMM_LOWER_LIST = 16;
 hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                        int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
                        hipStream_t st);

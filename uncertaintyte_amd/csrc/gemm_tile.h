@@ -11,6 +11,37 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int GB_M = 128, GB_N = 128;
 constexpr int G_PAD = 4;
 
+// Tile t of the lower triangle of a tiles x tiles grid (tn <= tm), ordered by
+// 4 x 4 blocks of tiles (blocks row by row over the lower triangle, tiles row
+// by row inside a block), so that a run of consecutive t covers a compact
+// region whose workgroups share A and B panels.
+__device__ __forceinline__ void lower_tile_blocked(int t, int tiles, int &tm, int &tn) {
+  const int nsb = (tiles + 3) / 4;
+  for (int sr = 0; sr < nsb; ++sr)
+    for (int sc = 0; sc <= sr; ++sc) {
+      const int r0 = 4 * sr, c0 = 4 * sc;
+      const int rn = min(4, tiles - r0), cn = min(4, tiles - c0);
+      for (int r = 0; r < rn; ++r) {
+        const int len = sr == sc ? min(r + 1, cn) : cn;  // diagonal block: tn <= tm
+        if (t < len) {
+          tm = r0 + r;
+          tn = c0 + t;
+          return;
+        }
+        t -= len;
+      }
+    }
+  tm = tn = 0;  // not reached for t below the tile count
+}
+
+// Workgroup id -> item of the launch such that each of the 8 XCDs (ids are
+// dealt to them round robin) takes one contiguous range of items.
+__device__ __forceinline__ int xcd_contiguous(int bid, int total) {
+  const int x = bid & 7, slot = bid >> 3;
+  const int base = total >> 3, extra = total & 7;
+  return x * base + min(x, extra) + slot;
+}
+
 // One 128 x 128 tile of C = alpha op(A) B + beta C for the workgroup's tile
 // index tile_bid (512 threads, 8 waves of 64 x 32; tiles in 8-row groups so
 // neighbouring workgroups share A panels in L2).  tri: triangular-operand flags
@@ -34,18 +65,25 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
   const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
-  int bid = tile_bid, part = 0;
-  if (SPLIT) {
-    const int nt = tiles_m * tiles_n;
-    part = bid / nt;
-    bid -= part * nt;
+  int bid = tile_bid, part = 0, tm, tn;
+  if (tri & GEMM_LOWER_LIST) {  // grid = parts x lower tiles, chunk-major
+    const int nl = tiles_m * (tiles_m + 1) / 2;
+    const int w = xcd_contiguous(bid, parts * nl);
+    part = w / nl;
+    lower_tile_blocked(w - part * nl, tiles_m, tm, tn);
+  } else {
+    if (SPLIT) {
+      const int nt = tiles_m * tiles_n;
+      part = bid / nt;
+      bid -= part * nt;
+    }
+    const int group = 8;
+    const int per_group = group * tiles_n;
+    const int first_m = (bid / per_group) * group;
+    const int gsize = min(tiles_m - first_m, group);
+    tm = first_m + (bid % per_group) % gsize;
+    tn = (bid % per_group) / gsize;
   }
-  const int group = 8;
-  const int per_group = group * tiles_n;
-  const int first_m = (bid / per_group) * group;
-  const int gsize = min(tiles_m - first_m, group);
-  const int tm = first_m + (bid % per_group) % gsize;
-  const int tn = (bid % per_group) / gsize;
   const int m0 = tm * GB_M, n0 = tn * GB_N;
   if ((tri & GEMM_C_LOWER) && n0 >= m0 + GB_M) return;  // tile strictly above the diagonal
   int kbeg = 0, kend = K;

@@ -1206,7 +1206,7 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
   HC(hipEventSynchronize(ctx->gemm_ev[1]));
   *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
   set_gemm_variant(0);
-  set_sym_parts(3);
+  set_sym_parts(0);
   HC(hipFree(buf));
   return UNC_OK;
 }
@@ -1239,8 +1239,12 @@ int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
 }
 
 int unc_diag_set_gemm_variant(int variant) {
-  if (variant >= 21 && variant <= 24) {  // K chunks of the symmetric chain product
+  if (variant >= 20 && variant <= 24) {  // K chunks of the symmetric chain product
     set_sym_parts(variant - 20);
+    return UNC_OK;
+  }
+  if (variant == 30 || variant == 31) {  // symmetric product: all tiles / lower-tile list
+    set_sym_list(variant - 30);
     return UNC_OK;
   }
   if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
