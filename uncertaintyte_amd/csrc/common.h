@@ -120,6 +120,7 @@ bool gemm_split_applies(int M, int N, int K);
 size_t gemm_sym_split_work_doubles(int n);
 void set_sym_parts(int p);  // tuning: K chunks of the symmetric form (0: automatic, at most 4)
 void set_sym_list(int on);  // tuning: 1 (default) lower tiles only in XCD-contiguous blocks
+void set_split_blocked(int on);  // tuning: 1 (default) split product in XCD-contiguous blocks
 hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                         double *work, size_t work_doubles, unsigned long long *absmax,
                         const int *active, hipStream_t st);
@@ -134,6 +135,9 @@ constexpr int GEMM_C_LOWER = 8;   // only tiles on or below the diagonal of C ar
 // with GEMM_C_LOWER on a square C: the grid lists only the lower tiles (times
 // the K parts), in 4 x 4 tile blocks, each XCD taking a contiguous range
 constexpr int GEMM_LOWER_LIST = 16;
+// all tiles (times the K parts) in 4 x 4 tile blocks, each XCD taking a
+// contiguous range of them
+constexpr int GEMM_BLOCKED = 32;
 hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                        int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
                        hipStream_t st);

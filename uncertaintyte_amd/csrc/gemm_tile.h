@@ -34,6 +34,31 @@ __device__ __forceinline__ void lower_tile_blocked(int t, int tiles, int &tm, in
   tm = tn = 0;  // not reached for t below the tile count
 }
 
+// Tile t of a tiles_m x tiles_n grid in 4 x 4 blocks (blocks row by row, tiles
+// row by row inside a block).
+__device__ __forceinline__ void full_tile_blocked(int t, int tiles_m, int tiles_n, int &tm,
+                                                  int &tn) {
+  const int bcols = (tiles_n + 3) / 4;
+  for (int sr = 0; 4 * sr < tiles_m; ++sr) {
+    const int rn = min(4, tiles_m - 4 * sr);
+    const int row_tiles = rn * tiles_n;  // tiles in this band of block rows
+    if (t >= row_tiles) {
+      t -= row_tiles;
+      continue;
+    }
+    for (int sc = 0; sc < bcols; ++sc) {
+      const int cn = min(4, tiles_n - 4 * sc);
+      if (t < rn * cn) {
+        tm = 4 * sr + t / cn;
+        tn = 4 * sc + t % cn;
+        return;
+      }
+      t -= rn * cn;
+    }
+  }
+  tm = tn = 0;  // not reached for t below the tile count
+}
+
 // Workgroup id -> item of the launch such that each of the 8 XCDs (ids are
 // dealt to them round robin) takes one contiguous range of items.
 __device__ __forceinline__ int xcd_contiguous(int bid, int total) {
@@ -71,6 +96,11 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
     const int w = xcd_contiguous(bid, parts * nl);
     part = w / nl;
     lower_tile_blocked(w - part * nl, tiles_m, tm, tn);
+  } else if (tri & GEMM_BLOCKED) {  // grid = parts x all tiles, chunk-major
+    const int nt = tiles_m * tiles_n;
+    const int w = xcd_contiguous(bid, parts * nt);
+    part = w / nt;
+    full_tile_blocked(w - part * nt, tiles_m, tiles_n, tm, tn);
   } else {
     if (SPLIT) {
       const int nt = tiles_m * tiles_n;

@@ -575,6 +575,9 @@ static int g_sym_parts = 0;  // 0: by the rule above
 void set_sym_parts(int p) { g_sym_parts = p < 0 ? 0 : (p > SYM_PARTS ? SYM_PARTS : p); }
 static int g_sym_list = 1;
 void set_sym_list(int on) { g_sym_list = on; }
+// the full split product in XCD-contiguous 4 x 4 tile blocks
+static int g_split_blocked = 1;
+void set_split_blocked(int on) { g_split_blocked = on; }
 
 bool gemm_split_applies(int M, int N, int K) {
   const long long tiles = cdiv(M, GB_M) * cdiv(N, GB_N);
@@ -617,7 +620,8 @@ hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, in
     return gemm_f64(M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, absmax, active, st);
   const int tiles = (int)cdiv(M, GB_M) * (int)cdiv(N, GB_N);
   hipLaunchKernelGGL((k_gemm_f64w8<16, false, true>), dim3(2 * tiles), dim3(512), 0, st, M, N, K,
-                     1.0, A, lda, B, ldb, 0.0, C, ldc, nullptr, active, 0, work, 2);
+                     1.0, A, lda, B, ldb, 0.0, C, ldc, nullptr, active, g_split_blocked ? GEMM_BLOCKED : 0,
+                     work, 2);
   const size_t mn = (size_t)M * N;
   if (ldc == M && mn % 4 == 0 && ((uintptr_t)C & 15) == 0 && ((uintptr_t)work & 15) == 0) {
     const size_t n4 = mn / 4;

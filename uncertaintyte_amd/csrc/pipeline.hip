@@ -1247,6 +1247,10 @@ int unc_diag_set_gemm_variant(int variant) {
     set_sym_list(variant - 30);
     return UNC_OK;
   }
+  if (variant == 40 || variant == 41) {  // split product: id order / XCD-contiguous blocks
+    set_split_blocked(variant - 40);
+    return UNC_OK;
+  }
   if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
   set_gemm_variant(variant);
   return UNC_OK;
