@@ -347,9 +347,10 @@ __global__ __launch_bounds__(256) void k_split_reduce_flat(size_t n4, double alp
 // v = alpha (C + W_1 + ...) and writes it to block (bi, bj) and its transpose
 // to block (bj, bi); in diagonal blocks the lower triangle is mirrored, so C is
 // exactly symmetric.  max|C| is folded over the lower triangle.
+template <int PARTS>
 __global__ __launch_bounds__(256) void k_split_reduce_sym(int n, double alpha,
                                                           double *__restrict__ C,
-                                                          const double *__restrict__ W, int parts,
+                                                          const double *__restrict__ W,
                                                           unsigned long long *absmax,
                                                           const int *active) {
   if (active && *active == 0) return;
@@ -363,22 +364,34 @@ __global__ __launch_bounds__(256) void k_split_reduce_sym(int n, double alpha,
   const int bj = x - bi * (bi + 1) / 2;
   const int i0 = bi * 64, j0 = bj * 64, tid = threadIdx.x;
   const size_t nn = (size_t)n * n;
-  double lmax = 0.0;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e & 63, c = e >> 6, gi = i0 + r, gj = j0 + c;
-    double v = 0.0;
-    if (gi < n && gj < n) {
-      const size_t o = (size_t)gi + (size_t)gj * n;
-      v = C[o];
-      for (int p = 1; p < parts; ++p) v += W[o + (size_t)(p - 1) * nn];
-      v *= alpha;
-      if (bi != bj || r >= c) lmax = fmax(lmax, fabs(v));
+  // every load of the block is issued before the first sum: 16 elements per
+  // thread, chunk by chunk, each added in chunk order
+  constexpr int E = 64 * 64 / 256;
+  double v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + 256 * q, gi = i0 + (e & 63), gj = j0 + (e >> 6);
+    v[q] = (gi < n && gj < n) ? C[(size_t)gi + (size_t)gj * n] : 0.0;
+  }
+#pragma unroll
+  for (int p = 1; p < PARTS; ++p)
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + 256 * q, gi = i0 + (e & 63), gj = j0 + (e >> 6);
+      if (gi < n && gj < n) v[q] += W[(size_t)gi + (size_t)gj * n + (size_t)(p - 1) * nn];
     }
-    V[r][c] = v;
+  double lmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+    const double s = alpha * v[q];
+    if (i0 + r < n && j0 + c < n && (bi != bj || r >= c)) lmax = fmax(lmax, fabs(s));
+    V[r][c] = s;
   }
   __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e & 63, c = e >> 6, gi = i0 + r, gj = j0 + c;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + 256 * q, r = e & 63, c = e >> 6, gi = i0 + r, gj = j0 + c;
     if (bi == bj) {  // mirror the lower triangle of the diagonal block
       if (gi < n && gj < n) C[(size_t)gi + (size_t)gj * n] = r >= c ? V[r][c] : V[c][r];
     } else {
@@ -607,8 +620,24 @@ hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, d
                        n, n, n, 1.0, A, n, B, n, 0.0, C, n, nullptr, active, GEMM_C_LOWER, work,
                        parts);
   const int nb = (int)cdiv(n, 64);
-  hipLaunchKernelGGL(k_split_reduce_sym, dim3(nb * (nb + 1) / 2), dim3(256), 0, st, n, alpha, C,
-                     work, parts, absmax, active);
+  const dim3 rgrid(nb * (nb + 1) / 2);
+  switch (parts) {
+    case 1:
+      hipLaunchKernelGGL(k_split_reduce_sym<1>, rgrid, dim3(256), 0, st, n, alpha, C, work, absmax,
+                         active);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_split_reduce_sym<2>, rgrid, dim3(256), 0, st, n, alpha, C, work, absmax,
+                         active);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_split_reduce_sym<3>, rgrid, dim3(256), 0, st, n, alpha, C, work, absmax,
+                         active);
+      break;
+    default:
+      hipLaunchKernelGGL(k_split_reduce_sym<4>, rgrid, dim3(256), 0, st, n, alpha, C, work, absmax,
+                         active);
+  }
   return hipGetLastError();
 }
 
