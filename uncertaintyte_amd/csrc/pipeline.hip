@@ -577,12 +577,16 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   int gemm_pairs = 0;
   double *X0 = dB;
   if (refine) {
+    // X0 from Gauss-Jordan is symmetric only to about cond * eps; symmetrised
+    // first, X0 R = X0 - X0 A X0 is symmetric too (up to rounding), so the
+    // lower-tile product serves.  (Left unsymmetrised, X0 R carries an
+    // antisymmetric part of the size of X0's asymmetry, which mirroring would
+    // fold into the result.)
+    HC(svd_symmetrize(N, dB, s));
     HC(residual_dd(N, dA, dB, dC, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    // the full product: X0 from Gauss-Jordan is symmetric only to about
-    // cond * eps, and X0 R carries an antisymmetric part of that size, which a
-    // lower-tile product would fold into the result (dE: scratch)
-    HC(gemm_f64_split(N, N, N, 1.0, dB, N, dC, N, dD, N, dE, nullptr, nullptr, s));
+    HC(gemm_f64_sym(N, 1.0, dB, dC, dD, (double *)ptr(B_SPLIT), gemm_sym_split_work_doubles(N),
+                    nullptr, nullptr, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     HC(sym_update(N, dB, dD, dE, s));
