@@ -361,57 +361,64 @@ __global__ __launch_bounds__(CP_THREADS) void k_chol_inv64p(int b, double *A, in
   if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
 }
 
-// P = X^T X (X = L^-1 in Xs) -> P (b x b, leading dimension b).  Each thread
-// forms a 4 x 4 set of entries, summing k in the same order for (i, j) and
-// (j, i), so P is exactly symmetric.
-__device__ __forceinline__ void store_xtx(int b, double (*Xs)[CL_B + 1], double *P) {
-  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
-  double acc[4][4];
+// 64 x 64 product on the FP64 matrix cores: wave w of the first four computes
+// the 32 x 32 quadrant (rows 32 (w & 1), columns 32 (w >> 1)) of
+// C = A[0:64][0:kk] B[0:kk][0:64] as 2 x 2 tiles of v_mfma_f64_16x16x4, k
+// ascending in steps of 4; a_at(row, k) and b_at(k, col) read the operands
+// (zero beyond kk up to the next multiple of 4).
+template <typename AF, typename BF>
+__device__ __forceinline__ void mfma_quadrant64(int kk, AF a_at, BF b_at, double4_t acc[2][2]) {
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
+  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int k = 0; k < CL_B; ++k) {
-    double xi[4], xj[4];
+    for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < kk; k0 += 4) {
+    const int k = k0 + (lane >> 4);
+    double av[2], bv[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xi[u] = Xs[k][ti + 16 * u];
+    for (int i = 0; i < 2; ++i) av[i] = a_at(wr + 16 * i + (lane & 15), k);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) xj[v] = Xs[k][tj + 16 * v];
+    for (int j = 0; j < 2; ++j) bv[j] = b_at(k, wc + 16 * j + (lane & 15));
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(xi[u], xj[v], acc[u][v]);
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = ti + 16 * u, j = tj + 16 * v;
-      if (i < b && j < b) P[(size_t)i + (size_t)j * b] = acc[u][v];
-    }
 }
 
-// 64 x 64 block product acc(4 x 4 per thread) = As[0:64][0:kk] Bs[0:kk][0:64]
-// (thread (ti, tj) of the first 256 owns rows ti + 16u, columns tj + 16v);
-// k ascending.
-__device__ __forceinline__ void lds_product64(int kk, double (*As)[CL_B + 1],
-                                              double (*Bs)[CL_B + 1], double acc[4][4]) {
-  const int ti = threadIdx.x & 15, tj = (threadIdx.x >> 4) & 15;
+// f(row, col, value, q) for the 16 entries q of the quadrant a thread holds
+template <typename F>
+__device__ __forceinline__ void quadrant64_for_each(const double4_t acc[2][2], F f) {
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
+  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int k = 0; k < kk; ++k) {
-    double a[4], c[4];
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = As[ti + 16 * u][k];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) c[v] = Bs[k][tj + 16 * v];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], c[v], acc[u][v]);
-  }
+      for (int r = 0; r < 4; ++r)
+        f(wr + 16 * i + (lane >> 4) + 4 * r, wc + 16 * j + (lane & 15), acc[i][j][r],
+          8 * i + 4 * j + r);
+}
+
+// P = X^T X (X = L^-1 in Xs) -> P (b x b, leading dimension b) on the matrix
+// cores, first four waves.  The entries on and below the diagonal are stored
+// at both (i, j) and (j, i), so P is exactly symmetric.
+__device__ __forceinline__ void store_xtx(int b, double (*Xs)[CL_B + 1], double *P) {
+  const int wave = (threadIdx.x >> 6) & 3;
+  if (wave == 2) return;  // the quadrant above the diagonal
+  double4_t acc[2][2];
+  mfma_quadrant64(CL_B, [&](int row, int k) { return Xs[k][row]; },
+                  [&](int k, int col) { return Xs[k][col]; }, acc);
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) {
+    if (i < b && j < b && i >= j) {
+      P[(size_t)i + (size_t)j * b] = v;
+      P[(size_t)j + (size_t)i * b] = v;
+    }
+  });
 }
 
 // P = W^-1 for the b x b SPD diagonal block of the blocked Gauss-Jordan
@@ -462,16 +469,12 @@ __global__ __launch_bounds__(CP_THREADS) void k_gj_panel_a(int n, int k0, int kb
   });
   stage64(Bs, [&](int i, int k) { return (i < kb && k < kb) ? P[i + k * kb] : 0.0; });
   __syncthreads();
-  double acc[4][4];
-  lds_product64(kb, As, Bs, acc);
-  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = g0 + ti + 16 * u, j = tj + 16 * v;
-      if (i < n && j < kb) T[(size_t)i + (size_t)j * n] = acc[u][v];
-    }
+  double4_t acc[2][2];
+  mfma_quadrant64(kb, [&](int row, int k) { return As[row][k]; },
+                  [&](int k, int col) { return Bs[k][col]; }, acc);
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) {
+    if (g0 + i < n && j < kb) T[(size_t)(g0 + i) + (size_t)j * n] = v;
+  });
 }
 
 // Second half of the panel, for the 64-column / 64-row block g:
@@ -518,16 +521,12 @@ __device__ void gj_finish_block(double (*As)[CL_B + 1], double (*Bs)[CL_B + 1], 
   }, act);
   __syncthreads();
   if (!act) return;
-  double acc[4][4];
-  lds_product64(kb, As, Bs, acc);
-  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = ti + 16 * u, j = g0 + tj + 16 * v;
-      if (i < kb && j < n) W[(size_t)(k0 + i) + (size_t)j * n] = acc[u][v];
-    }
+  double4_t acc[2][2];
+  mfma_quadrant64(kb, [&](int row, int k) { return As[row][k]; },
+                  [&](int k, int col) { return Bs[k][col]; }, acc);
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) {
+    if (i < kb && g0 + j < n) W[(size_t)(k0 + i) + (size_t)(g0 + j) * n] = v;
+  });
 }
 
 // Look-ahead inverse of the next diagonal block: D = Dsrc - T1 R1 (Dsrc: b x b
@@ -543,27 +542,20 @@ __device__ void gj_next_inverse(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], b
   stage64(Xs, [&](int k, int i) { return (k < kb && i < b) ? R1[(size_t)k + (size_t)i * ldr] : 0.0; },
           act);
   __syncthreads();
-  double acc[4][4], d[4][4];
-  const int ti = threadIdx.x & 15, tj = (threadIdx.x >> 4) & 15;
+  double4_t acc[2][2];
+  double d[16];  // D's lower triangle: Dsrc less the product
   if (act) {
-    lds_product64(kb, S, Xs, acc);
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = ti + 16 * u, k = tj + 16 * v;
-        d[u][v] = (i < b && k <= i) ? Dsrc[i + k * CL_B] : 0.0;
-      }
+    mfma_quadrant64(kb, [&](int row, int k) { return S[row][k]; },
+                    [&](int k, int col) { return Xs[k][col]; }, acc);
+    quadrant64_for_each(acc, [&](int i, int k, double v, int q) {
+      d[q] = (i < b && k <= i) ? Dsrc[i + k * CL_B] - v : 0.0;
+    });
   }
-  __syncthreads();
+  __syncthreads();  // S and Xs are read by the product until here
   if (act) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = ti + 16 * u, k = tj + 16 * v;
-        S[i][k] = (i < b && k < b) ? (k <= i ? d[u][v] - acc[u][v] : 0.0) : (i == k ? 1.0 : 0.0);
-      }
+    quadrant64_for_each(acc, [&](int i, int k, double, int q) {
+      S[i][k] = (i < b && k < b) ? d[q] : (i == k ? 1.0 : 0.0);
+    });
   }
   __syncthreads();
   const bool bad = chol_inv_lds(S, Xs, act);
