@@ -311,52 +311,93 @@ __global__ __launch_bounds__(128) void k_pt_blocks(Structure s, Blocks b) {
   b.pt_has[p] = 1;
 }
 
+// Per observation: W_s = D_c J_c^T J_p D_p (CP x 3) and Y = W_s V_s^-1.  A
+// workgroup takes 64 consecutive observations: their Jacobian rows (one
+// contiguous run) are staged through LDS with every load issued up front, one
+// thread per observation forms its blocks into LDS, and the two contiguous
+// output runs are written back coalesced (observations dropped by fixPts keep
+// their old contents, as before).
+constexpr int OB_OBS = 64, OB_THREADS = 128;
+
 template <int CP>
-__global__ __launch_bounds__(256) void k_obs_blocks(Structure s, Blocks b) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= s.nobs) return;
-  const int a = s.obs_cam[o], p = s.obs_pt[o];
-  if (a < 0 || p < 0) return;
-  constexpr int RL = CP + 3;
-  const double *r0 = s.vals + (size_t)o * 2 * RL;
-  const double *r1 = r0 + RL;
-  const double *dc = b.d + a * CP;
-  const double *dpp = b.d + s.N + 3 * p;
-  const double *ivp = b.Vi + (size_t)p * 9;
-  double dd[3], iv[9], jp0[3], jp1[3];
+__global__ __launch_bounds__(OB_THREADS) void k_obs_blocks(Structure s, Blocks b) {
+  constexpr int RL = CP + 3, IN = OB_OBS * 2 * RL, OUT = OB_OBS * CP * 3;
+  constexpr int NIN = (IN + OB_THREADS - 1) / OB_THREADS;
+  __shared__ double jin[IN], wout[OUT], yout[OUT];
+  __shared__ int valid[OB_OBS];
+  const int o0 = blockIdx.x * OB_OBS, tid = threadIdx.x;
+  const int nob = min(OB_OBS, s.nobs - o0);
+  {
+    const double *src = s.vals + (size_t)o0 * 2 * RL;
+    const int nin = nob * 2 * RL;
+    double v[NIN];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    dd[k] = dpp[k];
-    jp0[k] = r0[CP + k];
-    jp1[k] = r1[CP + k];
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) iv[k] = ivp[k];
-  double *Ws = b.Ws + (size_t)o * CP * 3;
-  double *Y = b.Y + (size_t)o * CP * 3;
-#pragma unroll
-  for (int i = 0; i < CP; ++i) {
-    const double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
-    double w[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double wv = fma(ci1, jp1[k], ci0 * jp0[k]);
-      w[k] = (wv * dci) * dd[k];
-      Ws[i * 3 + k] = w[k];
+    for (int r = 0; r < NIN; ++r) {
+      const int x = tid + OB_THREADS * r;
+      v[r] = x < nin ? src[x] : 0.0;
     }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double y = w[0] * iv[0 * 3 + m];
-      y = fma(w[1], iv[1 * 3 + m], y);
-      y = fma(w[2], iv[2 * 3 + m], y);
-      Y[i * 3 + m] = y;
+    for (int r = 0; r < NIN; ++r) {
+      const int x = tid + OB_THREADS * r;
+      if (x < IN) jin[x] = v[r];
+    }
+  }
+  __syncthreads();
+  if (tid < nob) {
+    const int o = o0 + tid;
+    const int a = s.obs_cam[o], p = s.obs_pt[o];
+    valid[tid] = a >= 0 && p >= 0;
+    if (a >= 0 && p >= 0) {
+      const double *r0 = jin + tid * 2 * RL;
+      const double *r1 = r0 + RL;
+      const double *dc = b.d + a * CP;
+      const double *dpp = b.d + s.N + 3 * p;
+      const double *ivp = b.Vi + (size_t)p * 9;
+      double dd[3], iv[9], jp0[3], jp1[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        dd[k] = dpp[k];
+        jp0[k] = r0[CP + k];
+        jp1[k] = r1[CP + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) iv[k] = ivp[k];
+      double *Ws = wout + tid * CP * 3;
+      double *Y = yout + tid * CP * 3;
+#pragma unroll
+      for (int i = 0; i < CP; ++i) {
+        const double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
+        double w[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double wv = fma(ci1, jp1[k], ci0 * jp0[k]);
+          w[k] = (wv * dci) * dd[k];
+          Ws[i * 3 + k] = w[k];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          double y = w[0] * iv[0 * 3 + m];
+          y = fma(w[1], iv[1 * 3 + m], y);
+          y = fma(w[2], iv[2 * 3 + m], y);
+          Y[i * 3 + m] = y;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double *Wg = b.Ws + (size_t)o0 * CP * 3, *Yg = b.Y + (size_t)o0 * CP * 3;
+  const int nout = nob * CP * 3;
+  for (int x = tid; x < nout; x += OB_THREADS) {
+    if (valid[x / (CP * 3)]) {
+      Wg[x] = wout[x];
+      Yg[x] = yout[x];
     }
   }
 }
 
 template <int CP>
 static void launch_obs_blocks_cp(const Structure &s, const Blocks &b, hipStream_t st) {
-  hipLaunchKernelGGL(k_obs_blocks<CP>, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s, b);
+  hipLaunchKernelGGL(k_obs_blocks<CP>, dim3(cdiv(s.nobs, OB_OBS)), dim3(OB_THREADS), 0, st, s, b);
 }
 
 hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st) {
