@@ -1011,6 +1011,30 @@ __global__ void k_sym_update(int n, const double *X0, const double *D, double *o
   out[x] = 0.5 * ((X0[x] + D[x]) + (X0[t] + D[t]));
 }
 
+// out = X0 + D for symmetric X0 and D (both exact mirror images): equal, bit
+// for bit, to sym_update's 0.5 ((X0 + D) + (X0 + D)^T), read coalesced.
+__global__ void k_add_sym(size_t n2, const double2 *__restrict__ X0, const double2 *__restrict__ D,
+                          double2 *__restrict__ out) {
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n2;
+       x += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = X0[x], b = D[x];
+    double2 r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    out[x] = r;
+  }
+}
+
+hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st) {
+  const size_t tot = (size_t)n * n;
+  if ((tot & 1) || ((uintptr_t)X0 & 15) || ((uintptr_t)D & 15) || ((uintptr_t)out & 15))
+    return sym_update(n, X0, D, out, st);
+  const size_t n2 = tot / 2;
+  hipLaunchKernelGGL(k_add_sym, dim3((unsigned)std::min<size_t>(cdiv((long long)n2, 256), 4096)),
+                     dim3(256), 0, st, n2, (const double2 *)X0, (const double2 *)D, (double2 *)out);
+  return hipGetLastError();
+}
+
 hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st) {
   size_t tot = (size_t)n * n;
   hipLaunchKernelGGL(k_sym_update, dim3(cdiv(tot, 256)), dim3(256), 0, st, n, X0, D, out);

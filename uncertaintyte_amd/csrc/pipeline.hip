@@ -589,7 +589,10 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
                     nullptr, nullptr, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
-    HC(sym_update(N, dB, dD, dE, s));
+    // X0 was symmetrised and the lower-tile product mirrors D: the plain sum
+    // is then already symmetric
+    if (gemm_sym_split_work_doubles(N) > 0) HC(add_symmetric(N, dB, dD, dE, s));
+    else HC(sym_update(N, dB, dD, dE, s));
     X0 = dE;
   }
   HC(hipEventRecord(E[7], s));
