@@ -71,9 +71,10 @@ def test_gemm_f64_split_k(M, N, K):
 @pytest.mark.parametrize("n", [1800, 1283, 2048])
 def test_gemm_f64_symmetric_product(n):
     """The Taylor chain's product T X0 of two powers of one symmetric matrix:
-    lower 128 x 128 tiles only, K in three or four chunks, lower triangle mirrored.  C
-    must be exactly symmetric, equal the full product to FP64 rounding
-    (ragged n included), and carry max|C|."""
+    lower 128 x 128 tiles only, K in three or four chunks, the epilogue adding
+    them in chunk order and mirroring the lower triangle.  C must be exactly
+    symmetric, equal the full product to FP64 rounding (ragged n included),
+    carry max|C| and be the same on a second run."""
     say(f"gemm sym {n}")
     rng = np.random.default_rng(n)
     Q = rng.standard_normal((n, n)) / np.sqrt(n)
@@ -85,6 +86,9 @@ def test_gemm_f64_symmetric_product(n):
     assert np.array_equal(C, C.T)
     assert np.abs(C - ref).max() <= 1e-13 * np.abs(S2).max() * np.abs(S).max() * n
     assert m == np.abs(C).max()
+    # the chunks are added in a fixed order: run to run the result is identical
+    C2, m2 = u.diag_gemm_sym(S2, S, alpha=0.5)
+    assert np.array_equal(C, C2) and m == m2
 
 
 def test_gemm_f64_split_k_declines_small_shapes():
