@@ -63,12 +63,12 @@ __global__ void k_pair_emit(Structure s, int t0, int t1, int a0, int base, const
   }
 }
 
-constexpr int SR_THREADS = 128, SR_CHUNK = 64, SR_GRID = 2048;
+constexpr int SR_THREADS = 128, SR_CHUNK = 32, SR_GRID = 2560;
 constexpr int SR_ROW = 27;  // cp * 3 for cp <= 9
 
 // one run (block (a, b)) per workgroup iteration; thread e owns element (i, j)
-// (upper triangle only on diagonal blocks).  Per chunk of 64 pairs: the pairs'
-// (o1, o2) are staged first, then every thread issues its share of the 2 x 64
+// (upper triangle only on diagonal blocks).  Per chunk of SR_CHUNK pairs: the pairs'
+// (o1, o2) are staged first, then every thread issues its share of the 2 x SR_CHUNK
 // Y / W rows as independent loads.
 __global__ __launch_bounds__(SR_THREADS) void k_schur_reduce(Structure s, Blocks bl, double *Z,
                                                              int a0, const unsigned int *ukeys,
