@@ -185,8 +185,8 @@ hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st);
 hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st);
 // out = X0 + D when both are exactly symmetric (the same value as sym_update)
 hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st);
-hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st);
-hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st);
+hipError_t te_step(int n, TeCtrl *ctrl, int i, double k, double *X, const double *T,
+                   hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
 // SVD algorithm modes (k_svd.hip)
 hipError_t svd_jt_scatter(int nrows, int npar, const int *rows, const int *cols,

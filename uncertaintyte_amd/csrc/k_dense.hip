@@ -1041,40 +1041,47 @@ hipError_t sym_update(int n, const double *X0, const double *D, double *out, hip
   return hipGetLastError();
 }
 
-// Taylor stopping rule (compute.cpp:146-156) evaluated on the device
-__global__ void k_te_control(TeCtrl *c, int i, double k) {
-  if (threadIdx.x != 0 || !c->active) return;
-  double m = __longlong_as_double((long long)c->maxbits[i]);
-  double change = fabs(k * m);
-  if (change < 1e-5 || change > c->old_change) {
-    c->active = 0;
-    return;
+// One Taylor step: the stopping rule (compute.cpp:146-156) and, for an accepted
+// term, iZupdate X = X + k T (compute.cpp:64-71), in one launch.  Every workgroup
+// evaluates the rule itself; workgroup 0 alone writes the new state.  The state a
+// workgroup reads is either the one before the step or the one workgroup 0 leaves,
+// and both give the same decision: accepting writes old_change = change, which
+// accepts again (change > change is false); stopping leaves old_change alone or
+// clears `active`.
+__global__ void k_te_step(TeCtrl *c, int i, double k, size_t nn, double *X, const double *T) {
+  __shared__ int accept_s;
+  if (threadIdx.x == 0) {
+    int accept = 0;
+    if (c->active) {
+      const double m = __longlong_as_double((long long)c->maxbits[i]);
+      const double change = fabs(k * m);
+      accept = !(change < 1e-5 || change > c->old_change);
+      if (blockIdx.x == 0) {
+        if (accept) {
+          c->old_change = change;
+          c->change[c->nterms] = change;
+          c->nterms = c->nterms + 1;
+          c->coef = k;
+        } else {
+          c->active = 0;
+        }
+      }
+    }
+    accept_s = accept;
   }
-  c->old_change = change;
-  c->change[c->nterms] = change;
-  c->nterms = c->nterms + 1;
-  c->coef = k;
-}
-
-hipError_t te_control(TeCtrl *ctrl, int i, double k, hipStream_t st) {
-  hipLaunchKernelGGL(k_te_control, dim3(1), dim3(64), 0, st, ctrl, i, k);
-  return hipGetLastError();
-}
-
-// iZupdate (compute.cpp:64-71): X = X + k T while the series is active
-__global__ void k_te_axpy(size_t nn, double *X, const double *T, const TeCtrl *c) {
-  if (!c->active) return;
-  const double k = c->coef;
+  __syncthreads();
+  if (!accept_s) return;
   for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < nn;
        x += (size_t)gridDim.x * blockDim.x)
     X[x] = X[x] + k * T[x];
 }
 
-hipError_t te_axpy(int n, double *X, const double *T, const TeCtrl *ctrl, hipStream_t st) {
+hipError_t te_step(int n, TeCtrl *ctrl, int i, double k, double *X, const double *T,
+                   hipStream_t st) {
   size_t nn = (size_t)n * n;
   unsigned int grid = cdiv(nn, 256);
   if (grid > 8192) grid = 8192;
-  hipLaunchKernelGGL(k_te_axpy, dim3(grid), dim3(256), 0, st, nn, X, T, ctrl);
+  hipLaunchKernelGGL(k_te_step, dim3(grid), dim3(256), 0, st, ctrl, i, k, nn, X, T);
   return hipGetLastError();
 }
 

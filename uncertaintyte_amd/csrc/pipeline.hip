@@ -624,8 +624,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   double *Tc = Tb0, *Tn = Tb1;
   for (int i = 1; i < 20; ++i) {
     const double k = pow(lambda, i) / (double)fact32(i - 1);
-    HC(te_control(ctrl, i, k, s));
-    HC(te_axpy(N, X, Tc, ctrl, s));
+    HC(te_step(N, ctrl, i, k, X, Tc, s));
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
     HC(gemm_f64_sym(N, 1.0, Tc, X0, Tn, sym_work, sym_work_doubles, &ctrl->maxbits[i + 1],
                     &ctrl->active, s));
