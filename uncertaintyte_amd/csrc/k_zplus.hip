@@ -448,26 +448,27 @@ __global__ __launch_bounds__(CP_THREADS) void k_gj_panel_a(int n, int k0, int kb
   __shared__ double As[CL_B][CL_B + 1];
   __shared__ double Bs[CL_B][CL_B + 1];
   const int g0 = blockIdx.x * CL_B;
-  {  // R[:, cols_g] = W[K, cols_g]; block 0: Dsrc = W[K1, K1]
-    double v[STAGE_R], w[STAGE_R];
+  // every load of the workgroup is issued before the first is used (one
+  // memory round trip): R's block of W[K, :], the next diagonal block, and the
+  // operands W[rows_g, K] and P of T's block
+  double v[STAGE_R], w[STAGE_R], a[STAGE_R], pv[STAGE_R];
 #pragma unroll
-    for (int r = 0; r < STAGE_R; ++r) {
-      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
-      v[r] = (i < kb && g0 + k < n) ? W[(size_t)(k0 + i) + (size_t)(g0 + k) * n] : 0.0;
-      w[r] = (blockIdx.x == 0 && i < kb1 && k < kb1) ? W[(size_t)(k1 + i) + (size_t)(k1 + k) * n]
-                                                     : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < STAGE_R; ++r) {
-      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
-      if (i < kb && g0 + k < n) R[(size_t)i + (size_t)(g0 + k) * kb] = v[r];
-      if (blockIdx.x == 0 && i < kb1 && k < kb1) Dsrc[i + k * CL_B] = w[r];
-    }
+  for (int r = 0; r < STAGE_R; ++r) {
+    const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+    v[r] = (i < kb && g0 + k < n) ? W[(size_t)(k0 + i) + (size_t)(g0 + k) * n] : 0.0;
+    w[r] = (blockIdx.x == 0 && i < kb1 && k < kb1) ? W[(size_t)(k1 + i) + (size_t)(k1 + k) * n]
+                                                   : 0.0;
+    a[r] = (g0 + i < n && k < kb) ? W[(size_t)(g0 + i) + (size_t)(k0 + k) * n] : 0.0;
+    pv[r] = (i < kb && k < kb) ? P[i + k * kb] : 0.0;
   }
-  stage64(As, [&](int i, int k) {
-    return (g0 + i < n && k < kb) ? W[(size_t)(g0 + i) + (size_t)(k0 + k) * n] : 0.0;
-  });
-  stage64(Bs, [&](int i, int k) { return (i < kb && k < kb) ? P[i + k * kb] : 0.0; });
+#pragma unroll
+  for (int r = 0; r < STAGE_R; ++r) {
+    const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+    if (i < kb && g0 + k < n) R[(size_t)i + (size_t)(g0 + k) * kb] = v[r];
+    if (blockIdx.x == 0 && i < kb1 && k < kb1) Dsrc[i + k * CL_B] = w[r];
+    As[i][k] = a[r];
+    Bs[i][k] = pv[r];
+  }
   __syncthreads();
   double4_t acc[2][2];
   mfma_quadrant64(kb, [&](int row, int k) { return As[row][k]; },
