@@ -537,19 +537,38 @@ __device__ void gj_finish_block(double (*As)[CL_B + 1], double (*Bs)[CL_B + 1], 
 __device__ void gj_next_inverse(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], bool act, int b,
                                 const double *Dsrc, const double *T1, int ldt, const double *R1,
                                 int ldr, int kb, double *Pn, unsigned int *err) {
-  // S = T1 (b x kb), Xs = R1 (kb x b)
-  stage64(S, [&](int i, int k) { return (i < b && k < kb) ? T1[(size_t)i + (size_t)k * ldt] : 0.0; },
-          act);
-  stage64(Xs, [&](int k, int i) { return (k < kb && i < b) ? R1[(size_t)k + (size_t)i * ldr] : 0.0; },
-          act);
-  __syncthreads();
+  // S = T1 (b x kb), Xs = R1 (kb x b); every load (with the thread's entries of
+  // Dsrc) is issued before the first is used
   double4_t acc[2][2];
   double d[16];  // D's lower triangle: Dsrc less the product
+  if (act) {
+    double tv[STAGE_R], rv[STAGE_R];
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r, i = x & 63, k = x >> 6;
+      tv[r] = (i < b && k < kb) ? T1[(size_t)i + (size_t)k * ldt] : 0.0;
+      rv[r] = (i < kb && k < b) ? R1[(size_t)i + (size_t)k * ldr] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    quadrant64_for_each(acc, [&](int i, int k, double, int q) {
+      d[q] = (i < b && k <= i) ? Dsrc[i + k * CL_B] : 0.0;
+    });
+#pragma unroll
+    for (int r = 0; r < STAGE_R; ++r) {
+      const int x = threadIdx.x + CP_THREADS * r;
+      S[x & 63][x >> 6] = tv[r];
+      Xs[x & 63][x >> 6] = rv[r];
+    }
+  }
+  __syncthreads();
   if (act) {
     mfma_quadrant64(kb, [&](int row, int k) { return S[row][k]; },
                     [&](int k, int col) { return Xs[k][col]; }, acc);
     quadrant64_for_each(acc, [&](int i, int k, double v, int q) {
-      d[q] = (i < b && k <= i) ? Dsrc[i + k * CL_B] - v : 0.0;
+      d[q] = (i < b && k <= i) ? d[q] - v : 0.0;
     });
   }
   __syncthreads();  // S and Xs are read by the product until here
