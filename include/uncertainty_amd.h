@@ -264,7 +264,8 @@ int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
    0-3 kernel variants; 20: automatic and 21-24: 1-4 K chunks for the symmetric
    chain product; 30 / 31: that product on all tiles / on the lower tiles in
    XCD-contiguous blocks (the default); 40 / 41: the split product in id
-   order / in XCD-contiguous blocks (the default). */
+   order / in XCD-contiguous blocks (the default); 50: the Schur reduction's
+   chunk size by launch size (the default), 51 / 52: 64 / 32 pairs per chunk. */
 int unc_diag_set_gemm_variant(int variant);
 
 const char *unc_error_string(int code);

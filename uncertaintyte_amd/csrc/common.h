@@ -85,7 +85,7 @@ hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base
                             unsigned int *keys, unsigned long long *vals, hipStream_t st);
 hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
                                const unsigned int *ukeys, const int *counts, const int *offs,
-                               const int *nruns, const unsigned long long *pairs,
+                               const int *nruns, const unsigned long long *pairs, long long npairs,
                                hipStream_t st);
 size_t pair_sort_temp_bytes(int n, int key_bits);
 hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out,
@@ -185,6 +185,7 @@ hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st);
 hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st);
 // out = X0 + D when both are exactly symmetric (the same value as sym_update)
 hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st);
+void set_schur_variant(int v);
 hipError_t te_step(int n, TeCtrl *ctrl, int i, double k, double *X, const double *T,
                    hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);

@@ -63,13 +63,14 @@ __global__ void k_pair_emit(Structure s, int t0, int t1, int a0, int base, const
   }
 }
 
-constexpr int SR_THREADS = 128, SR_CHUNK = 32, SR_GRID = 2560;
+constexpr int SR_THREADS = 128;
 constexpr int SR_ROW = 27;  // cp * 3 for cp <= 9
 
 // one run (block (a, b)) per workgroup iteration; thread e owns element (i, j)
 // (upper triangle only on diagonal blocks).  Per chunk of SR_CHUNK pairs: the pairs'
 // (o1, o2) are staged first, then every thread issues its share of the 2 x SR_CHUNK
 // Y / W rows as independent loads.
+template <int SR_CHUNK>
 __global__ __launch_bounds__(SR_THREADS) void k_schur_reduce(Structure s, Blocks bl, double *Z,
                                                              int a0, const unsigned int *ukeys,
                                                              const int *counts, const int *offs,
@@ -159,13 +160,25 @@ hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base
   return hipGetLastError();
 }
 
+static int g_schur_variant = 0;
+void set_schur_variant(int v) { g_schur_variant = v; }
+
+// npairs: pairs in this launch.  Large launches (long runs) take 64-pair chunks
+// (twice the loads in flight per round trip, 3 waves per SIMD); small ones 32-pair
+// chunks (5 waves per SIMD).  Measured: rings200 (1.22 M pairs) 0.74 -> 0.68 ms with
+// 32, walk1500 (16.4 M pairs) 11.3 -> 16.7 ms.
 hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
                                const unsigned int *ukeys, const int *counts, const int *offs,
-                               const int *nruns, const unsigned long long *pairs,
+                               const int *nruns, const unsigned long long *pairs, long long npairs,
                                hipStream_t st) {
   if (s.cp > 9) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_schur_reduce, dim3(SR_GRID), dim3(SR_THREADS), 0, st, s, b, Z, a0, ukeys,
-                     counts, offs, nruns, pairs);
+  const bool wide = g_schur_variant == 1 || (g_schur_variant == 0 && npairs >= (1ll << 22));
+  if (wide)
+    hipLaunchKernelGGL(k_schur_reduce<64>, dim3(2048), dim3(SR_THREADS), 0, st, s, b, Z, a0, ukeys,
+                       counts, offs, nruns, pairs);
+  else
+    hipLaunchKernelGGL(k_schur_reduce<32>, dim3(2560), dim3(SR_THREADS), 0, st, s, b, Z, a0, ukeys,
+                       counts, offs, nruns, pairs);
   return hipGetLastError();
 }
 

@@ -512,7 +512,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
         HC(pair_sort(k0, k1, v0, v1, np_, kb, ptmp, ptmp_bytes, s));
         HC(runs(k1, np_, rk, rc_, nrun, ptmp, ptmp_bytes, s));
         HC(exclusive_scan_int(rc_, ro, np_, ptmp, ptmp_bytes, s));
-        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, s));
+        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, np_, s));
       }
       a0 = a1;
     }
@@ -1255,6 +1255,10 @@ int unc_diag_set_gemm_variant(int variant) {
   }
   if (variant == 40 || variant == 41) {  // split product: id order / XCD-contiguous blocks
     set_split_blocked(variant - 40);
+    return UNC_OK;
+  }
+  if (variant >= 50 && variant <= 52) {  // Schur reduction: by size / 64-pair / 32-pair chunks
+    set_schur_variant(variant - 50);
     return UNC_OK;
   }
   if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
