@@ -48,15 +48,43 @@ __global__ void k_jt_scatter(int nrows, int npar, const int *__restrict__ rows,
   }
 }
 
-// M <- (M + M^T) / 2 on the lower triangle, mirrored
-__global__ void k_symmetrize(int n, double *M) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * n) return;
-  const int i = (int)(idx % n), j = (int)(idx / n);
-  if (i <= j) return;
-  const double v = 0.5 * (M[(size_t)i + (size_t)j * n] + M[(size_t)j + (size_t)i * n]);
-  M[(size_t)i + (size_t)j * n] = v;
-  M[(size_t)j + (size_t)i * n] = v;
+// M <- (M + M^T) / 2, each off-diagonal pair set to 0.5 (lower + upper).  One
+// workgroup per 64 x 64 tile (bi, bj), bi >= bj, of the lower triangle: it stages
+// that tile and its mirror (bj, bi) through LDS, so that both are read and
+// written along columns.
+constexpr int SYM_T = 64;
+__global__ __launch_bounds__(256) void k_symmetrize(int n, double *M) {
+  __shared__ double L[SYM_T][SYM_T + 1], U[SYM_T][SYM_T + 1];
+  const int t = blockIdx.x;
+  int bi = 0;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int i0 = bi * SYM_T, j0 = bj * SYM_T;
+  const int r = threadIdx.x & 63, c0 = threadIdx.x >> 6;
+  constexpr int PER = SYM_T * SYM_T / 256;
+  double lv[PER], uv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int c = c0 + 4 * k;
+    lv[k] = (i0 + r < n && j0 + c < n) ? M[(size_t)(i0 + r) + (size_t)(j0 + c) * n] : 0.0;
+    uv[k] = (j0 + r < n && i0 + c < n) ? M[(size_t)(j0 + r) + (size_t)(i0 + c) * n] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    L[r][c0 + 4 * k] = lv[k];
+    U[r][c0 + 4 * k] = uv[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int c = c0 + 4 * k;
+    const int li = i0 + r, lj = j0 + c;  // lower-tile entry (li, lj), mirror (lj, li) = U[c][r]
+    if (li < n && lj < n && li > lj)
+      M[(size_t)li + (size_t)lj * n] = 0.5 * (L[r][c] + U[c][r]);
+    const int ui = j0 + r, uj = i0 + c;  // upper-tile entry (ui, uj), mirror (uj, ui) = L[c][r]
+    if (ui < n && uj < n && uj > ui)
+      M[(size_t)ui + (size_t)uj * n] = 0.5 * (L[c][r] + U[r][c]);
+  }
 }
 
 // player at tournament position k in round r (m players, position 0 fixed)
@@ -265,7 +293,9 @@ hipError_t svd_jt_scatter(int nrows, int npar, const int *rows, const int *cols,
 }
 
 hipError_t svd_symmetrize(int n, double *M, hipStream_t st) {
-  hipLaunchKernelGGL(k_symmetrize, dim3(cdiv((long long)n * n, 256)), dim3(256), 0, st, n, M);
+  if (n <= 0) return hipSuccess;
+  const int nt = (n + SYM_T - 1) / SYM_T;
+  hipLaunchKernelGGL(k_symmetrize, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, n, M);
   return hipGetLastError();
 }
 
