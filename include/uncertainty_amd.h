@@ -219,8 +219,10 @@ int unc_scene_jacobian_device(unc_context *ctx, int cam_model, int num_cams, int
  *   unc_diag_gemm_f64     C = alpha A B + beta C with the FP64 MFMA GEMM;
  *                         *absmax_out = max|C| from its fused epilogue
  *   unc_diag_inverse      X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels,
- *                         plus the compensated Newton step when refine != 0
- *   unc_diag_residual_dd  R = I - A X with the compensated (Dot2) kernel
+ *                         plus the Newton step with an accurate residual when refine != 0
+ *   unc_diag_residual_dd  R = I - A X with the refinement's residual kernel
+ *                         (error-free extraction by default; compensated Dot2
+ *                         after unc_diag_set_gemm_variant(65))
  *   unc_diag_gemm_bench   timing of the GEMM alone */
 int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
                       const double *B, int ldb, double beta, double *C, int ldc,
@@ -255,17 +257,21 @@ int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
  * kernel (8 waves, K slab 16), 1-7 are tuning alternatives (5 is the 4-wave
  * kernel with K slab 32), 8 the split-K form of the production kernel. */
 int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
-/* Average duration (ms) of the n x n compensated residual R = I - A X over reps
- * launches; variant 0 is the production choice (64 x 16 tiles while the 64 x 32
- * grid would fill fewer than three rounds of resident workgroups, else 64 x 32),
- * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles. */
+/* Average duration (ms) of the n x n residual R = I - A X over reps launches;
+ * variant 0 is the production choice (error-free extraction against
+ * power-of-two bounds from the row maxima of A and the column maxima of X, on
+ * 64 x 16 tiles while the 64 x 32 grid would fill fewer than three rounds of
+ * resident workgroups, else 64 x 32); compensated dot products (Dot2) on
+ * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles chosen as for 0. */
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
 /* Select the GEMM kernel used by every later call in this process (tuning):
    0-3 kernel variants; 20: automatic and 21-24: 1-4 K chunks for the symmetric
    chain product; 30 / 31: that product on all tiles / on the lower tiles in
    XCD-contiguous blocks (the default); 40 / 41: the split product in id
    order / in XCD-contiguous blocks (the default); 50: the Schur reduction's
-   chunk size by launch size (the default), 51 / 52: 64 / 32 pairs per chunk. */
+   chunk size by launch size (the default), 51 / 52: 64 / 32 pairs per chunk;
+   60-65: the residual's form for unc_diag_residual_dd and the refinement
+   (the variants of unc_diag_residual_bench; 60 is the default). */
 int unc_diag_set_gemm_variant(int variant);
 
 const char *unc_error_string(int code);

@@ -101,17 +101,38 @@ def test_gemm_f64_split_k_declines_small_shapes():
     assert e.value.code == u.UNC_ERR_UNSUPPORTED
 
 
-def test_residual_dd_kernel():
-    say("residual_dd")
-    rng = np.random.default_rng(7)
-    n = 150
+def _residual_case(n, seed=7):
+    rng = np.random.default_rng(seed)
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     A = (Q * np.logspace(0, -8, n)) @ Q.T
     A = 0.5 * (A + A.T)
-    X = np.linalg.inv(A)
-    R = u.diag_residual_dd(A, X)
+    return A, np.linalg.inv(A)
+
+
+def test_residual_dd_kernel_compensated():
+    say("residual_dd (compensated)")
+    A, X = _residual_case(150)
+    R = u.diag_residual_dd(A, X, compensated=True)
     Ro = oracle.residual_dd(A, X)
     assert np.abs(R - Ro).max() <= 1e-13 * np.abs(Ro).max() + 1e-300
+
+
+@pytest.mark.parametrize("n", [1, 70, 150])
+def test_residual_dd_kernel_extraction(n):
+    """Error-free extraction form: the extracted parts sum exactly, the rest is
+    summed in FP64, so |R - R_exact| <= 17 n^3 eps^2 rowmax|A|_i colmax|X|_j
+    (+ 2 eps |R_ij| for the last two roundings).  Dropping the rest would leave
+    errors near eps n rowmax colmax, eleven orders above this bound at n = 150."""
+    say(f"residual_dd (extraction) n={n}")
+    A, X = _residual_case(n)
+    R = u.diag_residual_dd(A, X)
+    Ro = oracle.residual_dd(A, X)
+    eps = 2.0 ** -53
+    bound = (17.0 * n ** 3 * eps ** 2 * np.outer(np.abs(A).max(axis=1), np.abs(X).max(axis=0))
+             + 4 * eps * np.abs(Ro))
+    assert np.all(np.abs(R - Ro) <= bound)
+    # and in practice far inside it: 1e-11 of the largest entry
+    assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max() + 1e-300
 
 
 def test_inverse_kernel():

@@ -355,13 +355,22 @@ def diag_chol_inverse(A: np.ndarray):
     return np.tril(np.asarray(L)), np.asarray(X)
 
 
-def diag_residual_dd(A: np.ndarray, X: np.ndarray) -> np.ndarray:
+def diag_residual_dd(A: np.ndarray, X: np.ndarray, compensated: bool = False) -> np.ndarray:
+    """R = I - A X with the refinement's residual kernel (error-free extraction),
+    or with compensated dot products (Dot2) when `compensated`."""
     n = A.shape[0]
     Af = np.asfortranarray(A, dtype=np.float64)
     Xf = np.asfortranarray(X, dtype=np.float64)
     R = np.zeros((n, n), order="F")
-    _check(lib().unc_diag_residual_dd(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data),
-           "unc_diag_residual_dd")
+    L = lib()
+    if compensated:
+        _check(L.unc_diag_set_gemm_variant(65), "unc_diag_set_gemm_variant")
+    try:
+        _check(L.unc_diag_residual_dd(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data),
+               "unc_diag_residual_dd")
+    finally:
+        if compensated:
+            L.unc_diag_set_gemm_variant(60)
     return np.asarray(R)
 
 

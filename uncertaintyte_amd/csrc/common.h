@@ -173,9 +173,14 @@ hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const
                     const double *Nb, double *Xd, hipStream_t st);
 hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,
                           hipStream_t st);
-hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st);
-// 0: 64 x 16 or 64 x 32 tiles by size (default); 1: 64 x 64, 2: 64 x 48,
-// 3: 64 x 16, 4: 64 x 32 tiles
+// R = I - A X accurate to about eps |R|.  scratch: 2 n doubles for the
+// row/column maxima of the error-free extraction form, or nullptr for the
+// compensated dot products.
+hipError_t residual_dd(int n, const double *A, const double *X, double *R, double *scratch,
+                       hipStream_t st);
+// 0: extraction form when scratch is given, else compensated dot products on
+// 64 x 16 or 64 x 32 tiles by size (default); compensated dot products on
+// 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles by size
 void set_residual_variant(int v);
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st);

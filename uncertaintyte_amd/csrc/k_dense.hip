@@ -855,16 +855,153 @@ __global__ __launch_bounds__(256, QN == 4 ? 4 : QN == 3 ? 5 : QN == 2 ? 6 : 8) v
     }
 }
 
+// ---------------------------------------------------------------------------
+// R = I - A X with error-free extraction (Rump, Ogita, Oishi, "Accurate
+// floating-point summation part I", SIAM J. Sci. Comput. 31 (2008),
+// ExtractScalar, Lemma 3.3): for sigma = 2^k and |p| <= 2^-M sigma,
+// q = fl(fl(sigma + p) - sigma) is exact, q is a multiple of 2^-53 sigma and
+// p - q is exact.  With sigma_ij = 2^(e_i + f_j + M), where 2^e_i > max_k |A_ik|,
+// 2^f_j > max_k |X_kj| and 2^M >= n, every product p = fl(A_ik X_kj) qualifies,
+// and the n extracted parts q sum exactly (each partial sum is a multiple of
+// 2^-53 sigma no larger than sigma).  The rest, A_ik X_kj - q, comes from one
+// fma per term and is summed in plain FP64.  Six operations per term against
+// the ten of the compensated dot product, with a worst-case error of the same
+// order, n^3 eps^2 max|p|.
+// ---------------------------------------------------------------------------
+// mx[i] (i < n) = max_k |A_ik| as IEEE bits (zeroed first); one workgroup per
+// 64 rows x one of RM_CHUNKS column chunks
+constexpr int RM_CHUNKS = 16;
+__global__ __launch_bounds__(256) void k_row_absmax(int n, const double *__restrict__ A,
+                                                    unsigned long long *mx) {
+  __shared__ double part[4][64];
+  const int r = threadIdx.x & 63, c0 = threadIdx.x >> 6, i = blockIdx.x * 64 + r;
+  const int per = (n + RM_CHUNKS - 1) / RM_CHUNKS;
+  const int kb = blockIdx.y * per, ke = min(n, kb + per);
+  double m = 0.0;
+  if (i < n)
+    for (int k = kb + c0; k < ke; k += 4) m = fmax(m, fabs(A[(size_t)i + (size_t)k * n]));
+  part[c0][r] = m;
+  __syncthreads();
+  if (c0 == 0 && i < n) {
+    m = fmax(fmax(part[0][r], part[1][r]), fmax(part[2][r], part[3][r]));
+    atomicMax(&mx[i], (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// mx[j] = max_k |X_kj| as IEEE bits; one wave per column
+__global__ __launch_bounds__(256) void k_col_absmax(int n, const double *__restrict__ X,
+                                                    unsigned long long *mx) {
+  const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  double m = 0.0;
+  for (int k = lane; k < n; k += 64) m = fmax(m, fabs(X[(size_t)k + (size_t)j * n]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (lane == 0) mx[j] = (unsigned long long)__double_as_longlong(m);
+}
+
+template <int QN>
+__global__ __launch_bounds__(256, QN == 2 ? 6 : 8) void k_residual_ex(
+    int n, int M, const double *__restrict__ A, const double *__restrict__ X,
+    double *__restrict__ R, const unsigned long long *__restrict__ amax,
+    const unsigned long long *__restrict__ xmax) {
+  constexpr int RC = 16 * QN;
+  __shared__ double As[R_K][R_T + 1];
+  __shared__ double Xs[R_K][RC + 1];
+  const int tid = threadIdx.x;
+  const int tr = tid & 15, tc = tid >> 4;
+  const int i0 = blockIdx.x * R_T, j0 = blockIdx.y * RC;
+  double hi[4][QN], lo[4][QN], sg[4][QN];
+  int ea[4], ex[QN];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int gi = i0 + tr + 16 * p;
+    ea[p] = 0;
+    if (gi < n) (void)frexp(__longlong_as_double((long long)amax[gi]), &ea[p]);
+  }
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    const int gj = j0 + tc + 16 * q;
+    ex[q] = 0;
+    if (gj < n) (void)frexp(__longlong_as_double((long long)xmax[gj]), &ex[q]);
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      hi[p][q] = 0.0;
+      lo[p][q] = 0.0;
+      sg[p][q] = ldexp(1.0, ea[p] + ex[q] + M);
+    }
+  for (int k0 = 0; k0 < n; k0 += R_K) {
+    __syncthreads();
+    for (int x = tid; x < R_K * R_T; x += 256) {
+      const int m = x & 63, k = x >> 6;
+      const int gi = i0 + m, gk = k0 + k;
+      As[k][m] = (gi < n && gk < n) ? A[(size_t)gi + (size_t)gk * n] : 0.0;
+    }
+    for (int x = tid; x < R_K * RC; x += 256) {
+      const int kk = x & 15, jj = x >> 4;
+      const int gj = j0 + jj, gk = k0 + kk;
+      Xs[kk][jj] = (gj < n && gk < n) ? X[(size_t)gk + (size_t)gj * n] : 0.0;
+    }
+    __syncthreads();
+    // zeros past n give q = 0 and a zero remainder
+#pragma unroll
+    for (int k = 0; k < R_K; ++k) {
+      double av[4], xv[QN];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) av[p] = As[k][tr + 16 * p];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) xv[q] = Xs[k][tc + 16 * q];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+          const double P = av[p] * xv[q];
+          const double t = sg[p][q] + P;
+          const double e = t - sg[p][q];  // extracted part of P: exact
+          hi[p][q] = hi[p][q] + e;         // exact
+          lo[p][q] = lo[p][q] + fma(av[p], xv[q], -e);
+        }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int gi = i0 + tr + 16 * p, gj = j0 + tc + 16 * q;
+      if (gi < n && gj < n)
+        R[(size_t)gi + (size_t)gj * n] = (((gi == gj) ? 1.0 : 0.0) - hi[p][q]) - lo[p][q];
+    }
+}
+
 static int g_residual_variant = 0;
 void set_residual_variant(int v) { g_residual_variant = v; }
 
-hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipStream_t st) {
+hipError_t residual_dd(int n, const double *A, const double *X, double *R, double *scratch,
+                       hipStream_t st) {
   // 64 x 32 tiles at six workgroups per CU, unless they would need fewer than
   // three rounds of the 1536 resident slots: then the last round runs at low
   // occupancy for a large share of the time, and 64 x 16 tiles (eight per CU)
-  // balance better (N = 1800: 2.14 ms against 2.47 ms).
+  // balance better (N = 1800: 2.14 ms against 2.47 ms for the compensated form).
   const long long wg32 = (long long)cdiv(n, R_T) * cdiv(n, 32);
-  if (g_residual_variant == 0 && wg32 < 3 * 1536) {
+  if (scratch && g_residual_variant == 0) {  // error-free extraction
+    unsigned long long *amax = (unsigned long long *)scratch, *xmax = amax + n;
+    int M = 0;
+    while ((1ll << M) < (long long)n) ++M;
+    hipError_t e = hipMemsetAsync(amax, 0, sizeof(unsigned long long) * n, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_row_absmax, dim3(cdiv(n, 64), RM_CHUNKS), dim3(256), 0, st, n, A, amax);
+    hipLaunchKernelGGL(k_col_absmax, dim3(cdiv(n, 4)), dim3(256), 0, st, n, X, xmax);
+    if (wg32 < 3 * 1536)
+      hipLaunchKernelGGL(k_residual_ex<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n,
+                         M, A, X, R, amax, xmax);
+    else
+      hipLaunchKernelGGL(k_residual_ex<2>, dim3(cdiv(n, R_T), cdiv(n, 32)), dim3(256), 0, st, n,
+                         M, A, X, R, amax, xmax);
+    return hipGetLastError();
+  }
+  if ((g_residual_variant == 0 || g_residual_variant == 5) && wg32 < 3 * 1536) {
     hipLaunchKernelGGL(k_residual_dd<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n, A,
                        X, R);
     return hipGetLastError();
@@ -878,7 +1015,7 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, hipSt
   else if (g_residual_variant == 3)
     hipLaunchKernelGGL(k_residual_dd<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n, A,
                        X, R);
-  else  // 0 (large n) and 4: 64 x 32 tiles
+  else  // 0 and 5 (large n), 4: 64 x 32 tiles
     hipLaunchKernelGGL(k_residual_dd<2>, dim3(cdiv(n, R_T), cdiv(n, 32)), dim3(256), 0, st, n, A,
                        X, R);
   return hipGetLastError();

@@ -583,7 +583,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     // antisymmetric part of the size of X0's asymmetry, which mirroring would
     // fold into the result.)
     HC(svd_symmetrize(N, dB, s));
-    HC(residual_dd(N, dA, dB, dC, s));
+    HC(residual_dd(N, dA, dB, dC, dE, s));  // dE is free until X1 is formed
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
     HC(gemm_f64_sym(N, 1.0, dB, dC, dD, (double *)ptr(B_SPLIT), gemm_sym_split_work_doubles(N),
                     nullptr, nullptr, s));
@@ -1218,7 +1218,7 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
 }
 
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 4) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 5) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
@@ -1227,15 +1227,15 @@ int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
   hipStream_t s = ctx->stream;
   const size_t nn = (size_t)n * n;
   double *buf = nullptr;
-  if (hipMalloc((void **)&buf, sizeof(double) * 3 * nn) != hipSuccess) {
+  if (hipMalloc((void **)&buf, sizeof(double) * (3 * nn + 2 * (size_t)n)) != hipSuccess) {
     (void)hipGetLastError();
     return UNC_ERR_OOM;
   }
   hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
   set_residual_variant(variant);
-  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
   HC(hipEventRecord(ctx->gemm_ev[0], s));
-  for (int r = 0; r < reps; ++r) HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  for (int r = 0; r < reps; ++r) HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
   HC(hipEventRecord(ctx->gemm_ev[1], s));
   HC(hipEventSynchronize(ctx->gemm_ev[1]));
   *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
@@ -1255,6 +1255,10 @@ int unc_diag_set_gemm_variant(int variant) {
   }
   if (variant == 40 || variant == 41) {  // split product: id order / XCD-contiguous blocks
     set_split_blocked(variant - 40);
+    return UNC_OK;
+  }
+  if (variant >= 60 && variant <= 65) {  // residual R = I - A X: see set_residual_variant
+    set_residual_variant(variant - 60);
     return UNC_OK;
   }
   if (variant >= 50 && variant <= 52) {  // Schur reduction: by size / 64-pair / 32-pair chunks
@@ -1288,7 +1292,7 @@ int unc_diag_inverse(int n, const double *A, double *X, int refine) {
   if (rc) { (void)hipFree(buf); return rc; }
   double *res = dW;
   if (refine) {
-    HC(residual_dd(n, dA, dW, dR, s));
+    HC(residual_dd(n, dA, dW, dR, dD, s));  // dD is free until the product
     HC(gemm_f64(n, n, n, 1.0, dW, n, dR, n, 0.0, dD, n, nullptr, nullptr, s));
     HC(sym_update(n, dW, dD, dX, s));
     res = dX;
@@ -1375,10 +1379,10 @@ int unc_diag_residual_dd(int n, const double *A, const double *X, double *R) {
   hipStream_t s = ctx->stream;
   const size_t nn = (size_t)n * n;
   double *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * nn));
+  HC(hipMalloc((void **)&buf, sizeof(double) * (3 * nn + 2 * (size_t)n)));
   HC(hipMemcpyAsync(buf, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
   HC(hipMemcpyAsync(buf + nn, X, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, s));
+  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
   HC(hipMemcpyAsync(R, buf + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
   HC(hipFree(buf));
