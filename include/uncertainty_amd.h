@@ -262,7 +262,8 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
  * power-of-two bounds from the row maxima of A and the column maxima of X, on
  * 64 x 16 tiles while the 64 x 32 grid would fill fewer than three rounds of
  * resident workgroups, else 64 x 32); compensated dot products (Dot2) on
- * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles chosen as for 0. */
+ * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles chosen as for 0;
+ * extraction form on 6: 64 x 32, 7: 64 x 16 tiles. */
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
 /* Select the GEMM kernel used by every later call in this process (tuning):
    0-3 kernel variants; 20: automatic and 21-24: 1-4 K chunks for the symmetric
@@ -270,7 +271,7 @@ int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
    XCD-contiguous blocks (the default); 40 / 41: the split product in id
    order / in XCD-contiguous blocks (the default); 50: the Schur reduction's
    chunk size by launch size (the default), 51 / 52: 64 / 32 pairs per chunk;
-   60-65: the residual's form for unc_diag_residual_dd and the refinement
+   60-67: the residual's form for unc_diag_residual_dd and the refinement
    (the variants of unc_diag_residual_bench; 60 is the default). */
 int unc_diag_set_gemm_variant(int variant);
 

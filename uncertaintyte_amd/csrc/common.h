@@ -180,7 +180,8 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, doubl
                        hipStream_t st);
 // 0: extraction form when scratch is given, else compensated dot products on
 // 64 x 16 or 64 x 32 tiles by size (default); compensated dot products on
-// 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles by size
+// 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles by size;
+// extraction form on 6: 64 x 32, 7: 64 x 16 tiles
 void set_residual_variant(int v);
 hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
                             hipStream_t st);

@@ -985,7 +985,7 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, doubl
   // occupancy for a large share of the time, and 64 x 16 tiles (eight per CU)
   // balance better (N = 1800: 2.14 ms against 2.47 ms for the compensated form).
   const long long wg32 = (long long)cdiv(n, R_T) * cdiv(n, 32);
-  if (scratch && g_residual_variant == 0) {  // error-free extraction
+  if (scratch && (g_residual_variant == 0 || g_residual_variant >= 6)) {  // error-free extraction
     unsigned long long *amax = (unsigned long long *)scratch, *xmax = amax + n;
     int M = 0;
     while ((1ll << M) < (long long)n) ++M;
@@ -993,7 +993,7 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, doubl
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_row_absmax, dim3(cdiv(n, 64), RM_CHUNKS), dim3(256), 0, st, n, A, amax);
     hipLaunchKernelGGL(k_col_absmax, dim3(cdiv(n, 4)), dim3(256), 0, st, n, X, xmax);
-    if (wg32 < 3 * 1536)
+    if (g_residual_variant == 7 || (g_residual_variant == 0 && wg32 < 3 * 1536))
       hipLaunchKernelGGL(k_residual_ex<1>, dim3(cdiv(n, R_T), cdiv(n, 16)), dim3(256), 0, st, n,
                          M, A, X, R, amax, xmax);
     else

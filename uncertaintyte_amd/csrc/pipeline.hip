@@ -1218,7 +1218,7 @@ int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
 }
 
 int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 5) return UNC_ERR_BAD_INPUT;
+  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 7) return UNC_ERR_BAD_INPUT;
   int rc = UNC_OK;
   unc_context *ctx = default_context(0, &rc);
   if (!ctx) return rc;
@@ -1257,7 +1257,7 @@ int unc_diag_set_gemm_variant(int variant) {
     set_split_blocked(variant - 40);
     return UNC_OK;
   }
-  if (variant >= 60 && variant <= 65) {  // residual R = I - A X: see set_residual_variant
+  if (variant >= 60 && variant <= 67) {  // residual R = I - A X: see set_residual_variant
     set_residual_variant(variant - 60);
     return UNC_OK;
   }
