@@ -120,9 +120,10 @@ def test_residual_dd_kernel_compensated():
 @pytest.mark.parametrize("n", [1, 70, 150])
 def test_residual_dd_kernel_extraction(n):
     """Error-free extraction form: the extracted parts sum exactly, the rest is
-    summed in FP64, so |R - R_exact| <= 17 n^3 eps^2 rowmax|A|_i colmax|X|_j
-    (+ 2 eps |R_ij| for the last two roundings).  Dropping the rest would leave
-    errors near eps n rowmax colmax, eleven orders above this bound at n = 150."""
+    summed in FP64, so |R - R_exact| <= 16 n^3 eps^2 rowmax|A|_i colmax|X|_j
+    (+ 2 eps |R_ij| for the last two roundings; 17 below for slack).  Dropping
+    the rest would leave errors near eps n rowmax colmax, eleven orders above
+    this bound at n = 150."""
     say(f"residual_dd (extraction) n={n}")
     A, X = _residual_case(n)
     R = u.diag_residual_dd(A, X)

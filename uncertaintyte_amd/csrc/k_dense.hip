@@ -856,17 +856,17 @@ __global__ __launch_bounds__(256, QN == 4 ? 4 : QN == 3 ? 5 : QN == 2 ? 6 : 8) v
 }
 
 // ---------------------------------------------------------------------------
-// R = I - A X with error-free extraction (Rump, Ogita, Oishi, "Accurate
+// R = I - A X with error-free extraction (after Rump, Ogita, Oishi, "Accurate
 // floating-point summation part I", SIAM J. Sci. Comput. 31 (2008),
-// ExtractScalar, Lemma 3.3): for sigma = 2^k and |p| <= 2^-M sigma,
-// q = fl(fl(sigma + p) - sigma) is exact, q is a multiple of 2^-53 sigma and
-// p - q is exact.  With sigma_ij = 2^(e_i + f_j + M), where 2^e_i > max_k |A_ik|,
-// 2^f_j > max_k |X_kj| and 2^M >= n, every product p = fl(A_ik X_kj) qualifies,
-// and the n extracted parts q sum exactly (each partial sum is a multiple of
-// 2^-53 sigma no larger than sigma).  The rest, A_ik X_kj - q, comes from one
-// fma per term and is summed in plain FP64.  Six operations per term against
-// the ten of the compensated dot product, with a worst-case error of the same
-// order, n^3 eps^2 max|p|.
+// ExtractScalar): sigma_ij = 2^(e_i + f_j + M) with 2^e_i > max_k |A_ik|,
+// 2^f_j > max_k |X_kj| and 2^M >= 2n, so every product a x = A_ik X_kj has
+// |a x| <= sigma / (2n).  t = fma(a, x, sigma) then lies in [sigma/2, 2 sigma],
+// so q = t - sigma is exact (Sterbenz) and a multiple of 2^-53 sigma; the n
+// parts q sum exactly (every partial sum is such a multiple below sigma), and
+// the rest a x - q, at most 2^-53 sigma, comes from one more fma and is summed
+// in plain FP64.  Five operations per term against the ten of the compensated
+// dot product; the error is below 16 n^3 eps^2 max|A_i.| max|X_.j| + 2 eps |R_ij|,
+// the order of the compensated form's worst case.
 // ---------------------------------------------------------------------------
 // mx[i] (i < n) = max_k |A_ik| as IEEE bits (zeroed first); one workgroup per
 // 64 rows x one of RM_CHUNKS column chunks
@@ -957,9 +957,8 @@ __global__ __launch_bounds__(256, QN == 2 ? 6 : 8) void k_residual_ex(
       for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < QN; ++q) {
-          const double P = av[p] * xv[q];
-          const double t = sg[p][q] + P;
-          const double e = t - sg[p][q];  // extracted part of P: exact
+          const double t = fma(av[p], xv[q], sg[p][q]);
+          const double e = t - sg[p][q];  // extracted part of a x: exact
           hi[p][q] = hi[p][q] + e;         // exact
           lo[p][q] = lo[p][q] + fma(av[p], xv[q], -e);
         }
@@ -988,7 +987,7 @@ hipError_t residual_dd(int n, const double *A, const double *X, double *R, doubl
   if (scratch && (g_residual_variant == 0 || g_residual_variant >= 6)) {  // error-free extraction
     unsigned long long *amax = (unsigned long long *)scratch, *xmax = amax + n;
     int M = 0;
-    while ((1ll << M) < (long long)n) ++M;
+    while ((1ll << M) < 2ll * n) ++M;
     hipError_t e = hipMemsetAsync(amax, 0, sizeof(unsigned long long) * n, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_row_absmax, dim3(cdiv(n, 64), RM_CHUNKS), dim3(256), 0, st, n, A, amax);
