@@ -76,7 +76,7 @@ typedef struct unc_options {
   int pts2fix[3];     /* fixed point ids, -1 -1 -1 = none                             */
   int precision;      /* UNC_PREC_*                                                   */
   int refine;         /* Newton steps on the inverse; -1 = automatic (default):
-                         FP64 path 1 step with a compensated residual on
+                         FP64 path 1 step with an error-free residual on
                          (Z + lambda I)^-1; Z^+ modes 1 step on the null-space
                          basis N; 0 disables refinement                           */
   int device;         /* HIP device ordinal                                           */
