@@ -900,13 +900,15 @@ __global__ __launch_bounds__(256) void k_col_absmax(int n, const double *__restr
 }
 
 template <int QN>
-__global__ __launch_bounds__(256, QN == 2 ? 6 : 8) void k_residual_ex(
+__global__ __launch_bounds__(256, QN == 2 ? 5 : 8) void k_residual_ex(
     int n, int M, const double *__restrict__ A, const double *__restrict__ X,
     double *__restrict__ R, const unsigned long long *__restrict__ amax,
     const unsigned long long *__restrict__ xmax) {
   constexpr int RC = 16 * QN;
-  __shared__ double As[R_K][R_T + 1];
-  __shared__ double Xs[R_K][RC + 1];
+  // k-slab depth (two barriers per slab): 32 where the registers allow
+  constexpr int RX_K = QN == 1 ? 32 : 16;
+  __shared__ double As[RX_K][R_T + 1];
+  __shared__ double Xs[RX_K][RC + 1];
   const int tid = threadIdx.x;
   const int tr = tid & 15, tc = tid >> 4;
   const int i0 = blockIdx.x * R_T, j0 = blockIdx.y * RC;
@@ -932,22 +934,22 @@ __global__ __launch_bounds__(256, QN == 2 ? 6 : 8) void k_residual_ex(
       lo[p][q] = 0.0;
       sg[p][q] = ldexp(1.0, ea[p] + ex[q] + M);
     }
-  for (int k0 = 0; k0 < n; k0 += R_K) {
+  for (int k0 = 0; k0 < n; k0 += RX_K) {
     __syncthreads();
-    for (int x = tid; x < R_K * R_T; x += 256) {
+    for (int x = tid; x < RX_K * R_T; x += 256) {
       const int m = x & 63, k = x >> 6;
       const int gi = i0 + m, gk = k0 + k;
       As[k][m] = (gi < n && gk < n) ? A[(size_t)gi + (size_t)gk * n] : 0.0;
     }
-    for (int x = tid; x < R_K * RC; x += 256) {
-      const int kk = x & 15, jj = x >> 4;
+    for (int x = tid; x < RX_K * RC; x += 256) {
+      const int kk = x % RX_K, jj = x / RX_K;
       const int gj = j0 + jj, gk = k0 + kk;
       Xs[kk][jj] = (gj < n && gk < n) ? X[(size_t)gk + (size_t)gj * n] : 0.0;
     }
     __syncthreads();
     // zeros past n give q = 0 and a zero remainder
 #pragma unroll
-    for (int k = 0; k < R_K; ++k) {
+    for (int k = 0; k < RX_K; ++k) {
       double av[4], xv[QN];
 #pragma unroll
       for (int p = 0; p < 4; ++p) av[p] = As[k][tr + 16 * p];
