@@ -97,6 +97,10 @@ typedef struct unc_stats {
                            /* symmetric form that computes the lower tiles only         */
   int gemm_count;
   int num_obs_kept;        /* observations left after fixPts                          */
+                           /* time_norm_j and time_split_jj are 0: the column norms are
+                              the assembled blocks' diagonals and the blocks are
+                              written split; time_multiply_jj = camera blocks + W_s,
+                              time_inv_v = point blocks with their 3x3 inverses     */
   double stage_ms[12];     /* per device stage, see unc_stage_name()                  */
 } unc_stats;
 
@@ -108,7 +112,8 @@ typedef struct unc_debug {
   double *d;       /* N + 3*npc column scales                                     */
   double *Z;       /* N*N scaled Schur complement, before + lambda I              */
   double *X0;      /* N*N (Z + lambda I)^-1 as used by the chain                  */
-  double *X;       /* N*N Taylor result                                           */
+  double *X;       /* N*N Taylor result (FP64 path); on the Z^+ paths L^-1 of the
+                      Cholesky factor of Z + N0 N0^T (lower triangle)            */
 } unc_debug;
 
 typedef struct unc_context unc_context;
@@ -214,67 +219,6 @@ int unc_scene_jacobian_device(unc_context *ctx, int cam_model, int num_cams, int
                               const int *d_obs_cam, const int *d_obs_pt, int *d_rows,
                               int *d_cols, double *d_vals, void *stream);
 
-/* Kernel-level diagnostics on device 0 (host pointers, column-major); they
- * pin single kernels in the parity tests and have no reference counterpart.
- *   unc_diag_gemm_f64     C = alpha A B + beta C with the FP64 MFMA GEMM;
- *                         *absmax_out = max|C| from its fused epilogue
- *   unc_diag_inverse      X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels,
- *                         plus the Newton step with an accurate residual when refine != 0
- *   unc_diag_residual_dd  R = I - A X with the refinement's residual kernel
- *                         (error-free extraction by default; compensated Dot2
- *                         after unc_diag_set_gemm_variant(65))
- *   unc_diag_gemm_bench   timing of the GEMM alone */
-int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda,
-                      const double *B, int ldb, double beta, double *C, int ldc,
-                      double *absmax_out);
-/* C = alpha A B by the split-K form of the same GEMM (two workgroups per tile,
-   each over half of K, then an epilogue kernel that adds the halves and folds
-   max|C|); UNC_ERR_UNSUPPORTED where the library would not split this shape
-   (128 x 128 tiles between 96 and 256, K >= 1024). */
-int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
-                            const double *B, int ldb, double *C, int ldc, double *absmax_out);
-/* C = alpha A B for n x n column-major A, B whose product is symmetric (powers
-   of one symmetric matrix, as in the Taylor chain): only the 128 x 128 tiles on
-   and below the diagonal are computed, in three or four K chunks, and the lower triangle
-   is mirrored, so C is exactly symmetric; *absmax_out = max|C|.
-   UNC_ERR_UNSUPPORTED where the library would not use this form (same shapes
-   as unc_diag_gemm_f64_split). */
-int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
-                          double *absmax_out);
-int unc_diag_inverse(int n, const double *A, double *X, int refine);
-int unc_diag_residual_dd(int n, const double *A, const double *X, double *R);
-/* Cholesky A = L L^T of SPD A (n x n, column-major) and X = L^-1 by the recursive
-   kernels of the Z^+ path; L's upper triangle above the 64 x 64 diagonal blocks is
-   left as A's (only the lower triangle is the factor), X's upper triangle is 0. */
-int unc_diag_chol_inverse(int n, const double *A, double *L, double *X);
-/* average time of one 64 x 64 leaf kernel: 0 = panelled Cholesky + L^-1
-   (production, plus a 32 KB copy of its input), 1 = compact and 2 = unrolled
-   register forms of the same (plus the copy), 3 = SPD inverse by Cholesky
-   (the Gauss-Jordan panel inverse), 4 = in-register Gauss-Jordan panel inverse */
-int unc_diag_leaf_bench(int reps, int variant, double *avg_us);
-/* Average duration (ms) of the n x n x n FP64 GEMM over reps launches on
- * device-resident operands (throughput probe); variant 0 is the production
- * kernel (8 waves, K slab 16), 1-7 are tuning alternatives (5 is the 4-wave
- * kernel with K slab 32), 8 the split-K form of the production kernel. */
-int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms);
-/* Average duration (ms) of the n x n residual R = I - A X over reps launches;
- * variant 0 is the production choice (error-free extraction against
- * power-of-two bounds from the row maxima of A and the column maxima of X, on
- * 64 x 16 tiles while the 64 x 32 grid would fill fewer than three rounds of
- * resident workgroups, else 64 x 32); compensated dot products (Dot2) on
- * 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles chosen as for 0;
- * extraction form on 6: 64 x 32, 7: 64 x 16 tiles. */
-int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms);
-/* Select the GEMM kernel used by every later call in this process (tuning):
-   0-3 kernel variants; 20: automatic and 21-24: 1-4 K chunks for the symmetric
-   chain product; 30 / 31: that product on all tiles / on the lower tiles in
-   XCD-contiguous blocks (the default); 40 / 41: the split product in id
-   order / in XCD-contiguous blocks (the default); 50: the Schur reduction's
-   chunk size by launch size (the default), 51 / 52: 64 / 32 pairs per chunk;
-   60-67: the residual's form for unc_diag_residual_dd and the refinement
-   (the variants of unc_diag_residual_bench; 60 is the default). */
-int unc_diag_set_gemm_variant(int variant);
-
 const char *unc_error_string(int code);
 const char *unc_stage_name(int stage);
 int unc_abi_version(void);
@@ -312,7 +256,10 @@ int unc_server_submit(unc_server *srv, const unc_options *opts, int num_rows, in
                       double *cam_unc, double *pt_unc, unc_stats *stats,
                       long long *ticket);
 /* blocks until the ticket's request finished; status = its unc_compute code,
- * latency = submit to finish, queue = submit to start (milliseconds) */
+ * latency = submit to finish, queue = submit to start (milliseconds).  Each
+ * ticket is claimed by one wait: a second wait on it (also while the first
+ * is still waiting) returns UNC_ERR_BAD_INPUT.  unc_server_destroy must not
+ * run while a wait is in progress. */
 int unc_server_wait(unc_server *srv, long long ticket, int *status, double *latency_ms,
                     double *queue_ms);
 int unc_server_query(unc_server *srv, unc_server_stats *out);

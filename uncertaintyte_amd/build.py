@@ -3,9 +3,11 @@
     python -m uncertaintyte_amd.build          # -> uncertaintyte_amd/_build/
 
 Produces
-  _build/libuncertainty_amd.so   C ABI (include/uncertainty_amd.h) + the C++
-                                 getCovariances (include/uncertainty.h)
-  _build/uncertainty             CLI with the reference's flags
+  _build/libuncertainty_amd.so       C ABI (include/uncertainty_amd.h) + the C++
+                                     getCovariances (include/uncertainty.h)
+  _build/libuncertainty_amd_diag.so  the same kernels plus the kernel-level test
+                                     hooks of csrc/diag.h (tests only)
+  _build/uncertainty                 CLI with the reference's flags
 """
 from __future__ import annotations
 
@@ -27,6 +29,8 @@ HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_
 PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
                   "k_zplus.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CPP_SOURCES = ["uncertainty_api.cpp", "uncertainty_te.cpp", "scene_io.cpp", "server.cpp"]
+DIAG_SOURCES = ["diag.hip"]
+DIAG_LIB = os.path.join(OUT, "libuncertainty_amd_diag.so")
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")
 
@@ -57,15 +61,16 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     hipcc = _hipcc()
     t0 = time.time()
     objs = []
+    diag_objs = []
     procs = []
-    for src in HIP_SOURCES + CPP_SOURCES:
+    for src in HIP_SOURCES + CPP_SOURCES + DIAG_SOURCES:
         obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [hipcc] + HIPCC_FLAGS + PER_FILE_FLAGS.get(src, []) + lang + \
             ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                                             text=True)))
-        objs.append(obj)
+        (diag_objs if src in DIAG_SOURCES else objs).append(obj)
         while len([p for _, p in procs if p.poll() is None]) >= jobs:
             time.sleep(0.05)
     failed = False
@@ -81,6 +86,9 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    tmp = DIAG_LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + diag_objs)
+    os.replace(tmp, DIAG_LIB)
     _run([hipcc, "-O2", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "cli.cpp"),
           "-o", CLI, "-L", OUT, "-luncertainty_amd", "-Wl,-rpath,$ORIGIN"])

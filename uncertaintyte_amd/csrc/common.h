@@ -16,8 +16,8 @@ enum : unsigned int {
   ERRBIT_ZERO_COLUMN = 4u,   // a Jacobian column is all zero (d = 1/0)
   ERRBIT_SINGULAR_V = 8u,    // a 3x3 point block has no finite inverse
   ERRBIT_GJ_PIVOT = 32u,     // non-positive pivot in the dense inverse
-  ERRBIT_SCHUR_OVERFLOW = 64u,  // a staged Schur batch exceeded its pair buffer
   ERRBIT_PROJECTION = 128u,     // a point lies in a camera's focal plane (depth 0)
+  ERRBIT_DUPLICATE = 256u,      // a camera observes the same point twice
 };
 
 // Control block of the Taylor chain, resident in device memory so the whole
@@ -55,8 +55,8 @@ struct Structure {
   // outputs of decode
   int *obs_cam;                     // camera per observation, -1 when dropped
   int *obs_pt;                      // compact point per observation, -1 when dropped
-  int *cam_off, *cam_list;          // camera-grouped observations (row order)
-  int *pt_cnt, *pt_off, *pt_list;   // point-grouped observations (sorted by index)
+  int *cam_off, *cam_list;          // camera-grouped observations (point order)
+  int *pt_cnt, *pt_off, *pt_list;   // point-grouped observations (row order)
   int *pt_cursor;
   unsigned int *err;                // device error word
 };
@@ -66,34 +66,29 @@ struct Blocks {
   double *d;      // N + 3*npc
   double *Vi;     // npc*9
   double *Ws;     // nobs*cp*3
-  double *Y;      // nobs*cp*3
   int *pt_has;    // npc
 };
 
 hipError_t launch_decode(const Structure &s, hipStream_t st);
 hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st);
 size_t camera_sort_temp_bytes(int nobs, int nc);
-hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in, int *keys_out,
-                              int *vals_in, int *cam_list, int *cam_off, void *temp,
-                              size_t temp_bytes, hipStream_t st);
-hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st);
-// Schur complement by sorted camera-pair lists (k_schur.hip, k_sort.hip)
-hipError_t launch_pair_count(const Structure &s, int *cnt, hipStream_t st);
-hipError_t launch_cam_pair_start(const Structure &s, const int *pofs, const int *cnt, int *cps,
-                                 hipStream_t st);
-hipError_t launch_pair_emit(const Structure &s, int t0, int t1, int a0, int base, const int *pofs,
-                            unsigned int *keys, unsigned long long *vals, hipStream_t st);
-hipError_t launch_schur_reduce(const Structure &s, const Blocks &b, double *Z, int a0,
-                               const unsigned int *ukeys, const int *counts, const int *offs,
-                               const int *nruns, const unsigned long long *pairs, long long npairs,
-                               hipStream_t st);
-size_t pair_sort_temp_bytes(int n, int key_bits);
-hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out,
-                     const unsigned long long *vals_in, unsigned long long *vals_out, int n,
-                     int key_bits, void *temp, size_t temp_bytes, hipStream_t st);
-size_t runs_temp_bytes(int n);
-hipError_t runs(const unsigned int *keys, int n, unsigned int *unique, int *counts, int *nruns,
-                void *temp, size_t temp_bytes, hipStream_t st);
+// cam_list / cam_off: the kept observations grouped by camera, each camera's in
+// point order (a stable sort by camera of the point-grouped list).  Also
+// pt_cams[u] = camera of the u-th point-grouped observation and cam_pts[t] =
+// point of the t-th camera-grouped one.
+hipError_t sort_obs_by_camera(const Structure &s, int *pt_cams, int *cam_pts, int *keys_in,
+                              int *keys_out, int *vals_in, void *temp, size_t temp_bytes,
+                              hipStream_t st);
+// assembly in three launches: camera blocks U_s and d_c; point blocks V_s,
+// d_p and V_s^-1; per-observation W_s
+hipError_t launch_cam_blocks(const Structure &s, const Blocks &b, hipStream_t st);
+hipError_t launch_point_blocks(const Structure &s, const Blocks &b, hipStream_t st);
+hipError_t launch_obs_blocks(const Structure &s, const Blocks &b, hipStream_t st);
+// Schur complement (k_schur.hip): cam_pts / pt_cams from sort_obs_by_camera;
+// ghash holds 4 ints per observation (the global tables of cameras with more
+// than 4096 observations)
+hipError_t launch_schur(const Structure &s, const Blocks &b, double *Z, const int *cam_pts,
+                        const int *pt_cams, int *ghash, hipStream_t st);
 size_t scan_temp_bytes(int n);
 hipError_t exclusive_scan_int(const int *in, int *out, int n, void *temp, size_t temp_bytes,
                               hipStream_t st);
@@ -107,7 +102,6 @@ hipError_t launch_point_output(const Structure &s, const Blocks &b, const double
 hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, double beta, double *C, int ldc, unsigned long long *absmax,
                     const int *active, hipStream_t st);
-void set_gemm_variant(int v);
 // C = alpha A B with the K range split over two workgroups per tile where that
 // fills idle CUs (gemm_split_applies); work: M x N doubles of scratch, or
 // nullptr for the unsplit kernel.  Same absmax / active contract as gemm_f64.
@@ -117,13 +111,11 @@ bool gemm_split_applies(int M, int N, int K);
 // diagonal are computed, over three or four K chunks where gemm_split_applies (work:
 // gemm_sym_split_work_doubles(n) doubles), and the lower triangle is mirrored;
 // otherwise the plain product.  Same absmax / active contract as gemm_f64.
+// parts: K chunks (1..4), 0 = by the launch-size rule.
 size_t gemm_sym_split_work_doubles(int n);
-void set_sym_parts(int p);  // tuning: K chunks of the symmetric form (0: automatic, at most 4)
-void set_sym_list(int on);  // tuning: 1 (default) lower tiles only in XCD-contiguous blocks
-void set_split_blocked(int on);  // tuning: 1 (default) split product in XCD-contiguous blocks
 hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                         double *work, size_t work_doubles, unsigned long long *absmax,
-                        const int *active, hipStream_t st);
+                        const int *active, hipStream_t st, int parts = 0);
 hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
                           const double *B, int ldb, double *C, int ldc, double *work,
                           unsigned long long *absmax, const int *active, hipStream_t st);
@@ -144,7 +136,6 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // gauge-fixed Z^+ path (k_zplus.hip)
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st);
-void set_leaf_variant(int v);  // 0: panelled, 256 threads (default), 1: compact, 2: unrolled registers
 // P = W^-1 of a b x b SPD block (b <= 64) by Cholesky, L^-1 and X^T X (k_zplus.hip)
 hipError_t spd_inverse64(int b, const double *W, int ldw, double *P, unsigned int *err,
                          hipStream_t st);
@@ -173,27 +164,22 @@ hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const
                     const double *Nb, double *Xd, hipStream_t st);
 hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,
                           hipStream_t st);
-// R = I - A X accurate to about eps |R|.  scratch: 2 n doubles for the
-// row/column maxima of the error-free extraction form, or nullptr for the
-// compensated dot products.
-hipError_t residual_dd(int n, const double *A, const double *X, double *R, double *scratch,
-                       hipStream_t st);
-// 0: extraction form when scratch is given, else compensated dot products on
-// 64 x 16 or 64 x 32 tiles by size (default); compensated dot products on
-// 1: 64 x 64, 2: 64 x 48, 3: 64 x 16, 4: 64 x 32 tiles, 5: tiles by size;
-// extraction form on 6: 64 x 32, 7: 64 x 16 tiles
-void set_residual_variant(int v);
-hipError_t gj_small_inverse(int b, const double *W, int ldw, double *P, unsigned int *err,
-                            hipStream_t st);
+// R = I - A X accurate to about eps |R| by error-free extraction.  scratch:
+// 2 n doubles for the row/column maxima.  tile_cols 16 or 32 forces 64 x 16 or
+// 64 x 32 tiles; anything else chooses by the grid's fill of the device.
+hipError_t residual_ex(int n, const double *A, const double *X, double *R, double *scratch,
+                       hipStream_t st, int tile_cols = 0);
 hipError_t copy_block(int m, int n, const double *src, int lds, double *dst, int ldd, double scale,
                       hipStream_t st);
 hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st);
 hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st);
 // out = X0 + D when both are exactly symmetric (the same value as sym_update)
 hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st);
-void set_schur_variant(int v);
+hipError_t te_init(TeCtrl *ctrl, hipStream_t st);
 hipError_t te_step(int n, TeCtrl *ctrl, int i, double k, double *X, const double *T,
                    hipStream_t st);
+// A[0, n) = 0 (A 16-byte aligned)
+hipError_t fill_zero(double *A, size_t n, hipStream_t st);
 hipError_t absmax_f64(size_t n, const double *A, unsigned long long *out, hipStream_t st);
 // SVD algorithm modes (k_svd.hip)
 hipError_t svd_jt_scatter(int nrows, int npar, const int *rows, const int *cols,
@@ -218,6 +204,16 @@ hipError_t launch_eig_cameras(const double *cam, int nc, int cp, double *vals, d
                               hipStream_t st);
 hipError_t launch_eig_points(const double *pts, long long np, double *vals, double *vecs,
                              hipStream_t st);
+
+// pipeline algorithms (pipeline.hip) for the diagnostics library: blocked
+// Gauss-Jordan inverse of SPD W in place (P3: 3 x 64 x 64, T: n x 64, R: 64 x n
+// doubles of scratch); recursive Cholesky A = L L^T with X = L^-1 (W: scratch
+// of (n/2)^2 doubles); the C ABI code of a device error word.  They return
+// UNC_* codes.
+int dense_inverse_gj(int n, double *W, double *P3, double *T, double *R, unsigned int *err,
+                     hipStream_t s);
+int cholesky_inverse(int n, double *A, double *X, double *W, unsigned int *err, hipStream_t s);
+int error_code(unsigned int device_error_bits);
 
 // 1/sqrt(x) and 1/x for normal, finite x: the hardware estimates refined by two
 // Newton steps each (quadratic convergence from the estimate's >= 20 bits),

@@ -8,9 +8,9 @@
 //   3x3 cofactor inverse              ScaledSparseMatrix.cpp:210-230
 //
 // Arithmetic contract (built with -ffp-contract=off): each block entry is a
-// sequential chain of explicit fma() over the group's observations in row
-// order, exactly the order of the CPU oracle, so U_s, d, V_s^-1, W_s and Y are
-// bit-identical to it.  Scaling is applied to the assembled blocks
+// sequential chain of explicit fma() over the group's observations -- a
+// camera's in point order, a point's in row order -- exactly the order of the
+// CPU oracle, so U_s, d, V_s^-1 and W_s are bit-identical to it.  Scaling is applied to the assembled blocks
 // (M_s = D M D) rather than to J, so J is read once.
 #include "common.h"
 
@@ -66,10 +66,7 @@ __global__ void k_decode(Structure s) {
 __global__ void k_init_lists(Structure s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < s.npc) { s.pt_cnt[i] = 0; s.pt_cursor[i] = 0; }
-  if (i == 0) {
-    s.err[0] = 0u;
-    s.err[1] = 0u;  // largest co-visibility count (filled by k_covis)
-  }
+  if (i == 0) s.err[0] = 0u;
 }
 
 hipError_t launch_decode(const Structure &s, hipStream_t st) {
@@ -182,7 +179,8 @@ hipError_t launch_point_lists(const Structure &s, int *scan_tmp, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// camera blocks: one wave per camera, lane = upper-triangle entry (i, j)
+// camera blocks: one wave per camera, lane = upper-triangle entry (i, j), the
+// camera's observations in point order
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_cam_assemble(Structure s, Blocks b) {
   // 64 observations at a time are staged in LDS (lane t loads observation t's
@@ -261,8 +259,8 @@ __device__ inline void inv3x3_ref(const double v[9], double o[9]) {
 // point blocks, two passes:
 //  * one thread per compact point: V_p over its observations in list order,
 //    d_p, V_s^-1;
-//  * one thread per observation in row order (coalesced J rows, W_s and Y
-//    streamed out): W_s = D_c (J_c^T J_p) D_p and Y = W_s V_s^-1.
+//  * one thread per observation in row order (coalesced J rows, W_s streamed
+//    out): W_s = D_c (J_c^T J_p) D_p.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(128) void k_pt_blocks(Structure s, Blocks b) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -311,19 +309,20 @@ __global__ __launch_bounds__(128) void k_pt_blocks(Structure s, Blocks b) {
   b.pt_has[p] = 1;
 }
 
-// Per observation: W_s = D_c J_c^T J_p D_p (CP x 3) and Y = W_s V_s^-1.  A
-// workgroup takes 64 consecutive observations: their Jacobian rows (one
-// contiguous run) are staged through LDS with every load issued up front, one
-// thread per observation forms its blocks into LDS, and the two contiguous
-// output runs are written back coalesced (observations dropped by fixPts keep
-// their old contents, as before).
+// Per observation: W_s = D_c J_c^T J_p D_p (CP x 3).  (Y = W_s V_s^-1 is not
+// stored: the Schur kernel forms it where it is used.)  A workgroup takes 64
+// consecutive observations: their Jacobian rows (one contiguous run) are
+// staged through LDS with every load issued up front, one thread per
+// observation forms its block into LDS, and the contiguous output run is
+// written back coalesced (observations dropped by fixPts keep their old
+// contents).
 constexpr int OB_OBS = 64, OB_THREADS = 128;
 
 template <int CP>
 __global__ __launch_bounds__(OB_THREADS) void k_obs_blocks(Structure s, Blocks b) {
   constexpr int RL = CP + 3, IN = OB_OBS * 2 * RL, OUT = OB_OBS * CP * 3;
   constexpr int NIN = (IN + OB_THREADS - 1) / OB_THREADS;
-  __shared__ double jin[IN], wout[OUT], yout[OUT];
+  __shared__ double jin[IN], wout[OUT];
   __shared__ int valid[OB_OBS];
   const int o0 = blockIdx.x * OB_OBS, tid = threadIdx.x;
   const int nob = min(OB_OBS, s.nobs - o0);
@@ -352,47 +351,30 @@ __global__ __launch_bounds__(OB_THREADS) void k_obs_blocks(Structure s, Blocks b
       const double *r1 = r0 + RL;
       const double *dc = b.d + a * CP;
       const double *dpp = b.d + s.N + 3 * p;
-      const double *ivp = b.Vi + (size_t)p * 9;
-      double dd[3], iv[9], jp0[3], jp1[3];
+      double dd[3], jp0[3], jp1[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         dd[k] = dpp[k];
         jp0[k] = r0[CP + k];
         jp1[k] = r1[CP + k];
       }
-#pragma unroll
-      for (int k = 0; k < 9; ++k) iv[k] = ivp[k];
       double *Ws = wout + tid * CP * 3;
-      double *Y = yout + tid * CP * 3;
 #pragma unroll
       for (int i = 0; i < CP; ++i) {
         const double ci0 = r0[i], ci1 = r1[i], dci = dc[i];
-        double w[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           const double wv = fma(ci1, jp1[k], ci0 * jp0[k]);
-          w[k] = (wv * dci) * dd[k];
-          Ws[i * 3 + k] = w[k];
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          double y = w[0] * iv[0 * 3 + m];
-          y = fma(w[1], iv[1 * 3 + m], y);
-          y = fma(w[2], iv[2 * 3 + m], y);
-          Y[i * 3 + m] = y;
+          Ws[i * 3 + k] = (wv * dci) * dd[k];
         }
       }
     }
   }
   __syncthreads();
-  double *Wg = b.Ws + (size_t)o0 * CP * 3, *Yg = b.Y + (size_t)o0 * CP * 3;
+  double *Wg = b.Ws + (size_t)o0 * CP * 3;
   const int nout = nob * CP * 3;
-  for (int x = tid; x < nout; x += OB_THREADS) {
-    if (valid[x / (CP * 3)]) {
-      Wg[x] = wout[x];
-      Yg[x] = yout[x];
-    }
-  }
+  for (int x = tid; x < nout; x += OB_THREADS)
+    if (valid[x / (CP * 3)]) Wg[x] = wout[x];
 }
 
 template <int CP>
@@ -400,9 +382,17 @@ static void launch_obs_blocks_cp(const Structure &s, const Blocks &b, hipStream_
   hipLaunchKernelGGL(k_obs_blocks<CP>, dim3(cdiv(s.nobs, OB_OBS)), dim3(OB_THREADS), 0, st, s, b);
 }
 
-hipError_t launch_assemble(const Structure &s, const Blocks &b, hipStream_t st) {
+hipError_t launch_cam_blocks(const Structure &s, const Blocks &b, hipStream_t st) {
   hipLaunchKernelGGL(k_cam_assemble, dim3(s.nc), dim3(64), 0, st, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_blocks(const Structure &s, const Blocks &b, hipStream_t st) {
   if (s.npc > 0) hipLaunchKernelGGL(k_pt_blocks, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_obs_blocks(const Structure &s, const Blocks &b, hipStream_t st) {
   if (s.nobs > 0) {
     switch (s.cp) {
       case 1: launch_obs_blocks_cp<1>(s, b, st); break;

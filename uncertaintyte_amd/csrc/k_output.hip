@@ -57,62 +57,51 @@ hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, 
   return hipGetLastError();
 }
 
-// per observation (row order): the point step's 3x3 contribution
-// M_o = W_o^T X_aa W_o (W_o = W_s of the observation, X_aa its camera's block),
-// written over Y (no longer needed after the Schur complement)
+// per point: G = sum over its observations in list order of
+// M_o = W_o^T X_aa W_o (W_o = W_s of the observation, X_aa its camera's block
+// of X, compact and row-major), then Sigma_p = D_p (V^-1 + V^-1 G V^-1) D_p
+// at the original point id.  One thread per point; X blocks come from L2.
 template <int CP>
-__global__ __launch_bounds__(256) void k_point_obs(Structure s, Blocks b,
-                                                   const double *__restrict__ Xd) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= s.nobs) return;
-  const int a = s.obs_cam[o];
-  if (a < 0 || s.obs_pt[o] < 0) return;
-  const double *Ws = b.Ws + (size_t)o * CP * 3;
-  const double *Xa = Xd + (size_t)a * CP * CP;  // row-major block
-  double w[CP * 3];
-#pragma unroll
-  for (int q = 0; q < CP * 3; ++q) w[q] = Ws[q];
-  double H[CP * 3];
-#pragma unroll
-  for (int i = 0; i < CP; ++i) {
-    const double x0 = Xa[i * CP];
-    double h0 = x0 * w[0], h1 = x0 * w[1], h2 = x0 * w[2];
-#pragma unroll
-    for (int j = 1; j < CP; ++j) {
-      const double x = Xa[i * CP + j];
-      h0 = fma(x, w[j * 3 + 0], h0);
-      h1 = fma(x, w[j * 3 + 1], h1);
-      h2 = fma(x, w[j * 3 + 2], h2);
-    }
-    H[i * 3 + 0] = h0;
-    H[i * 3 + 1] = h1;
-    H[i * 3 + 2] = h2;
-  }
-  double *M = b.Y + (size_t)o * (CP * 3 > 9 ? CP * 3 : 9);
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      double m = w[k] * H[l];
-#pragma unroll
-      for (int i = 1; i < CP; ++i) m = fma(w[i * 3 + k], H[i * 3 + l], m);
-      M[k * 3 + l] = m;
-    }
-}
-
-// per point: G = sum of its observations' M in list order, then
-// Sigma_p = D_p (V^-1 + V^-1 G V^-1) D_p at the original point id
-__global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
-                                                      double *__restrict__ out, int npts_out) {
+__global__ __launch_bounds__(256) void k_point_cov(Structure s, Blocks b,
+                                                   const double *__restrict__ Xd,
+                                                   double *__restrict__ out, int npts_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= s.npc || !b.pt_has[p]) return;
-  const int cp = s.cp, N = s.N;
-  const int stride = cp * 3 > 9 ? cp * 3 : 9;
+  const int N = s.N;
   double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = s.pt_off[p]; t < s.pt_off[p + 1]; ++t) {
-    const double *M = b.Y + (size_t)s.pt_list[t] * stride;
+    const int o = s.pt_list[t];
+    const int a = s.obs_cam[o];
+    const double *Ws = b.Ws + (size_t)o * CP * 3;
+    const double *Xa = Xd + (size_t)a * CP * CP;  // row-major block
+    double w[CP * 3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) G[k] = G[k] + M[k];
+    for (int q = 0; q < CP * 3; ++q) w[q] = Ws[q];
+    double H[CP * 3];
+#pragma unroll
+    for (int i = 0; i < CP; ++i) {
+      const double x0 = Xa[i * CP];
+      double h0 = x0 * w[0], h1 = x0 * w[1], h2 = x0 * w[2];
+#pragma unroll
+      for (int j = 1; j < CP; ++j) {
+        const double x = Xa[i * CP + j];
+        h0 = fma(x, w[j * 3 + 0], h0);
+        h1 = fma(x, w[j * 3 + 1], h1);
+        h2 = fma(x, w[j * 3 + 2], h2);
+      }
+      H[i * 3 + 0] = h0;
+      H[i * 3 + 1] = h1;
+      H[i * 3 + 2] = h2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        double m = w[k] * H[l];
+#pragma unroll
+        for (int i = 1; i < CP; ++i) m = fma(w[i * 3 + k], H[i * 3 + l], m);
+        G[k * 3 + l] = G[k * 3 + l] + m;
+      }
   }
   const double *Vi = b.Vi + (size_t)p * 9;
   double v[9];
@@ -147,31 +136,27 @@ __global__ __launch_bounds__(128) void k_point_output(Structure s, Blocks b,
 }
 
 template <int CP>
-static void launch_point_obs_cp(const Structure &s, const Blocks &b, const double *Xd,
-                                hipStream_t st) {
-  hipLaunchKernelGGL(k_point_obs<CP>, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s, b, Xd);
+static void launch_point_cov_cp(const Structure &s, const Blocks &b, const double *Xd,
+                                double *pt_out, int npts_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_point_cov<CP>, dim3(cdiv(s.npc, 256)), dim3(256), 0, st, s, b, Xd, pt_out,
+                     npts_out);
 }
 
-// Y must hold max(cp * 3, 9) doubles per observation
 hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *Xd,
                                double *pt_out, int npts_out, hipStream_t st) {
   if (s.npc <= 0) return hipSuccess;
-  if (s.nobs > 0) {
-    switch (s.cp) {
-      case 1: launch_point_obs_cp<1>(s, b, Xd, st); break;
-      case 2: launch_point_obs_cp<2>(s, b, Xd, st); break;
-      case 3: launch_point_obs_cp<3>(s, b, Xd, st); break;
-      case 4: launch_point_obs_cp<4>(s, b, Xd, st); break;
-      case 5: launch_point_obs_cp<5>(s, b, Xd, st); break;
-      case 6: launch_point_obs_cp<6>(s, b, Xd, st); break;
-      case 7: launch_point_obs_cp<7>(s, b, Xd, st); break;
-      case 8: launch_point_obs_cp<8>(s, b, Xd, st); break;
-      case 9: launch_point_obs_cp<9>(s, b, Xd, st); break;
-      default: return hipErrorInvalidValue;
-    }
+  switch (s.cp) {
+    case 1: launch_point_cov_cp<1>(s, b, Xd, pt_out, npts_out, st); break;
+    case 2: launch_point_cov_cp<2>(s, b, Xd, pt_out, npts_out, st); break;
+    case 3: launch_point_cov_cp<3>(s, b, Xd, pt_out, npts_out, st); break;
+    case 4: launch_point_cov_cp<4>(s, b, Xd, pt_out, npts_out, st); break;
+    case 5: launch_point_cov_cp<5>(s, b, Xd, pt_out, npts_out, st); break;
+    case 6: launch_point_cov_cp<6>(s, b, Xd, pt_out, npts_out, st); break;
+    case 7: launch_point_cov_cp<7>(s, b, Xd, pt_out, npts_out, st); break;
+    case 8: launch_point_cov_cp<8>(s, b, Xd, pt_out, npts_out, st); break;
+    case 9: launch_point_cov_cp<9>(s, b, Xd, pt_out, npts_out, st); break;
+    default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(k_point_output, dim3(cdiv(s.npc, 128)), dim3(128), 0, st, s, b, pt_out,
-                     npts_out);
   return hipGetLastError();
 }
 

@@ -1,27 +1,39 @@
-// k_sort.hip -- stable grouping of observations by camera.
+// k_sort.hip -- grouping of observations by camera.
 //
 // The reference accepts any row order (its sparse products do not care), so
 // camera a's observations need not be contiguous (04_Daliborka.jacob is
-// point-major).  A stable LSD radix sort of (camera, observation) pairs gives
-// every camera its observations in row order, the order the camera-block
-// assembly and the Schur sweep accumulate in.  Dropped observations (fixed
-// points) get the key num_cams and sort to the end.
+// point-major).  A stable LSD radix sort by camera of the point-grouped list
+// gives every camera its observations in point order, the order the
+// camera-block assembly and the Schur complement accumulate in.  Observations
+// dropped by fixPts are not in the point lists; the tail of the key array gets
+// the key num_cams and sorts to the end.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_run_length_encode.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
 
 namespace unc {
 
-__global__ void k_sort_keys(int nobs, int nc, const int *obs_cam, int *keys, int *vals) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nobs) return;
-  int c = obs_cam[o];
-  keys[o] = c >= 0 ? c : nc;
-  vals[o] = o;
+// key of the t-th point-grouped observation (its camera), value the observation
+__global__ void k_sort_keys(Structure s, int *pt_cams, int *keys, int *vals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= s.nobs) return;
+  if (t < s.pt_off[s.npc]) {
+    const int o = s.pt_list[t], c = s.obs_cam[o];
+    pt_cams[t] = c;
+    keys[t] = c;
+    vals[t] = o;
+  } else {
+    keys[t] = s.nc;
+    vals[t] = -1;
+  }
+}
+
+__global__ void k_cam_pts(Structure s, int *cam_pts) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < s.cam_off[s.nc]) cam_pts[t] = s.obs_pt[s.cam_list[t]];
 }
 
 static int key_bits(int nc) {
@@ -48,57 +60,24 @@ __global__ void k_segment_offsets(const int *keys, int n, int nc, int *off) {
   for (int c = prev + 1; c <= cur; ++c) off[c] = i;
 }
 
-hipError_t sort_obs_by_camera(int nobs, int nc, const int *obs_cam, int *keys_in, int *keys_out,
-                              int *vals_in, int *cam_list, int *cam_off, void *temp,
-                              size_t temp_bytes, hipStream_t st) {
+hipError_t sort_obs_by_camera(const Structure &s, int *pt_cams, int *cam_pts, int *keys_in,
+                              int *keys_out, int *vals_in, void *temp, size_t temp_bytes,
+                              hipStream_t st) {
+  const int nobs = s.nobs, nc = s.nc;
   if (nobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_keys, dim3(cdiv(nobs, 256)), dim3(256), 0, st, nobs, nc, obs_cam,
-                     keys_in, vals_in);
+  hipLaunchKernelGGL(k_sort_keys, dim3(cdiv(nobs, 256)), dim3(256), 0, st, s, pt_cams, keys_in,
+                     vals_in);
   size_t tb = temp_bytes;
-  hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys_in, keys_out, vals_in, cam_list,
+  hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys_in, keys_out, vals_in, s.cam_list,
                                            (unsigned int)nobs, 0, key_bits(nc), st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_segment_offsets, dim3(cdiv(nobs + 1, 256)), dim3(256), 0, st, keys_out, nobs,
-                     nc, cam_off);
+                     nc, s.cam_off);
+  hipLaunchKernelGGL(k_cam_pts, dim3(cdiv(nobs, 256)), dim3(256), 0, st, s, cam_pts);
   return hipGetLastError();
 }
 
-// ---- pair grouping for the Schur complement (k_schur.hip)
-// stable sort of (block key, packed pair); then runs of equal keys; exclusive
-// scans of int counts.  Temp sizes are for n elements.
-size_t pair_sort_temp_bytes(int n, int key_bits) {
-  size_t b = 0;
-  if (rocprim::radix_sort_pairs(nullptr, b, (unsigned int *)nullptr, (unsigned int *)nullptr,
-                                (unsigned long long *)nullptr, (unsigned long long *)nullptr,
-                                (unsigned int)n, 0, key_bits) != hipSuccess)
-    return 0;
-  return b;
-}
-
-hipError_t pair_sort(const unsigned int *keys_in, unsigned int *keys_out,
-                     const unsigned long long *vals_in, unsigned long long *vals_out, int n,
-                     int key_bits, void *temp, size_t temp_bytes, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  size_t tb = temp_bytes;
-  return rocprim::radix_sort_pairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (unsigned int)n,
-                                   0, key_bits, st);
-}
-
-size_t runs_temp_bytes(int n) {
-  size_t b = 0;
-  if (rocprim::run_length_encode(nullptr, b, (const unsigned int *)nullptr, (unsigned int)n,
-                                 (unsigned int *)nullptr, (int *)nullptr, (int *)nullptr) !=
-      hipSuccess)
-    return 0;
-  return b;
-}
-
-hipError_t runs(const unsigned int *keys, int n, unsigned int *unique, int *counts, int *nruns,
-                void *temp, size_t temp_bytes, hipStream_t st) {
-  size_t tb = temp_bytes;
-  return rocprim::run_length_encode(temp, tb, keys, (unsigned int)n, unique, counts, nruns, st);
-}
-
+// ---- exclusive scans of int counts (temp sizes are for n elements)
 size_t scan_temp_bytes(int n) {
   size_t b = 0;
   if (rocprim::exclusive_scan(nullptr, b, (const int *)nullptr, (int *)nullptr, 0, (size_t)n,

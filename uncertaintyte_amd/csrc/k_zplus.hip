@@ -25,142 +25,11 @@
 
 namespace unc {
 
-// ---------------------------------------------------------------------------
-// 64x64 leaf: Cholesky A = L L^T and X = L^-1, both lower triangular with zero
-// upper parts, by one wave.
-//   Cholesky, left-looking: lane i owns row i of L in registers; at step j it
-//   forms s = A(i,j) - L(i,0:j) . L(j,0:j) with row j read back from LDS,
-//   the pivot comes from lane j by a shuffle.
-//   Inverse, column-parallel forward substitution: lane j owns column j of X,
-//   L is read from LDS by broadcast.
-// Every loop is unrolled, so the rows live in registers.  A's lower triangle
-// is read (the upper one may hold stale values).
-// ---------------------------------------------------------------------------
+// Leaves are 64 x 64 blocks (one workgroup each).
 constexpr int CL_B = 64;
 
-__global__ __launch_bounds__(64) void k_chol_inv64(int b, double *A, int lda, double *Xo, int ldx,
-                                                   unsigned int *err) {
-  __shared__ double Ls[CL_B][CL_B + 1];
-  const int i = threadIdx.x;
-  double row[CL_B];
-#pragma unroll
-  for (int k = 0; k < CL_B; ++k)
-    row[k] = (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0) : (i == k ? 1.0 : 0.0);
-  bool bad = false;
-#pragma unroll
-  for (int j = 0; j < CL_B; ++j) {
-    double s0 = row[j], s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-    for (int k = 0; k + 3 < j; k += 4) {
-      s0 = fma(-row[k], Ls[j][k], s0);
-      s1 = fma(-row[k + 1], Ls[j][k + 1], s1);
-      s2 = fma(-row[k + 2], Ls[j][k + 2], s2);
-      s3 = fma(-row[k + 3], Ls[j][k + 3], s3);
-    }
-#pragma unroll
-    for (int k = j & ~3; k < j; ++k) s0 = fma(-row[k], Ls[j][k], s0);
-    const double s = (s0 + s1) + (s2 + s3);
-    const double piv = __shfl(s, j);
-    if (!(piv > 0.0)) bad = true;
-    const double d = sqrt(piv);
-    const double l = (i > j) ? s / d : (i == j ? d : 0.0);
-    row[j] = l;
-    Ls[i][j] = l;
-    __syncthreads();
-  }
-  if (i < b) {
-#pragma unroll
-    for (int k = 0; k < CL_B; ++k)
-      if (k < b) A[(size_t)i + (size_t)k * lda] = row[k];
-  }
-  // X = L^-1: lane j solves L x = e_j
-  const int j = i;
-  double x[CL_B];
-#pragma unroll
-  for (int r = 0; r < CL_B; ++r) {
-    double s0 = (r == j) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-    for (int k = 0; k + 1 < r; k += 2) {
-      s0 = fma(-Ls[r][k], x[k], s0);
-      s1 = fma(-Ls[r][k + 1], x[k + 1], s1);
-    }
-    if (r & 1) s0 = fma(-Ls[r][r - 1], x[r - 1], s0);
-    x[r] = (r >= j) ? (s0 + s1) / Ls[r][r] : 0.0;
-  }
-  __syncthreads();
-  // transpose through LDS so the column-major stores are coalesced
-#pragma unroll
-  for (int r = 0; r < CL_B; ++r) Ls[r][j] = x[r];
-  __syncthreads();
-  if (i < b) {
-#pragma unroll
-    for (int c = 0; c < CL_B; ++c)
-      if (c < b) Xo[(size_t)i + (size_t)c * ldx] = (i >= c) ? Ls[i][c] : 0.0;
-  }
-  if (bad && i == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
-}
-
-// Compact variant of the leaf: same algorithms with the rows in LDS and rolled
-// loops (a small code footprint), reciprocals instead of divisions.  Measured
-// 92 us against 68 us for the register form (its LDS latency chains dominate);
-// kept for the leaf benchmark.
-__global__ __launch_bounds__(64) void k_chol_inv64c(int b, double *A, int lda, double *Xo, int ldx,
-                                                    unsigned int *err) {
-  __shared__ double Ls[CL_B][CL_B + 1];
-  __shared__ double Xs[CL_B][CL_B + 1];
-  __shared__ double rinv[CL_B];
-  const int i = threadIdx.x;
-  for (int k = 0; k < CL_B; ++k)
-    Ls[i][k] = (i < b && k < b) ? (k <= i ? A[(size_t)i + (size_t)k * lda] : 0.0)
-                                : (i == k ? 1.0 : 0.0);
-  __syncthreads();
-  bool bad = false;
-  for (int j = 0; j < CL_B; ++j) {
-    double s0 = Ls[i][j], s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int k = 0;
-    for (; k + 3 < j; k += 4) {
-      s0 = fma(-Ls[i][k], Ls[j][k], s0);
-      s1 = fma(-Ls[i][k + 1], Ls[j][k + 1], s1);
-      s2 = fma(-Ls[i][k + 2], Ls[j][k + 2], s2);
-      s3 = fma(-Ls[i][k + 3], Ls[j][k + 3], s3);
-    }
-    for (; k < j; ++k) s0 = fma(-Ls[i][k], Ls[j][k], s0);
-    const double s = (s0 + s1) + (s2 + s3);
-    const double piv = __shfl(s, j);
-    if (!(piv > 0.0)) bad = true;
-    const double d = sqrt(piv), rd = 1.0 / d;
-    const double l = (i > j) ? s * rd : (i == j ? d : 0.0);
-    __syncthreads();
-    Ls[i][j] = l;
-    if (i == j) rinv[j] = rd;
-    __syncthreads();
-  }
-  // X = L^-1, lane c owns column c (rows below c; zeros above)
-  const int c = i;
-  for (int r = 0; r < CL_B; ++r) {
-    double s0 = (r == c) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int k = 0;
-    for (; k + 3 < r; k += 4) {
-      s0 = fma(-Ls[r][k], Xs[k][c], s0);
-      s1 = fma(-Ls[r][k + 1], Xs[k + 1][c], s1);
-      s2 = fma(-Ls[r][k + 2], Xs[k + 2][c], s2);
-      s3 = fma(-Ls[r][k + 3], Xs[k + 3][c], s3);
-    }
-    for (; k < r; ++k) s0 = fma(-Ls[r][k], Xs[k][c], s0);
-    Xs[r][c] = (r >= c) ? ((s0 + s1) + (s2 + s3)) * rinv[r] : 0.0;
-  }
-  __syncthreads();
-  if (i < b) {
-    for (int k = 0; k < b; ++k) {
-      A[(size_t)i + (size_t)k * lda] = Ls[i][k];
-      Xo[(size_t)i + (size_t)k * ldx] = Xs[i][k];
-    }
-  }
-  if (bad && i == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
-}
-
 // ---------------------------------------------------------------------------
-// Panelled leaf (the default): 256 threads, the block in LDS, 8-column panels.
+// Panelled leaf: 256 threads, the block in LDS, 8-column panels.
 //   Cholesky, right-looking by panels with one panel of look-ahead: all four
 //   waves first apply panel p's rank-8 update to the next panel's 8 columns;
 //   then wave 0 factors that panel (lane = row; the panel rows stay in
@@ -613,18 +482,10 @@ __global__ __launch_bounds__(512, 1) void k_gj_update(int n, int k0, int kb, dou
   gj_next_inverse(S, Xs, act, kb1, Dsrc, T + k1, n, R + (size_t)k1 * kb, kb, kb, Pn, err);
 }
 
-static int g_leaf_variant = 0;
-void set_leaf_variant(int v) { g_leaf_variant = v; }
-
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st) {
   if (b < 1 || b > CL_B) return hipErrorInvalidValue;
-  if (g_leaf_variant == 1)
-    hipLaunchKernelGGL(k_chol_inv64c, dim3(1), dim3(64), 0, st, b, A, lda, X, ldx, err);
-  else if (g_leaf_variant == 2)
-    hipLaunchKernelGGL(k_chol_inv64, dim3(1), dim3(64), 0, st, b, A, lda, X, ldx, err);
-  else
-    hipLaunchKernelGGL(k_chol_inv64p, dim3(1), dim3(CP_THREADS), 0, st, b, A, lda, X, ldx, err);
+  hipLaunchKernelGGL(k_chol_inv64p, dim3(1), dim3(CP_THREADS), 0, st, b, A, lda, X, ldx, err);
   return hipGetLastError();
 }
 

@@ -44,10 +44,9 @@ struct Sizes {
 
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
-  B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR,
-  B_ERR, B_SCAN, B_PCNT, B_POFS, B_CPS, B_PK0, B_PK1, B_PV0, B_PV1, B_RUNK, B_RUNC,
-  B_RUNO, B_NRUN, B_PTMP, B_XD, B_US, B_D, B_VI, B_WS, B_Y, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP,
-  B_NULL, B_DA, B_DB, B_DC, B_DD, B_DE, B_SPLIT, B_COUNT
+  B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR, B_ERR, B_SCAN, B_CAM_PTS, B_PT_CAMS,
+  B_GHASH, B_XD, B_US, B_D, B_VI, B_WS, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP, B_NULL, B_DA, B_DB,
+  B_DC, B_DD, B_DE, B_SPLIT, B_COUNT
 };
 
 struct Layout {
@@ -58,17 +57,7 @@ struct Layout {
 constexpr int GJ_BLOCK = 64;
 size_t zplus_work_doubles(int n);
 
-size_t pair_capacity(int nobs) { return std::max<size_t>((size_t)4 * (size_t)nobs, (size_t)1 << 20); }
-
-size_t pair_temp_bytes(int nobs) {
-  const int cap = (int)pair_capacity(nobs);
-  size_t b = pair_sort_temp_bytes(cap, 32);
-  b = std::max(b, runs_temp_bytes(cap));
-  b = std::max(b, scan_temp_bytes(std::max(cap, nobs)));
-  return b + 256;
-}
-
-Layout make_layout(const Sizes &z, bool refine) {
+Layout make_layout(const Sizes &z) {
   size_t sz[B_COUNT];
   sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
   sz[B_CAM_OFF] = sizeof(int) * ((size_t)z.nc + 1);
@@ -77,23 +66,13 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_PT_CNT] = sz[B_PT_CURSOR] = sz[B_PT_HAS] = sizeof(int) * (size_t)z.npc;
   sz[B_PT_OFF] = sizeof(int) * ((size_t)z.npc + 1);
   sz[B_ERR] = 64;
-  // camera-pair lists of the Schur complement: one camera's pairs never exceed
-  // the observations of its points, so pair_capacity(nobs) >= any camera
-  const size_t cap = pair_capacity(z.nobs);
-  sz[B_PCNT] = sz[B_POFS] = sizeof(int) * (size_t)z.nobs;
-  sz[B_CPS] = sizeof(int) * ((size_t)z.nc + 1);
-  sz[B_PK0] = sz[B_PK1] = sz[B_RUNK] = sizeof(unsigned int) * cap;
-  sz[B_PV0] = sz[B_PV1] = sizeof(unsigned long long) * cap;
-  sz[B_RUNC] = sz[B_RUNO] = sizeof(int) * cap;
-  sz[B_NRUN] = sizeof(int);
-  sz[B_PTMP] = pair_temp_bytes(z.nobs);
   sz[B_SCAN] = sizeof(int) * ((size_t)std::max(z.npc, z.nc) / 1024 + 2);
+  sz[B_CAM_PTS] = sz[B_PT_CAMS] = sizeof(int) * (size_t)z.nobs;
+  sz[B_GHASH] = 4 * sizeof(int) * (size_t)z.nobs;  // global Schur tables of large cameras
   sz[B_US] = sz[B_XD] = sizeof(double) * (size_t)z.nc * z.cp * z.cp;
   sz[B_D] = sizeof(double) * ((size_t)z.N + 3 * (size_t)z.npc);
   sz[B_VI] = sizeof(double) * 9 * (size_t)z.npc;
   sz[B_WS] = sizeof(double) * 3 * (size_t)z.cp * z.nobs;
-  // Y, then the point step's per-observation 3x3 terms
-  sz[B_Y] = sizeof(double) * (size_t)std::max(3 * z.cp, 9) * z.nobs;
   sz[B_CTRL] = sizeof(TeCtrl);
   sz[B_P] = sizeof(double) * 3 * GJ_BLOCK * GJ_BLOCK;
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
@@ -101,7 +80,6 @@ Layout make_layout(const Sizes &z, bool refine) {
   sz[B_DE] = sizeof(double) * z.nn;
   sz[B_SPLIT] = sizeof(double) * gemm_sym_split_work_doubles(z.N);  // Taylor chain K chunks
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
-  (void)refine;
   Layout L;
   size_t o = 0;
   for (int i = 0; i < B_COUNT; ++i) {
@@ -114,8 +92,8 @@ Layout make_layout(const Sizes &z, bool refine) {
 
 int map_err(unsigned int e) {
   if (e & (ERRBIT_LAYOUT | ERRBIT_RANGE)) return UNC_ERR_BAD_INPUT;
+  if (e & ERRBIT_DUPLICATE) return UNC_ERR_UNSUPPORTED;
   if (e & (ERRBIT_ZERO_COLUMN | ERRBIT_SINGULAR_V | ERRBIT_GJ_PIVOT)) return UNC_ERR_SINGULAR;
-  if (e & ERRBIT_SCHUR_OVERFLOW) return UNC_ERR_UNSUPPORTED;
   return UNC_ERR_BAD_INPUT;
 }
 
@@ -423,21 +401,36 @@ int svd_run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const 
   return UNC_OK;
 }
 
+// Workspace of a run: the context's cached buffer (grown on demand), or a
+// caller-provided region of at least the layout's size (no allocation).
+struct Workspace {
+  char *base = nullptr;
+  size_t bytes = 0;
+};
+
 int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
         const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
-        unc_debug *dbg, hipStream_t s, bool host_api_upload_timed) {
+        unc_debug *dbg, hipStream_t s, bool host_api_upload_timed, const Workspace *fixed_ws) {
   if (o && (o->algorithm == UNC_ALG_SVD_QR_ITERATION ||
-            o->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER))
+            o->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER)) {
+    if (fixed_ws) return UNC_ERR_UNSUPPORTED;
     return svd_run(ctx, o, nrows, ncols, d_rows, d_cols, d_vals, d_cam, d_pt, st, s);
+  }
   Sizes z;
   int fix[3];
   int rc = check_options(o, nrows, ncols, &z, fix);
   if (rc) return rc;
   const bool refine = o->refine != 0;
-  Layout L = make_layout(z, refine);
-  rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total, s);
-  if (rc) return rc;
-  char *w = ctx->ws;
+  const Layout L = make_layout(z);
+  char *w = nullptr;
+  if (fixed_ws) {
+    if (fixed_ws->bytes < L.total) return UNC_ERR_OOM;
+    w = fixed_ws->base;
+  } else {
+    rc = ensure(&ctx->ws, &ctx->ws_bytes, L.total, s);
+    if (rc) return rc;
+    w = ctx->ws;
+  }
   auto ptr = [&](int b) { return (void *)(w + L.off[b]); };
 
   Structure S;
@@ -452,7 +445,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   S.err = (unsigned int *)ptr(B_ERR);
   Blocks Bk;
   Bk.Us = (double *)ptr(B_US); Bk.d = (double *)ptr(B_D); Bk.Vi = (double *)ptr(B_VI);
-  Bk.Ws = (double *)ptr(B_WS); Bk.Y = (double *)ptr(B_Y); Bk.pt_has = (int *)ptr(B_PT_HAS);
+  Bk.Ws = (double *)ptr(B_WS); Bk.pt_has = (int *)ptr(B_PT_HAS);
   TeCtrl *ctrl = (TeCtrl *)ptr(B_CTRL);
   double *dA = (double *)ptr(B_DA), *dB = (double *)ptr(B_DB), *dC = (double *)ptr(B_DC);
   double *dD = (double *)ptr(B_DD), *dE = (double *)ptr(B_DE);
@@ -462,67 +455,74 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   if (!host_api_upload_timed) HC(hipEventRecord(E[0], s));
   HC(hipEventRecord(E[1], s));
 
-  // ---- decode + fixPts
+  // ---- decode + fixPts.  Invalid observations are flagged in the error word
+  // and treated as dropped by every later kernel; the word is read once, with
+  // the results, so no stage waits for the host.
   HC(launch_decode(S, s));
-  unsigned int herr = 0;
-  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  if (herr) return map_err(herr);
   HC(hipEventRecord(E[2], s));
   HC(launch_point_lists(S, (int *)ptr(B_SCAN), s));
-  HC(sort_obs_by_camera(z.nobs, z.nc, S.obs_cam, (int *)ptr(B_SORT_K0), (int *)ptr(B_SORT_K1),
-                        (int *)ptr(B_SORT_V), S.cam_list, S.cam_off, ptr(B_SORT_TMP),
+  int *cam_pts = (int *)ptr(B_CAM_PTS), *pt_cams = (int *)ptr(B_PT_CAMS);
+  HC(sort_obs_by_camera(S, pt_cams, cam_pts, (int *)ptr(B_SORT_K0), (int *)ptr(B_SORT_K1),
+                        (int *)ptr(B_SORT_V), ptr(B_SORT_TMP),
                         camera_sort_temp_bytes(z.nobs, z.nc) + 256, s));
   HC(hipEventRecord(E[3], s));
-  // ---- (a) + (b)
-  HC(launch_assemble(S, Bk, s));
+  // ---- (a) + (b): camera blocks, point blocks + 3x3 inverses, W_s
+  HC(launch_cam_blocks(S, Bk, s));
+  HC(hipEventRecord(E[11], s));
+  HC(launch_point_blocks(S, Bk, s));
+  HC(hipEventRecord(E[12], s));
+  HC(launch_obs_blocks(S, Bk, s));
   HC(hipEventRecord(E[4], s));
   HC(hipEventRecord(E[5], s));
-  // ---- (c) camera-pair lists: counts, positions, per-camera starts
-  int *pcnt = (int *)ptr(B_PCNT), *pofs = (int *)ptr(B_POFS), *cps = (int *)ptr(B_CPS);
-  void *ptmp = ptr(B_PTMP);
-  const size_t ptmp_bytes = pair_temp_bytes(z.nobs);
-  HC(launch_pair_count(S, pcnt, s));
-  HC(exclusive_scan_int(pcnt, pofs, z.nobs, ptmp, ptmp_bytes, s));
-  HC(launch_cam_pair_start(S, pofs, pcnt, cps, s));
-  std::vector<int> h_cps(z.nc + 1), h_off(z.nc + 1);
-  HC(hipMemcpyAsync(h_cps.data(), cps, sizeof(int) * (z.nc + 1), hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(h_off.data(), S.cam_off, sizeof(int) * (z.nc + 1), hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  if (herr) return map_err(herr);
-  // ---- (c) Schur into dA, camera ranges whose pairs fit the list capacity
-  HC(hipMemsetAsync(dA, 0, sizeof(double) * nn, s));
-  {
-    const long long cap = (long long)pair_capacity(z.nobs);
-    unsigned int *k0 = (unsigned int *)ptr(B_PK0), *k1 = (unsigned int *)ptr(B_PK1);
-    unsigned long long *v0 = (unsigned long long *)ptr(B_PV0), *v1 = (unsigned long long *)ptr(B_PV1);
-    unsigned int *rk = (unsigned int *)ptr(B_RUNK);
-    int *rc_ = (int *)ptr(B_RUNC), *ro = (int *)ptr(B_RUNO), *nrun = (int *)ptr(B_NRUN);
-    int a0 = 0;
-    while (a0 < z.nc) {
-      int a1 = a0 + 1;
-      if ((long long)h_cps[a1] - h_cps[a0] > cap) return UNC_ERR_UNSUPPORTED;
-      while (a1 < z.nc && (long long)h_cps[a1 + 1] - h_cps[a0] <= cap) ++a1;
-      const int np_ = h_cps[a1] - h_cps[a0];
-      if (np_ > 0) {
-        int kb = 1;
-        while (kb < 32 && ((unsigned long long)1 << kb) < (unsigned long long)(a1 - a0) * z.nc) ++kb;
-        HC(launch_pair_emit(S, h_off[a0], h_off[a1], a0, h_cps[a0], pofs, k0, v0, s));
-        HC(pair_sort(k0, k1, v0, v1, np_, kb, ptmp, ptmp_bytes, s));
-        HC(runs(k1, np_, rk, rc_, nrun, ptmp, ptmp_bytes, s));
-        HC(exclusive_scan_int(rc_, ro, np_, ptmp, ptmp_bytes, s));
-        HC(launch_schur_reduce(S, Bk, dA, a0, rk, rc_, ro, nrun, v1, np_, s));
-      }
-      a0 = a1;
-    }
-  }
+  // ---- (c) Schur complement into dA; blocks of cameras without a shared
+  // point stay zero
+  HC(fill_zero(dA, nn, s));
+  HC(launch_schur(S, Bk, dA, cam_pts, pt_cams, (int *)ptr(B_GHASH), s));
   HC(hipEventRecord(E[6], s));
   if (dbg && dbg->Z) HC(hipMemcpyAsync(dbg->Z, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  unsigned int herr = 0;
+  int kept = 0;
+  auto fill_stats = [&](double lambda, int nterms, const double *chg) {
+    std::memset(st, 0, sizeof(*st));
+    st->lambda = lambda;
+    st->fixed_pts[0] = fix[0]; st->fixed_pts[1] = fix[1]; st->fixed_pts[2] = fix[2];
+    st->num_terms = nterms;
+    for (int i = 0; i < nterms && i < 20; ++i) st->cycle_change[i] = chg[i];
+    st->num_obs_kept = kept;
+    st->stage_ms[1] = ev_ms(E[1], E[2]);
+    st->stage_ms[2] = ev_ms(E[2], E[3]);
+    st->stage_ms[3] = ev_ms(E[3], E[4]);
+    st->stage_ms[5] = ev_ms(E[5], E[6]);
+    st->stage_ms[6] = ev_ms(E[6], E[7]);
+    st->stage_ms[7] = ev_ms(E[7], E[8]);
+    st->stage_ms[8] = ev_ms(E[8], E[9]);
+    st->stage_ms[9] = ev_ms(E[9], E[10]);
+    st->stage_ms[11] = ev_ms(E[host_api_upload_timed ? 0 : 1], E[10]);
+    if (host_api_upload_timed) st->stage_ms[0] = ev_ms(E[0], E[1]);
+    st->time_create_j = st->stage_ms[0] * 1e-3;
+    st->time_fix_j = (st->stage_ms[1] + st->stage_ms[2]) * 1e-3;
+    // column norms are the diagonals of the assembled blocks and the blocks are
+    // written split: J^T J = camera blocks + W_s, V^-1 = point blocks
+    st->time_norm_j = 0.0;
+    st->time_multiply_jj = (ev_ms(E[3], E[11]) + ev_ms(E[12], E[4])) * 1e-3;
+    st->time_split_jj = 0.0;
+    st->time_inv_v = ev_ms(E[11], E[12]) * 1e-3;
+    st->time_compose_z = st->stage_ms[5] * 1e-3;
+    st->time_inv_z = st->stage_ms[6] * 1e-3;
+    st->time_te = st->stage_ms[7] * 1e-3;
+    st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
+    st->time_all = st->stage_ms[11] * 1e-3;
+  };
+  auto copy_debug_blocks = [&]() -> int {
+    if (!dbg) return UNC_OK;
+    if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
+    if (dbg->Vinv_s) HC(hipMemcpyAsync(dbg->Vinv_s, Bk.Vi, sizeof(double) * 9 * (size_t)z.npc, hipMemcpyDeviceToHost, s));
+    if (dbg->d) HC(hipMemcpyAsync(dbg->d, Bk.d, sizeof(double) * ((size_t)N + 3 * (size_t)z.npc), hipMemcpyDeviceToHost, s));
+    return UNC_OK;
+  };
   if (o->precision != UNC_PREC_FP64) {
     // gauge-fixed Z^+ path
     const int steps = o->refine >= 0 ? o->refine : 1;
-    int null_iters = steps;
     double *Xd = (double *)ptr(B_XD);
     rc = zplus(N, steps, dA, dB, dC, dD, (double *)ptr(B_NULL), S, Xd, S.err, s);
     if (rc) return rc;
@@ -533,38 +533,15 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     HC(hipMemsetAsync(d_pt, 0, sizeof(double) * 6 * (size_t)z.np, s));
     HC(launch_point_output(S, Bk, Xd, d_pt, z.np, s));
     HC(hipEventRecord(E[10], s));
-    if (dbg) {
-      if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
-      if (dbg->Vinv_s) HC(hipMemcpyAsync(dbg->Vinv_s, Bk.Vi, sizeof(double) * 9 * (size_t)z.npc, hipMemcpyDeviceToHost, s));
-      if (dbg->d) HC(hipMemcpyAsync(dbg->d, Bk.d, sizeof(double) * ((size_t)N + 3 * (size_t)z.npc), hipMemcpyDeviceToHost, s));
-      if (dbg->X) HC(hipMemcpyAsync(dbg->X, dD, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-    }
+    rc = copy_debug_blocks();
+    if (rc) return rc;
+    // X: the inverse Cholesky factor L^-1 of Z + N0 N0^T (lower triangle)
+    if (dbg && dbg->X) HC(hipMemcpyAsync(dbg->X, dC, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
     HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(&kept, S.pt_off + z.npc, sizeof(int), hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
     if (herr) return map_err(herr);
-    if (st) {
-      std::memset(st, 0, sizeof(*st));
-      st->lambda = 0.0;
-      st->fixed_pts[0] = fix[0]; st->fixed_pts[1] = fix[1]; st->fixed_pts[2] = fix[2];
-      st->num_terms = 0;
-      st->stage_ms[1] = ev_ms(E[1], E[2]);
-      st->stage_ms[2] = ev_ms(E[2], E[3]);
-      st->stage_ms[3] = ev_ms(E[3], E[4]);
-      st->stage_ms[5] = ev_ms(E[5], E[6]);
-      st->stage_ms[6] = ev_ms(E[6], E[7]);
-      st->stage_ms[8] = ev_ms(E[8], E[9]);
-      st->stage_ms[9] = ev_ms(E[9], E[10]);
-      st->stage_ms[11] = ev_ms(E[host_api_upload_timed ? 0 : 1], E[10]);
-      if (host_api_upload_timed) st->stage_ms[0] = ev_ms(E[0], E[1]);
-      st->time_create_j = st->stage_ms[0] * 1e-3;
-      st->time_fix_j = (st->stage_ms[1] + st->stage_ms[2]) * 1e-3;
-      st->time_multiply_jj = st->stage_ms[3] * 1e-3;
-      st->time_compose_z = st->stage_ms[5] * 1e-3;
-      st->time_inv_z = st->stage_ms[6] * 1e-3;
-      st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
-      st->time_all = st->stage_ms[11] * 1e-3;
-      st->num_obs_kept = null_iters;
-    }
+    if (st) fill_stats(0.0, 0, nullptr);
     return UNC_OK;
   }
   // ---- (d) A = Z + lambda I ; X0 = A^-1
@@ -576,6 +553,9 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   hipEvent_t *GE = ctx->gemm_ev;
   int gemm_pairs = 0;
   double *X0 = dB;
+  // Taylor-chain K chunks (the symmetric products' scratch)
+  double *sym_work = (double *)ptr(B_SPLIT);
+  const size_t sym_work_doubles = gemm_sym_split_work_doubles(N);
   if (refine) {
     // X0 from Gauss-Jordan is symmetric only to about cond * eps; symmetrised
     // first, X0 R = X0 - X0 A X0 is symmetric too (up to rounding), so the
@@ -583,15 +563,14 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     // antisymmetric part of the size of X0's asymmetry, which mirroring would
     // fold into the result.)
     HC(svd_symmetrize(N, dB, s));
-    HC(residual_dd(N, dA, dB, dC, dE, s));  // dE is free until X1 is formed
+    HC(residual_ex(N, dA, dB, dC, dE, s));  // dE is free until X1 is formed
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64_sym(N, 1.0, dB, dC, dD, (double *)ptr(B_SPLIT), gemm_sym_split_work_doubles(N),
-                    nullptr, nullptr, s));
+    HC(gemm_f64_sym(N, 1.0, dB, dC, dD, sym_work, sym_work_doubles, nullptr, nullptr, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
     ++gemm_pairs;
     // X0 was symmetrised and the lower-tile product mirrors D: the plain sum
     // is then already symmetric
-    if (gemm_sym_split_work_doubles(N) > 0) HC(add_symmetric(N, dB, dD, dE, s));
+    if (sym_work_doubles > 0) HC(add_symmetric(N, dB, dD, dE, s));
     else HC(sym_update(N, dB, dD, dE, s));
     X0 = dE;
   }
@@ -601,21 +580,8 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   double *X = dA;
   double *Tb0 = dC, *Tb1 = dD;
   // X0^(i+2) = T_i X0 is symmetric: only its lower tiles are computed
-  double *sym_work = (double *)ptr(B_SPLIT);
-  const size_t sym_work_doubles = gemm_sym_split_work_doubles(N);
   HC(hipMemcpyAsync(X, X0, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
-  {
-    TeCtrl h;
-    std::memset(&h, 0, sizeof(h));
-    const double dmin = 2.2250738585072014e-308;  // DBL_MIN, absMax's start value
-    unsigned long long mb;
-    std::memcpy(&mb, &dmin, sizeof(mb));
-    for (int i = 0; i < 21; ++i) h.maxbits[i] = mb;
-    h.old_change = 1.7976931348623157e308;  // DBL_MAX
-    h.active = 1;
-    HC(hipMemcpyAsync(ctrl, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    HC(hipStreamSynchronize(s));  // h is a stack object
-  }
+  HC(te_init(ctrl, s));
   const int first_chain_pair = gemm_pairs;
   HC(hipEventRecord(GE[2 * gemm_pairs], s));
   HC(gemm_f64_sym(N, 1.0, X0, X0, Tb0, sym_work, sym_work_doubles, &ctrl->maxbits[1], nullptr, s));
@@ -641,23 +607,17 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   HC(launch_extract_diag(S, X, (double *)ptr(B_XD), s));
   HC(launch_point_output(S, Bk, (double *)ptr(B_XD), d_pt, z.np, s));
   HC(hipEventRecord(E[10], s));
-  if (dbg) {
-    if (dbg->U_s) HC(hipMemcpyAsync(dbg->U_s, Bk.Us, sizeof(double) * z.nc * z.cp * z.cp, hipMemcpyDeviceToHost, s));
-    if (dbg->Vinv_s) HC(hipMemcpyAsync(dbg->Vinv_s, Bk.Vi, sizeof(double) * 9 * (size_t)z.npc, hipMemcpyDeviceToHost, s));
-    if (dbg->d) HC(hipMemcpyAsync(dbg->d, Bk.d, sizeof(double) * ((size_t)N + 3 * (size_t)z.npc), hipMemcpyDeviceToHost, s));
-  }
+  rc = copy_debug_blocks();
+  if (rc) return rc;
   TeCtrl hc;
   HC(hipMemcpyAsync(&hc, ctrl, sizeof(hc), hipMemcpyDeviceToHost, s));
   HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HC(hipMemcpyAsync(&kept, S.pt_off + z.npc, sizeof(int), hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
   if (herr) return map_err(herr);
 
   if (st) {
-    std::memset(st, 0, sizeof(*st));
-    st->lambda = lambda;
-    st->fixed_pts[0] = fix[0]; st->fixed_pts[1] = fix[1]; st->fixed_pts[2] = fix[2];
-    st->num_terms = hc.nterms;
-    for (int i = 0; i < hc.nterms && i < 20; ++i) st->cycle_change[i] = hc.change[i];
+    fill_stats(lambda, hc.nterms, hc.change);
     double gms = 0.0;
     int gcount = 0;
     // executed products of the Taylor chain: T1 and one per accepted term
@@ -672,25 +632,6 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     // entries of 2 N flops each
     const double nd = (double)N;
     st->gemm_flops = (sym_work_doubles > 0 ? nd * nd * (nd + 1.0) : 2.0 * nd * nd * nd) * gcount;
-    st->stage_ms[1] = ev_ms(E[1], E[2]);
-    st->stage_ms[2] = ev_ms(E[2], E[3]);
-    st->stage_ms[3] = ev_ms(E[3], E[4]);
-    st->stage_ms[5] = ev_ms(E[5], E[6]);
-    st->stage_ms[6] = ev_ms(E[6], E[7]);
-    st->stage_ms[7] = ev_ms(E[7], E[8]);
-    st->stage_ms[8] = ev_ms(E[8], E[9]);
-    st->stage_ms[9] = ev_ms(E[9], E[10]);
-    st->stage_ms[11] = ev_ms(E[host_api_upload_timed ? 0 : 1], E[10]);
-    if (host_api_upload_timed) st->stage_ms[0] = ev_ms(E[0], E[1]);
-    st->time_create_j = st->stage_ms[0] * 1e-3;
-    st->time_fix_j = (st->stage_ms[1] + st->stage_ms[2]) * 1e-3;
-    st->time_multiply_jj = st->stage_ms[3] * 1e-3;
-    st->time_compose_z = st->stage_ms[5] * 1e-3;
-    st->time_inv_z = st->stage_ms[6] * 1e-3;
-    st->time_te = st->stage_ms[7] * 1e-3;
-    st->time_pts_unc = (st->stage_ms[8] + st->stage_ms[9]) * 1e-3;
-    st->time_all = st->stage_ms[11] * 1e-3;
-    st->num_obs_kept = -1;
   }
   return UNC_OK;
 }
@@ -776,13 +717,16 @@ int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int nu
   HC(hipMemcpyAsync(d_rows, rows, sizeof(int) * (num_rows + 1), hipMemcpyHostToDevice, s));
   HC(hipMemcpyAsync(d_cols, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
   HC(hipMemcpyAsync(d_vals, values, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
-  rc = run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_vals, d_cam, d_pt, stats, dbg, s, true);
+  rc = run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_vals, d_cam, d_pt, stats, dbg, s, true,
+           nullptr);
   if (rc) return rc;
   HC(hipMemcpyAsync(cam_unc, d_cam, sizeof(double) * cam_n, hipMemcpyDeviceToHost, s));
   HC(hipMemcpyAsync(pt_unc, d_pt, sizeof(double) * pt_n, hipMemcpyDeviceToHost, s));
   HC(hipStreamSynchronize(s));
   for (size_t i = 0; i < cam_n; ++i)
     if (!std::isfinite(cam_unc[i])) return UNC_ERR_NUMERIC;
+  for (size_t i = 0; i < pt_n; ++i)
+    if (!std::isfinite(pt_unc[i])) return UNC_ERR_NUMERIC;
   return UNC_OK;
 }
 
@@ -866,7 +810,7 @@ int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows, 
   HC(hipSetDevice(ctx->device));
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
   return run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_values, d_cam_unc, d_pt_unc, stats,
-             nullptr, s, false);
+             nullptr, s, false, nullptr);
 }
 
 int unc_eig_cameras_device(unc_context *ctx, int num_cams, int cam_params,
@@ -1049,7 +993,7 @@ unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   Sizes z;
   z.nc = o->num_cams; z.cp = o->cam_params; z.np = o->num_points; z.npc = z.np - 3;
   z.nobs = num_rows / 2; z.N = z.nc * z.cp; z.nn = (size_t)z.N * z.N;
-  return (unsigned long long)make_layout(z, o->refine != 0).total;
+  return (unsigned long long)make_layout(z).total;
 }
 
 const char *unc_error_string(int code) {
@@ -1081,312 +1025,17 @@ int unc_device_lds_bytes(int device) {
   return lds;
 }
 
-// ---------------------------------------------------------------------------
-// kernel-level diagnostics (host pointers; used by the parity tests to pin one
-// kernel at a time)
-// ---------------------------------------------------------------------------
-int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
-                      int ldb, double beta, double *C, int ldc, double *absmax_out) {
-  if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < K || ldc < M) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * (na + nb + nc) + 256));
-  double *dA = (double *)buf, *dB = dA + na, *dC = dB + nb;
-  unsigned long long *dm = (unsigned long long *)(dC + nc);
-  HC(hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
-  HC(gemm_f64(M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, dm, nullptr, s));
-  unsigned long long mb = 0;
-  HC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
-  return UNC_OK;
-}
-
-int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
-                            const double *B, int ldb, double *C, int ldc, double *absmax_out) {
-  if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < K || ldc < M) return UNC_ERR_BAD_INPUT;
-  if (!gemm_split_applies(M, N, K)) return UNC_ERR_UNSUPPORTED;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N,
-               nw = (size_t)M * N;
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * (na + nb + nc + nw) + 256));
-  double *dA = (double *)buf, *dB = dA + na, *dC = dB + nb, *dW = dC + nc;
-  unsigned long long *dm = (unsigned long long *)(dW + nw);
-  HC(hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
-  HC(gemm_f64_split(M, N, K, alpha, dA, lda, dB, ldb, dC, ldc, dW, dm, nullptr, s));
-  unsigned long long mb = 0;
-  HC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
-  return UNC_OK;
-}
-
-int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
-                          double *absmax_out) {
-  if (n <= 0 || !A || !B || !C) return UNC_ERR_BAD_INPUT;
-  if (!gemm_split_applies(n, n, n)) return UNC_ERR_UNSUPPORTED;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n, nw = gemm_sym_split_work_doubles(n);
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * (3 * nn + nw) + 256));
-  double *dA = (double *)buf, *dB = dA + nn, *dC = dB + nn, *dW = dC + nn;
-  unsigned long long *dm = (unsigned long long *)(dW + nw);
-  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dB, B, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(hipMemsetAsync(dC, 0, sizeof(double) * nn, s));
-  HC(hipMemsetAsync(dm, 0, sizeof(unsigned long long), s));
-  HC(gemm_f64_sym(n, alpha, dA, dB, dC, dW, nw, dm, nullptr, s));
-  unsigned long long mb = 0;
-  HC(hipMemcpyAsync(C, dC, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  if (absmax_out) std::memcpy(absmax_out, &mb, sizeof(double));
-  return UNC_OK;
-}
-
-__global__ void k_fill_pattern(size_t n, double *A, double scale) {
-  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n;
-       x += (size_t)gridDim.x * blockDim.x)
-    A[x] = scale * (double)((x * 2654435761ull) % 1000003ull) / 1000003.0 - 0.5 * scale;
-}
-
-int unc_diag_gemm_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 12) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n;
-  double *buf = nullptr;
-  if (hipMalloc((void **)&buf, sizeof(double) * 6 * nn) != hipSuccess) {
-    (void)hipGetLastError();
-    return UNC_ERR_OOM;
-  }
-  hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
-  const bool split = variant == 8, sym = variant >= 9;
-  set_gemm_variant(split || sym ? 0 : variant);
-  if (sym) set_sym_parts(variant - 8);  // 9..12: symmetric form with 1..4 K chunks
-  auto one = [&]() {
-    if (sym)
-      return gemm_f64_sym(n, 1.0, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, 3 * nn, nullptr,
-                          nullptr, s);
-    return split ? gemm_f64_split(n, n, n, 1.0, buf, n, buf + nn, n, buf + 2 * nn, n, buf + 3 * nn,
-                                  nullptr, nullptr, s)
-                 : gemm_f64(n, n, n, 1.0, buf, n, buf + nn, n, 0.0, buf + 2 * nn, n, nullptr,
-                            nullptr, s);
-  };
-  HC(one());
-  HC(hipEventRecord(ctx->gemm_ev[0], s));
-  for (int r = 0; r < reps; ++r) HC(one());
-  HC(hipEventRecord(ctx->gemm_ev[1], s));
-  HC(hipEventSynchronize(ctx->gemm_ev[1]));
-  *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
-  set_gemm_variant(0);
-  set_sym_parts(0);
-  HC(hipFree(buf));
-  return UNC_OK;
-}
-
-int unc_diag_residual_bench(int n, int reps, int variant, double *avg_ms) {
-  if (n <= 0 || reps <= 0 || !avg_ms || variant < 0 || variant > 7) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n;
-  double *buf = nullptr;
-  if (hipMalloc((void **)&buf, sizeof(double) * (3 * nn + 2 * (size_t)n)) != hipSuccess) {
-    (void)hipGetLastError();
-    return UNC_ERR_OOM;
-  }
-  hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, s, 2 * nn, buf, 1.0);
-  set_residual_variant(variant);
-  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
-  HC(hipEventRecord(ctx->gemm_ev[0], s));
-  for (int r = 0; r < reps; ++r) HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
-  HC(hipEventRecord(ctx->gemm_ev[1], s));
-  HC(hipEventSynchronize(ctx->gemm_ev[1]));
-  *avg_ms = ev_ms(ctx->gemm_ev[0], ctx->gemm_ev[1]) / reps;
-  set_residual_variant(0);
-  HC(hipFree(buf));
-  return UNC_OK;
-}
-
-int unc_diag_set_gemm_variant(int variant) {
-  if (variant >= 20 && variant <= 24) {  // K chunks of the symmetric chain product
-    set_sym_parts(variant - 20);
-    return UNC_OK;
-  }
-  if (variant == 30 || variant == 31) {  // symmetric product: all tiles / lower-tile list
-    set_sym_list(variant - 30);
-    return UNC_OK;
-  }
-  if (variant == 40 || variant == 41) {  // split product: id order / XCD-contiguous blocks
-    set_split_blocked(variant - 40);
-    return UNC_OK;
-  }
-  if (variant >= 60 && variant <= 67) {  // residual R = I - A X: see set_residual_variant
-    set_residual_variant(variant - 60);
-    return UNC_OK;
-  }
-  if (variant >= 50 && variant <= 52) {  // Schur reduction: by size / 64-pair / 32-pair chunks
-    set_schur_variant(variant - 50);
-    return UNC_OK;
-  }
-  if (variant < 0 || variant > 3) return UNC_ERR_BAD_INPUT;
-  set_gemm_variant(variant);
-  return UNC_OK;
-}
-
-int unc_diag_inverse(int n, const double *A, double *X, int refine) {
-  if (n <= 0) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n;
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf,
-               sizeof(double) * (5 * nn + (size_t)GJ_BLOCK * (3 * GJ_BLOCK + 2 * n)) + 256));
-  double *dA = (double *)buf, *dW = dA + nn, *dR = dW + nn, *dD = dR + nn, *dX = dD + nn;
-  double *P = dX + nn, *T = P + 3 * GJ_BLOCK * GJ_BLOCK, *Rp = T + (size_t)GJ_BLOCK * n;
-  unsigned int *err = (unsigned int *)(Rp + (size_t)GJ_BLOCK * n);
-  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
-  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(dW, dA, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
-  rc = gj_inverse(n, dW, P, T, Rp, err, s);
-  if (rc) { (void)hipFree(buf); return rc; }
-  double *res = dW;
-  if (refine) {
-    HC(residual_dd(n, dA, dW, dR, dD, s));  // dD is free until the product
-    HC(gemm_f64(n, n, n, 1.0, dW, n, dR, n, 0.0, dD, n, nullptr, nullptr, s));
-    HC(sym_update(n, dW, dD, dX, s));
-    res = dX;
-  }
-  unsigned int herr = 0;
-  HC(hipMemcpyAsync(X, res, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  return herr ? map_err(herr) : UNC_OK;
-}
-
-int unc_diag_chol_inverse(int n, const double *A, double *L, double *X) {
-  if (n <= 0 || !A || !L || !X) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n;
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * nn + 256));
-  double *dA = (double *)buf, *dX = dA + nn, *dW = dX + nn;
-  unsigned int *err = (unsigned int *)(dW + nn);
-  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
-  HC(hipMemsetAsync(dX, 0, sizeof(double) * nn, s));
-  HC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  rc = chol_inv(n, dA, dX, dW, 0, n, err, s);
-  if (rc) { (void)hipFree(buf); return rc; }
-  unsigned int herr = 0;
-  HC(hipMemcpyAsync(L, dA, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(X, dX, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  return herr ? map_err(herr) : UNC_OK;
-}
-
-int unc_diag_leaf_bench(int reps, int variant, double *avg_us) {
-  if (reps <= 0 || !avg_us || variant < 0 || variant > 4) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const int n = 64;
-  std::vector<double> h((size_t)n * n);
-  for (int c = 0; c < n; ++c)
-    for (int r = 0; r < n; ++r) h[r + (size_t)c * n] = (r == c) ? n + 1.0 : 1.0 / (1.0 + r + c);
-  char *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * 3 * n * n + 256));
-  double *dA = (double *)buf, *dW = dA + n * n, *dX = dW + n * n;
-  unsigned int *err = (unsigned int *)(dX + n * n);
-  HC(hipMemcpyAsync(dA, h.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
-  HC(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
-  set_leaf_variant(variant <= 2 ? variant : 0);
-  hipEvent_t e0 = ctx->stage_ev[0], e1 = ctx->stage_ev[1];
-  auto one = [&]() -> hipError_t {
-    if (variant == 3) return spd_inverse64(n, dA, n, dX, err, s);
-    if (variant == 4) return gj_small_inverse(n, dA, n, dX, err, s);
-    hipError_t e = hipMemcpyAsync(dW, dA, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s);
-    return e != hipSuccess ? e : chol_inv_leaf(n, dW, n, dX, n, err, s);
-  };
-  HC(one());
-  HC(hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r) HC(one());
-  HC(hipEventRecord(e1, s));
-  HC(hipStreamSynchronize(s));
-  set_leaf_variant(0);
-  *avg_us = 1e3 * ev_ms(e0, e1) / reps;
-  HC(hipFree(buf));
-  return UNC_OK;
-}
-
-int unc_diag_residual_dd(int n, const double *A, const double *X, double *R) {
-  if (n <= 0) return UNC_ERR_BAD_INPUT;
-  int rc = UNC_OK;
-  unc_context *ctx = default_context(0, &rc);
-  if (!ctx) return rc;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  HC(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const size_t nn = (size_t)n * n;
-  double *buf = nullptr;
-  HC(hipMalloc((void **)&buf, sizeof(double) * (3 * nn + 2 * (size_t)n)));
-  HC(hipMemcpyAsync(buf, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(buf + nn, X, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  HC(residual_dd(n, buf, buf + nn, buf + 2 * nn, buf + 3 * nn, s));
-  HC(hipMemcpyAsync(R, buf + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  HC(hipFree(buf));
-  return UNC_OK;
-}
-
 }  // extern "C"
+
+// ---- entry points of the internal algorithms for the diagnostics library
+// (uncertaintyte_amd/csrc/diag.hip), which tests kernels one at a time
+namespace unc {
+int dense_inverse_gj(int n, double *W, double *P3, double *T, double *R, unsigned int *err,
+                     hipStream_t s) {
+  return gj_inverse(n, W, P3, T, R, err, s);
+}
+int cholesky_inverse(int n, double *A, double *X, double *W, unsigned int *err, hipStream_t s) {
+  return chol_inv(n, A, X, W, 0, n, err, s);
+}
+int error_code(unsigned int device_error_bits) { return map_err(device_error_bits); }
+}  // namespace unc

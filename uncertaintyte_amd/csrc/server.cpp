@@ -126,8 +126,17 @@ struct unc_server {
       lk.lock();
       --running;
       me.busy = false;
+      // the admitted estimate is the layout the run allocates; should the
+      // context hold more than was admitted, its buffers are dropped so that
+      // the budget holds between requests too
+      const size_t actual = unc_ctx_held_bytes(me.ctx);
       held_total -= me.held;
-      me.held = unc_ctx_held_bytes(me.ctx);
+      if (actual > me.held) {
+        unc_ctx_release(me.ctx);
+        me.held = 0;
+      } else {
+        me.held = actual;
+      }
       held_total += me.held;
       if (held_total > peak_held) peak_held = held_total;
       r->status = rc;
@@ -211,13 +220,13 @@ int unc_server_wait(unc_server *s, long long ticket, int *status, double *latenc
   if (!s) return UNC_ERR_BAD_INPUT;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return UNC_ERR_BAD_INPUT;
+  if (it == s->reqs.end()) return UNC_ERR_BAD_INPUT;  // unknown or already claimed
   unc_server::Req *r = it->second;
+  s->reqs.erase(it);  // claimed: a second waiter on this ticket gets BAD_INPUT
   s->cv_done.wait(lk, [&] { return r->done; });
   if (status) *status = r->status;
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->end - r->arrival).count();
   if (queue_ms) *queue_ms = std::chrono::duration<double, std::milli>(r->start - r->arrival).count();
-  s->reqs.erase(it);
   delete r;
   return UNC_OK;
 }

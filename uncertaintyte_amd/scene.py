@@ -292,21 +292,24 @@ def walk_scene(n_cams=1500, n_pts=1_000_000, m=5.0, seed=3, short_frac=0.10, win
     cams = _camera_params(rng, C, R)
     n_short = int(round(short_frac * n_pts))
     n_norm = n_pts - n_short
-    # normal points: anchor camera s, depth 2..12 ahead, within the field of view
-    s = rng.integers(window, n_cams, n_norm)
+    # normal points: anchor camera s, depth 2..12 ahead, within the field of view,
+    # seen from the window of cameras s - window + 1 .. s (those that exist)
+    s = rng.integers(1, n_cams, n_norm)
     depth = rng.uniform(2.0, 12.0, n_norm)
     lat = rng.uniform(-0.35, 0.35, (n_norm, 2)) * depth[:, None]
     Rs = R[s]
     X1 = C[s] + depth[:, None] * Rs[:, 2] + lat[:, :1] * Rs[:, 0] + lat[:, 1:] * Rs[:, 1]
     cand1 = s[:, None] - np.arange(window)[None, :]
+    cand1[cand1 < 0] = -1
     # short-baseline points: far (depth 40..80), seen by 2..4 consecutive cameras
-    s2 = rng.integers(8, n_cams, n_short)
+    s2 = rng.integers(1, n_cams, n_short)
     depth2 = rng.uniform(40.0, 80.0, n_short)
     lat2 = rng.uniform(-0.3, 0.3, (n_short, 2)) * depth2[:, None]
     Rs2 = R[s2]
     X2 = C[s2] + depth2[:, None] * Rs2[:, 2] + lat2[:, :1] * Rs2[:, 0] + lat2[:, 1:] * Rs2[:, 1]
     cand2 = -np.ones((n_short, window), np.int64)
     cand2[:, :4] = s2[:, None] - np.arange(4)[None, :]
+    cand2[cand2 < 0] = -1
     X = np.concatenate([X1, X2])
     oc1, op1 = _tracks_uniform(rng, n_norm, cand1, m)
     oc2, op2 = _tracks_uniform(rng, n_short, cand2[:, :4], 0, lo=2, hi=4)
