@@ -43,6 +43,7 @@ extern "C" {
 #define UNC_ERR_DEVICE 5           /* HIP runtime error                                  */
 #define UNC_ERR_UNSUPPORTED 6      /* option combination not implemented on the device   */
 #define UNC_ERR_NUMERIC 7          /* NaN/Inf in the result                              */
+#define UNC_ERR_TOO_LARGE 8        /* workspace beyond UNC_NOALLOC_SCRATCH_BYTES           */
 
 /* cov::Options::_algorithm values (include/uncertainty.h:11-13) */
 #define UNC_ALG_SVD_QR_ITERATION 0
@@ -142,6 +143,23 @@ int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows,
                        int num_cols, const int *d_rows, const int *d_cols,
                        const double *d_values, double *d_cam_unc,
                        double *d_pt_unc, unc_stats *stats, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): unc_compute_device with every device buffer of the solve
+ * carved from a fixed scratch area of UNC_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each device it runs on; the call allocates nothing.
+ * ctx must be a context from unc_context_create (whose creation takes the
+ * scratch's device address).  A solve whose unc_workspace_bytes() exceeds the
+ * scratch returns UNC_ERR_TOO_LARGE (a message names both sizes).  All calls
+ * on one device share its scratch and find the last call's contents there
+ * (the solve initialises everything it reads): calls that overlap on one GPU
+ * race and must be serialised by the caller.  TAYLOR_EXPANSION only. */
+#define UNC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int unc_compute_device_noalloc(unc_context *ctx, const unc_options *opts, int num_rows,
+                               int num_cols, const int *d_rows, const int *d_cols,
+                               const double *d_values, double *d_cam_unc,
+                               double *d_pt_unc, unc_stats *stats, void *stream);
 
 /* unc_compute plus host copies of intermediates (parity tests). */
 int unc_compute_debug(unc_context *ctx, const unc_options *opts, int num_rows,

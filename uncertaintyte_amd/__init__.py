@@ -37,6 +37,8 @@ UNC_ERR_OOM = 4
 UNC_ERR_DEVICE = 5
 UNC_ERR_UNSUPPORTED = 6
 UNC_ERR_NUMERIC = 7
+UNC_ERR_TOO_LARGE = 8
+NOALLOC_SCRATCH_BYTES = 64 << 20
 PREC_FP64, PREC_MIXED, PREC_ZPLUS_FP64 = 0, 1, 2
 SVD_QR_ITERATION, SVD_DEVIDE_AND_CONQUER, TAYLOR_EXPANSION = 0, 1, 2
 
@@ -99,6 +101,9 @@ def lib():
                               vp, vp, ctypes.POINTER(Stats)]
     L.unc_compute_device.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int, ctypes.c_int, vp,
                                      vp, vp, vp, vp, ctypes.POINTER(Stats), vp]
+    L.unc_compute_device_noalloc.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int,
+                                             ctypes.c_int, vp, vp, vp, vp, vp,
+                                             ctypes.POINTER(Stats), vp]
     L.unc_compute_debug.argtypes = [vp, ctypes.POINTER(Options), ctypes.c_int, ctypes.c_int, vp, vp,
                                     vp, vp, vp, ctypes.POINTER(Stats), ctypes.POINTER(Debug)]
     L.unc_workspace_bytes.argtypes = [ctypes.POINTER(Options), ctypes.c_int]

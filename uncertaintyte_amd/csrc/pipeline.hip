@@ -636,12 +636,55 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   return UNC_OK;
 }
 
+// ---- fixed scratch of the no-allocation entry point.  One array per device
+// (each GPU the library runs on holds it until the process ends).  Host code
+// never uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device when its first context is created
+// (the library's set-up call), and is kept per device.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[UNC_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+int scratch_setup(int device) {  // with `device` current
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return UNC_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    std::fprintf(stderr, "uncertainty_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+                 device, hipGetErrorString(e));
+    (void)hipGetLastError();
+    return UNC_ERR_DEVICE;
+  }
+  g_scratch_ptr[device] = (char *)p;
+  return UNC_OK;
+}
+
+char *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+int context_init(unc_context *c) {
+  HC(hipSetDevice(c->device));
+  HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) ==
+          hipSuccess &&
+      lds > 0)
+    c->max_lds = lds;
+  for (auto &e : c->stage_ev) HC(hipEventCreate(&e));
+  for (auto &e : c->gemm_ev) HC(hipEventCreate(&e));
+  return scratch_setup(c->device);
+}
+
 std::mutex g_default_mu;
-unc_context *g_default[64] = {nullptr};
+unc_context *g_default[kMaxDevices] = {nullptr};
 
 unc_context *default_context(int device, int *rc) {
   std::lock_guard<std::mutex> lk(g_default_mu);
-  if (device < 0 || device >= 64) {
+  if (device < 0 || device >= kMaxDevices) {
     *rc = UNC_ERR_BAD_INPUT;
     return nullptr;
   }
@@ -757,17 +800,16 @@ int unc_context_create(int device, unc_context **out) {
     (void)hipGetLastError();
     return UNC_ERR_DEVICE;
   }
-  if (device < 0 || device >= ndev) return UNC_ERR_BAD_INPUT;
+  if (device < 0 || device >= ndev || device >= kMaxDevices) return UNC_ERR_BAD_INPUT;
   unc_context *c = new unc_context();
   c->device = device;
-  HC(hipSetDevice(device));
-  HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  int lds = 0;
-  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
-      lds > 0)
-    c->max_lds = lds;
-  for (auto &e : c->stage_ev) HC(hipEventCreate(&e));
-  for (auto &e : c->gemm_ev) HC(hipEventCreate(&e));
+  for (auto &e : c->stage_ev) e = nullptr;
+  for (auto &e : c->gemm_ev) e = nullptr;
+  const int rc = context_init(c);
+  if (rc) {
+    unc_context_destroy(c);
+    return rc;
+  }
   *out = c;
   return UNC_OK;
 }
@@ -776,8 +818,10 @@ void unc_context_destroy(unc_context *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &e : c->stage_ev) (void)hipEventDestroy(e);
-  for (auto &e : c->gemm_ev) (void)hipEventDestroy(e);
+  for (auto &e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->gemm_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->ws) (void)hipFreeAsync(c->ws, c->stream);
   if (c->io) (void)hipFreeAsync(c->io, c->stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -811,6 +855,34 @@ int unc_compute_device(unc_context *ctx, const unc_options *opts, int num_rows, 
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
   return run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_values, d_cam_unc, d_pt_unc, stats,
              nullptr, s, false, nullptr);
+}
+
+int unc_compute_device_noalloc(unc_context *ctx, const unc_options *opts, int num_rows,
+                               int num_cols, const int *d_rows, const int *d_cols,
+                               const double *d_values, double *d_cam_unc, double *d_pt_unc,
+                               unc_stats *stats, void *stream) {
+  if (!ctx || !opts || !d_rows || !d_cols || !d_values || !d_cam_unc || !d_pt_unc)
+    return UNC_ERR_BAD_INPUT;
+  if (opts->algorithm != UNC_ALG_TAYLOR_EXPANSION) return UNC_ERR_UNSUPPORTED;
+  const unsigned long long need = unc_workspace_bytes(opts, num_rows);
+  if (need == 0) return UNC_ERR_BAD_INPUT;
+  if (need > UNC_NOALLOC_SCRATCH_BYTES) {
+    std::fprintf(stderr,
+                 "uncertainty_amd: unc_compute_device_noalloc: this solve needs %llu bytes of "
+                 "device workspace, the fixed scratch holds %llu\n",
+                 need, (unsigned long long)UNC_NOALLOC_SCRATCH_BYTES);
+    return UNC_ERR_TOO_LARGE;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HC(hipSetDevice(ctx->device));
+  char *base = scratch_of(ctx->device);
+  if (!base) return UNC_ERR_DEVICE;
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  Workspace ws;
+  ws.base = base;
+  ws.bytes = UNC_NOALLOC_SCRATCH_BYTES;
+  return run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_values, d_cam_unc, d_pt_unc, stats,
+             nullptr, s, false, &ws);
 }
 
 int unc_eig_cameras_device(unc_context *ctx, int num_cams, int cam_params,
@@ -1006,6 +1078,7 @@ const char *unc_error_string(int code) {
     case UNC_ERR_DEVICE: return "HIP runtime error";
     case UNC_ERR_UNSUPPORTED: return "unsupported option or layout";
     case UNC_ERR_NUMERIC: return "non-finite result";
+    case UNC_ERR_TOO_LARGE: return "the solve needs more device workspace than the fixed scratch of the no-allocation entry point";
     default: return "unknown error";
   }
 }
