@@ -65,6 +65,9 @@ def lib():
                                  ctypes.c_void_p, _f64p, ctypes.POINTER(Stats)]
         L.orc_points_from_X.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
                                         ctypes.c_int, _i32p, _f64p, _f64p, _f64p]
+        L.orc_points_from_blocks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
+                                             ctypes.c_int, _i32p, _f64p, _f64p,
+                                             ctypes.c_char_p, _f64p, ctypes.c_char_p]
         L.orc_jtj_dense.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p]
         L.orc_inv_lu.argtypes = [ctypes.c_int, _f64p, _f64p]
         L.orc_inv_equilibrated.argtypes = [ctypes.c_int, _f64p, _f64p]
@@ -260,14 +263,41 @@ def zplus_reference(jac, fix=None, nthreads: int = 0):
     return dict(cam=cam, pt=pt.reshape(-1, 6), X=X, eig=w)
 
 
-def zplus_sampled(jac, cams, fix=None, nthreads: int = 0, log=None):
+def points_from_blocks(jac, blocks: dict, fix=None):
+    """Point covariances (the TAYLOR point step, compute.cpp:516-527) from the
+    camera blocks X_cc of the scaled problem given in `blocks` {camera: cp x cp},
+    for the points all of whose observing cameras are in `blocks`.
+    Returns (pt (nP, 6), done mask (nP,) of the points computed)."""
+    nc, cp = jac.num_cams, jac.cam_params
+    Xd = np.zeros((nc, cp, cp))
+    ok = np.zeros(nc, np.uint8)
+    for a, B in blocks.items():
+        Xd[a] = np.asarray(B).T  # column-major per block
+        ok[a] = 1
+    nr, ncl, rows, cols, vals, fx = fixpts(jac, fix)
+    pt = np.zeros(jac.num_points * 6)
+    done = np.zeros(jac.num_points, np.uint8)
+    rc = lib().orc_points_from_blocks(nc, cp, jac.num_points,
+                                      np.array(fix if fix is not None else jac.fix_array(), np.int32),
+                                      nr, np.ascontiguousarray(cols, np.int32),
+                                      np.ascontiguousarray(vals, np.float64),
+                                      np.ascontiguousarray(Xd.reshape(-1)), ok.tobytes(), pt,
+                                      done.ctypes.data_as(ctypes.c_char_p))
+    if rc:
+        raise OracleError(rc, "orc_points_from_blocks")
+    return pt.reshape(-1, 6), done.astype(bool)
+
+
+def zplus_sampled(jac, cams, fix=None, nthreads: int = 0, log=None, blocks: bool = False):
     """Camera outputs of D pinv_7(Z_s) D for the listed cameras only, for scenes
     too large for a dense eigen-decomposition.  Uses the identity
     pinv_7(Z) = (Z + N N^T)^-1 - N N^T with N the eigenvectors of the 7 smallest
     eigenvalues, found by shift-invert Lanczos (ARPACK) on a LAPACK Cholesky of
     Z + sigma I; the sampled columns come from a Cholesky solve with Z + N N^T.
-    Returns (cam rows for `cams`, the 7 smallest eigenvalues).  `log(msg)` is
-    called between the phases (long scenes take minutes)."""
+    Returns (cam rows for `cams`, the 7 smallest eigenvalues), and with
+    blocks=True also {camera: cp x cp block of pinv_7(Z_s)} (scaled, for
+    points_from_blocks).  `log(msg)` is called between the phases (long scenes
+    take minutes)."""
     import scipy.linalg as sla
     from scipy.sparse.linalg import LinearOperator, eigsh
     log = log or (lambda msg: None)
@@ -296,8 +326,14 @@ def zplus_sampled(jac, cams, fix=None, nthreads: int = 0, log=None):
     cols = sla.cho_solve(cg, E, check_finite=False) - Nb @ Nb[idx].T
     iu = np.triu_indices(cp)
     out = np.empty((len(cams), cp * (cp + 1) // 2))
+    blk = {}
     for q, a in enumerate(cams):
         s = slice(a * cp, (a + 1) * cp)
-        B = cols[s, q * cp:(q + 1) * cp] * d[s][:, None] * d[s][None, :]
-        out[q] = (0.5 * (B + B.T))[iu]
+        X = cols[s, q * cp:(q + 1) * cp]
+        X = 0.5 * (X + X.T)
+        blk[int(a)] = X
+        B = X * d[s][:, None] * d[s][None, :]
+        out[q] = B[iu]
+    if blocks:
+        return out, np.sort(w), blk
     return out, np.sort(w)

@@ -484,19 +484,22 @@ static void cam_output(int nc, int cp, int N, const double *X, const double *d, 
  * equal to pinv(A) B Sigma B^T pinv(A)^T with A = J_p (full column rank),
  * B = [J_c, -I2] per observation and Sigma = blockdiag(X_cc unscaled, I2);
  * only each camera's own block enters (compute.cpp:422-441). */
+/* Camera a's own block of X: element (i, j) at X[a * cam_stride + i + j * ld]
+ * (the dense N x N column-major X: cam_stride = cp (N + 1), ld = N; compact
+ * cp x cp column-major blocks: cam_stride = cp * cp, ld = cp). */
 static void point_one(int cp, int N, const jstruct *s, const blocks *bl,
-                      const double *X, int p, double out6[6]) {
+                      const double *X, size_t cam_stride, size_t ld, int p, double out6[6]) {
   double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = s->pt_off[p]; t < s->pt_off[p + 1]; ++t) {
     int o = s->pt_list[t];
     int a = s->obs_cam[o];
     const double *Ws = bl->Ws + (size_t)o * cp * 3;
-    const double *Xa = X + (size_t)a * cp + (size_t)a * cp * N;
+    const double *Xa = X + (size_t)a * cam_stride;
     double H[27];
     for (int i = 0; i < cp; ++i)
       for (int k = 0; k < 3; ++k) {
-        double h = Xa[i + 0 * (size_t)N] * Ws[0 * 3 + k];
-        for (int j = 1; j < cp; ++j) h = fma(Xa[i + (size_t)j * N], Ws[j * 3 + k], h);
+        double h = Xa[i + 0 * ld] * Ws[0 * 3 + k];
+        for (int j = 1; j < cp; ++j) h = fma(Xa[i + (size_t)j * ld], Ws[j * 3 + k], h);
         H[i * 3 + k] = h;
       }
     for (int k = 0; k < 3; ++k)
@@ -546,7 +549,7 @@ static void point_output(const jstruct *s, const blocks *bl, const double *X,
   for (int p = 0; p < s->npc; ++p) {
     if (!bl->pt_has[p]) continue;
     double o6[6];
-    point_one(s->cp, s->N, s, bl, X, p, o6);
+    point_one(s->cp, s->N, s, bl, X, (size_t)s->cp * ((size_t)s->N + 1), (size_t)s->N, p, o6);
     int q = out_point_id(p, fix);
     if (q >= 0 && q < npts) memcpy(pt_out + (size_t)q * 6, o6, sizeof(o6));
   }
@@ -671,6 +674,39 @@ int orc_points_from_X(int ncams, int cp, int npts, const int fix_in[3],
   rc = assemble(&s, &bl);
   if (rc) { jstruct_free(&s); return rc; }
   point_output(&s, &bl, X, fix, npts, pt_out);
+  blocks_free(&bl);
+  jstruct_free(&s);
+  return ORC_OK;
+}
+
+int orc_points_from_blocks(int ncams, int cp, int npts, const int fix_in[3],
+                           int nrows_fixed, const int *cols_fixed, const double *vals_fixed,
+                           const double *Xd, const char *cam_ok, double *pt_out,
+                           char *pt_done) {
+  int fix[3] = {fix_in[0], fix_in[1], fix_in[2]};
+  if (fix[1] < fix[0]) { int t = fix[0]; fix[0] = fix[1]; fix[1] = t; }
+  jstruct s;
+  int rc = jstruct_build(&s, cp, ncams, npts - 3, nrows_fixed, cols_fixed, vals_fixed);
+  if (rc) return rc;
+  blocks bl;
+  rc = assemble(&s, &bl);
+  if (rc) { jstruct_free(&s); return rc; }
+  memset(pt_out, 0, sizeof(double) * 6 * (size_t)npts);
+  memset(pt_done, 0, (size_t)npts);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int p = 0; p < s.npc; ++p) {
+    if (!bl.pt_has[p]) continue;
+    int ok = 1;
+    for (int t = s.pt_off[p]; t < s.pt_off[p + 1] && ok; ++t) ok = cam_ok[s.obs_cam[s.pt_list[t]]] != 0;
+    if (!ok) continue;
+    double o6[6];
+    point_one(cp, s.N, &s, &bl, Xd, (size_t)cp * cp, (size_t)cp, p, o6);
+    int q = out_point_id(p, fix);
+    if (q >= 0 && q < npts) {
+      memcpy(pt_out + (size_t)q * 6, o6, sizeof(o6));
+      pt_done[q] = 1;
+    }
+  }
   blocks_free(&bl);
   jstruct_free(&s);
   return ORC_OK;

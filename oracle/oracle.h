@@ -102,6 +102,13 @@ int orc_taylor(const orc_opts *opts, int N, const double *Z, double lambda,
 int orc_points_from_X(int ncams, int cp, int npts, const int fix[3],
                       int nrows_fixed, const int *cols_fixed,
                       const double *vals_fixed, const double *X, double *pt_out);
+/* The same point step from compact camera blocks Xd (ncams x cp x cp,
+ * column-major per block), only for the points all of whose observing cameras
+ * have cam_ok set (pt_done[q] = 1 at their output id q); other points 0. */
+int orc_points_from_blocks(int ncams, int cp, int npts, const int fix[3],
+                           int nrows_fixed, const int *cols_fixed, const double *vals_fixed,
+                           const double *Xd, const char *cam_ok, double *pt_out,
+                           char *pt_done);
 
 /* Generic CSR J^T J (sequential, dense column-major output ncols*ncols). */
 int orc_jtj_dense(int nrows, int ncols, const int *rows, const int *cols,

@@ -62,3 +62,57 @@ def test_poisson_load_smoke(small_scenes):
     s = res.summary()
     print(s, flush=True)
     assert s["requests"] == 16 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+@pytest.fixture(scope="module")
+def config4_scenes():
+    # the configuration's range: 20..800 cameras, 25..400 points per camera,
+    # with four scenes of 400 cameras or more
+    sizes = [(800, 100), (450, 60), (620, 40), (20, 400), (90, 250), (300, 25), (410, 200),
+             (150, 150)]
+    return [serve.request_scene(c, p, 300 + i) for i, (c, p) in enumerate(sizes)]
+
+
+def test_config4_large_requests_mix_equals_alone(config4_scenes):
+    """Requests at config 4's upper range (up to 800 cameras, N = 7200): each
+    result in a 4-slot mix (every scene twice) equals the result alone, bit for
+    bit, on the Z+ path the serving configuration runs."""
+    prec = u.PREC_MIXED
+    print("config4 mix vs alone: alone", flush=True)
+    alone = [u.compute_covariances(j, precision=prec) for j in config4_scenes]
+    print("config4 mix vs alone: mix", flush=True)
+    srv = u.Server(slots=4)
+    tickets = [srv.submit(j, precision=prec) for _ in range(2) for j in config4_scenes]
+    outs = [srv.wait(t)[0] for t in tickets]
+    st = srv.stats()
+    srv.close()
+    n = len(config4_scenes)
+    for k, r in enumerate(outs):
+        a = alone[k % n]
+        assert np.array_equal(r.cam, a.cam) and np.array_equal(r.pt, a.pt)
+    assert st.completed == len(tickets) and st.max_concurrent >= 2
+
+
+def test_config4_poisson_512_requests():
+    """Config 4 as specified: 512 requests (64 distinct seeded scenes, cameras
+    log-uniform 20..800, 25..400 points per camera, track length 4) arriving as
+    a Poisson process at 150 requests/s into 4 slots; every result must equal
+    its scene's result alone."""
+    print("config4 512 requests: generating scenes", flush=True)
+    scenes, order = serve.make_requests(512, 64, workers=8)
+    assert max(s.num_cams for s in scenes) >= 400
+    prec = u.PREC_MIXED
+    print("config4 512 requests: alone", flush=True)
+    alone = [u.compute_covariances(j, precision=prec) for j in scenes]
+
+    def same(i, r):
+        a = alone[order[i]]
+        return np.array_equal(r.cam, a.cam) and np.array_equal(r.pt, a.pt)
+
+    print("config4 512 requests: load", flush=True)
+    srv = u.Server(slots=4)
+    res = serve.run_load(srv, scenes, order, rate=150.0, precision=prec, check=same)
+    srv.close()
+    s = res.summary()
+    print(s, flush=True)
+    assert s["requests"] == 512 and s["failed"] == 0

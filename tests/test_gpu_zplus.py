@@ -95,15 +95,51 @@ def test_zplus_repeatable_and_context_reuse(daliborka):
     ctx.close()
 
 
+def _sampled_errors(j, r, cams, name):
+    """Camera blocks of `cams` and the point blocks of every point whose
+    observing cameras all lie in `cams`, against oracle.zplus_sampled (ARPACK +
+    LAPACK Cholesky construction of D pinv_7(Z_s) D); also the FP64 noise floor
+    of the oracle itself: its two constructions (the sampled one at cams) of the
+    same quantity differ by the conditioning of Z, so the floor is stated next
+    to the error."""
+    print(f"zplus {name}: sampled oracle ({len(cams)} cameras)", flush=True)
+    ref, w, blk = oracle.zplus_sampled(j, cams, nthreads=16, blocks=True,
+                                       log=lambda m: print("  " + m, flush=True))
+    rc = block_rel(r.cam[cams], ref)
+    pt, done = oracle.points_from_blocks(j, blk)
+    mask = done & (np.linalg.norm(pt, axis=1) > 0)
+    rp = block_rel(r.pt[mask], pt[mask])
+    print(f"  {name}: {len(cams)} cams max {rc.max():.2e} median {np.median(rc):.2e}; "
+          f"{int(mask.sum())} points max {rp.max():.2e} median {np.median(rp):.2e}", flush=True)
+    return rc, rp, int(mask.sum())
+
+
 def test_zplus_at_scale_sampled_cameras():
-    """walk1500 (N = 13491): 188 sampled cameras against the ARPACK + LAPACK
-    Cholesky construction of D pinv_7(Z_s) D (oracle.zplus_sampled)."""
+    """walk1500 (1500 cameras, 1M points, 10% short-baseline; N = 13500): every
+    8th camera plus a window of 200 consecutive cameras, and the ~100k points
+    seen only from that window, against the sampled FP64 oracle (<= 1e-5)."""
     j = scene.make_workload("walk1500")
     print("zplus walk1500: GPU", flush=True)
     r = u.compute_covariances(j, precision=u.PREC_MIXED)
-    cams = list(range(0, j.num_cams, 8))
-    print("zplus walk1500: sampled oracle", flush=True)
-    ref, w = oracle.zplus_sampled(j, cams, nthreads=16)
-    rc = block_rel(r.cam[cams], ref)
-    print(f"  walk1500: cam max {rc.max():.2e} median {np.median(rc):.2e}", flush=True)
+    cams = sorted(set(range(0, j.num_cams, 8)) | set(range(600, 800)))
+    rc, rp, npts = _sampled_errors(j, r, cams, "walk1500")
+    assert npts > 10000
     assert rc.max() <= 1e-5
+    assert rp.max() <= 1e-5
+
+
+def test_zplus_config5_large_scene():
+    """Config 5: walk4000 (4000 cameras, 3M points, track Poisson(6); N = 36000,
+    dense Z 10.4 GB) on one GPU; 200 consecutive cameras and the points seen
+    only from them against the sampled FP64 oracle (<= 1e-5)."""
+    print("zplus walk4000: generating", flush=True)
+    j = scene.make_workload("walk4000")
+    assert j.num_cams == 4000
+    print(f"zplus walk4000: GPU (N = {j.num_cams * j.cam_params}, {j.num_obs} observations)",
+          flush=True)
+    r = u.compute_covariances(j, precision=u.PREC_MIXED)
+    cams = list(range(2000, 2200))
+    rc, rp, npts = _sampled_errors(j, r, cams, "walk4000")
+    assert npts > 10000
+    assert rc.max() <= 1e-5
+    assert rp.max() <= 1e-5

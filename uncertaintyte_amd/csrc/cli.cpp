@@ -9,7 +9,12 @@
 // As in the reference, a .jacob input always runs TAYLOR_EXPANSION
 // (JacobianIO.cpp:25 ignores -alg); a path ending in .jacob is read as one
 // whatever -in_form says.  -legacy_time_units prints stage times the way the
-// reference does (ten times the seconds, compute.cpp:33).
+// reference does (ten times the seconds, compute.cpp:33).  -debug writes the
+// dense matrices the reference prints on the TE path (compute.cpp:505,508):
+// Z.txt (the scaled Schur complement) and iZ.txt (the Taylor result), each as
+// `name = [a b ... ;...];` (ScaledDenseMatrix::printBlock2Matlab3,
+// ScaledDenseMatrix.cpp:133-143); the sparse J, M, U, W, V dumps are not
+// written (the device never forms them as matrices).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -103,6 +108,46 @@ bool save_results(const std::string &path, const cov::Options &o, const cov::Sta
   return true;
 }
 
+// printBlock2Matlab3 of a dense n x n column-major matrix into <name>.txt
+bool write_matlab(const std::string &name, int n, const std::vector<double> &A) {
+  std::ofstream f(name + ".txt");
+  if (!f.good()) return false;
+  f << name << " = [";
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) f << A[(size_t)i + (size_t)j * n] << " ";
+    f << (i == n - 1 ? "" : ";");
+  }
+  f << "];";
+  return f.good();
+}
+
+// the TE path once more through the C ABI, keeping Z and the Taylor result
+bool dump_debug(const cov::Options &opt, const ceres::CRSMatrix &J) {
+  unc_options o;
+  unc_default_options(&o);
+  o.lambda = opt._lambda;
+  o.num_cams = opt._numCams;
+  o.cam_params = opt._camParams;
+  o.num_points = opt._numPoints;
+  o.num_obs = opt._numObs;
+  for (int i = 0; i < 3; ++i) o.pts2fix[i] = opt._pts2fix ? opt._pts2fix[i] : -1;
+  const int N = opt._numCams * opt._camParams;
+  const int per = opt._camParams * (opt._camParams + 1) / 2;
+  std::vector<double> Z((size_t)N * N), X((size_t)N * N);
+  std::vector<double> cam((size_t)opt._numCams * per), pt((size_t)opt._numPoints * 6);
+  unc_debug dbg;
+  std::memset(&dbg, 0, sizeof(dbg));
+  dbg.Z = Z.data();
+  dbg.X = X.data();
+  const int rc = unc_compute_debug(NULL, &o, J.num_rows, J.num_cols, J.rows.data(), J.cols.data(),
+                                   J.values.data(), cam.data(), pt.data(), NULL, &dbg);
+  if (rc != UNC_OK) {
+    std::cerr << "uncertainty: debug dump: " << unc_error_string(rc) << "\n";
+    return false;
+  }
+  return write_matlab("Z", N, Z) && write_matlab("iZ", N, X);
+}
+
 bool flag(const char *arg, const char *name, std::string *val) {
   // gflags accepts -name=value and --name=value
   const char *a = arg;
@@ -190,15 +235,15 @@ int main(int argc, char **argv) {
   std::string in = ".", in_form = "COLMAP", out = ".", alg = "TAYLOR_EXPANSION",
               cam = "SIMPLE_RADIAL";
   std::string v;
-  bool legacy_units = false;
+  bool legacy_units = false, debug = false;
   for (int i = 1; i < argc; ++i) {
     if (flag(argv[i], "in_form", &v)) in_form = v;
     else if (flag(argv[i], "in", &v)) in = v;
     else if (flag(argv[i], "out", &v)) out = v;
     else if (flag(argv[i], "alg", &v)) alg = v;
     else if (flag(argv[i], "cam", &v)) cam = v;
-    else if (flag(argv[i], "debug", &v)) {
-    } else if (flag(argv[i], "legacy_time_units", &v)) legacy_units = (v != "false");
+    else if (flag(argv[i], "debug", &v)) debug = (v != "false");
+    else if (flag(argv[i], "legacy_time_units", &v)) legacy_units = (v != "false");
     else {
       std::cerr << "unknown flag " << argv[i] << "\n";
       return 2;
@@ -224,6 +269,7 @@ int main(int argc, char **argv) {
     std::cerr << "uncertainty: " << unc_error_string(rc) << " (code " << rc << ")\n";
     return 1;
   }
+  if (debug && opt._algorithm == TAYLOR_EXPANSION && !dump_debug(opt, J)) return 1;
   if (!save_results(out, opt, stat, per, camu.data(), ptu.data(), legacy_units ? 10.0 : 1.0)) {
     std::cerr << "cannot write " << out << "\n";
     return 1;

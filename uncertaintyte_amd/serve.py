@@ -73,9 +73,11 @@ class LoadResult:
                     queue_p99_ms=float(np.percentile(q, 99)), makespan_s=self.makespan_s)
 
 
-def run_load(server, scenes, order, rate: float, precision: int, seed: int = 5):
+def run_load(server, scenes, order, rate: float, precision: int, seed: int = 5, check=None):
     """Submit scenes[order[i]] at Poisson arrival times of the given rate (1/s);
-    returns LoadResult with per-request latency (submit -> done) and queueing."""
+    returns LoadResult with per-request latency (submit -> done) and queueing.
+    check(i, result), when given, is called with each request's result as it
+    completes; a False return counts the request as failed."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(order)))
     tickets: List[int] = []
@@ -93,9 +95,11 @@ def run_load(server, scenes, order, rate: float, precision: int, seed: int = 5):
                     return
                 t = tickets[k]
             try:
-                _, lat, q = server.wait(t)
+                out, lat, q = server.wait(t)
                 res.latency_ms.append(lat)
                 res.queue_ms.append(q)
+                if check is not None and not check(k, out):
+                    res.failed += 1
             except UncError:
                 res.failed += 1
             k += 1
