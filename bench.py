@@ -258,6 +258,64 @@ def gemm_traffic():
     return None
 
 
+def cpu_reference_arithmetic(J, threads):
+    """The reference's TE arithmetic on the host, timed: fixPts, J^T J blocks,
+    3x3 inverses and the Schur complement (the C oracle, OpenMP, `threads`),
+    then SDM::inv -- row and column 2-norm equilibration, LAPACK dgetrf +
+    dgetri, unscaling (ScaledDenseMatrix.cpp:150-279) -- and the Taylor chain
+    by dgemm with the reference's coefficients and stopping rule
+    (compute.cpp:121-160), both through the host BLAS (scipy's OpenBLAS,
+    `threads` threads), then the camera output and point step (oracle).  No
+    compensated refinement.  Returns (seconds, n_terms, BLAS name)."""
+    import numpy as np
+    import scipy.linalg.blas as blas
+    import scipy.linalg.lapack as lapack
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    import oracle
+    t0 = time.perf_counter()
+    Z, d = oracle.front(J, nthreads=threads)
+    N = Z.shape[0]
+    lam = oracle.auto_lambda(N) if J.lam == -1 else J.lam
+    with threadpool_limits(limits=threads):
+        A = np.asfortranarray(Z + lam * np.eye(N))
+        sL = np.sqrt((A * A).sum(axis=1))
+        A /= sL[:, None]
+        sR = np.sqrt((A * A).sum(axis=0))
+        A /= sR[None, :]
+        lu, piv, info = lapack.dgetrf(A, overwrite_a=True)
+        Xi, info2 = lapack.dgetri(lu, piv, overwrite_lu=True)
+        if info or info2:
+            raise RuntimeError("LAPACK inverse failed")
+        X0 = np.asfortranarray(Xi / sR[:, None] / sL[None, :])
+        X = X0.copy(order="F")
+        T = blas.dgemm(1.0, X0, X0)
+        old, terms = np.finfo(np.float64).max, 0
+        for i in range(1, 20):
+            k = lam ** i / float(oracle.factorial32(i - 1))
+            change = abs(k * max(np.abs(T).max(), np.finfo(np.float64).tiny))
+            if change < 1e-5 or change > old:
+                break
+            old = change
+            terms += 1
+            X += k * T
+            T = blas.dgemm(1.0, T, X0)
+    cp = J.cam_params
+    iu = np.triu_indices(cp)
+    blocks = {}
+    cam = np.empty((J.num_cams, cp * (cp + 1) // 2))
+    for a in range(J.num_cams):
+        sl = slice(a * cp, (a + 1) * cp)
+        B = X[sl, sl]
+        cam[a] = ((B + B.T / 2) * d[sl][:, None] * d[sl][None, :])[iu]
+        blocks[a] = B
+    oracle.points_from_blocks(J, blocks)
+    secs = time.perf_counter() - t0
+    blas_name = ", ".join(sorted({f"{x.get('internal_api')} {x.get('version')}"
+                                  for x in threadpool_info() if x.get("user_api") == "blas"}))
+    return secs, terms, blas_name
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,16 +401,19 @@ def main() -> int:
     schur_bytes = 8 * (81 * nC + 27 * nO + 9 * nP) + 8 * N * N
 
     cpu = None
+    parity = None
     if not args.no_cpu:
         import oracle
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        t1 = time.perf_counter()
-        oracle.pipeline(J, nthreads=threads)
-        cs = time.perf_counter() - t1
+        # the reference's arithmetic with the host BLAS (the baseline)
+        cs, cterms, blas_name = cpu_reference_arithmetic(J, threads)
         # the reference hard-codes 8 OpenMP / Eigen threads (compute.cpp:460-463)
+        cs8, _, _ = cpu_reference_arithmetic(J, 8)
+        # the parity oracle (C restatement with a compensated Newton step and its
+        # own blocked dgemm), whose output is the GPU's parity reference
         t1 = time.perf_counter()
-        oracle.pipeline(J, nthreads=8)
-        cs8 = time.perf_counter() - t1
+        o_ref = oracle.pipeline(J, nthreads=threads)
+        co = time.perf_counter() - t1
         model = ""
         try:
             for line in open("/proc/cpuinfo"):
@@ -363,10 +424,32 @@ def main() -> int:
             pass
         cpu = {"value": round(1.0 / cs, 4), "unit": "solves/s", "cores": threads, "kind": "port",
                "value_8_threads": round(1.0 / cs8, 4), "cpu_model": model,
-               "nproc": os.cpu_count(),
-               "sample": f"1 full solve of the same {args.workload} scene ({cs:.1f} s on {threads} "
-                         f"threads, {cs8:.1f} s on 8), C oracle (OpenMP, its own blocked FP64 "
-                         f"dgemm and compensated refinement; no host BLAS)"}
+               "nproc": os.cpu_count(), "blas": blas_name, "taylor_terms": cterms,
+               "parity_oracle_solves_per_s": round(1.0 / co, 4),
+               "sample": f"1 full solve of the same {args.workload} scene with the reference's "
+                         f"arithmetic ({cs:.2f} s on {threads} threads, {cs8:.2f} s on 8): C "
+                         f"oracle front end (fixPts, blocks, Schur; OpenMP), equilibrated LAPACK "
+                         f"dgetrf/dgetri and the dgemm Taylor chain through {blas_name}, outputs"}
+        # parity of the GPU result of the last timed solve against the oracle,
+        # beside the FP64 noise floor (the reference-shaped unrefined LU inverse
+        # against the refined oracle on the same Z)
+        if args.precision == "fp64":
+            o_lu = oracle.pipeline(J, inverse_mode=oracle.INV_REFERENCE, nthreads=threads)
+            gcam = cam.cpu().numpy().reshape(nC, per)
+            gpt = pt.cpu().numpy().reshape(nP, 6)
+
+            def brel(a, b):
+                nb = np.linalg.norm(b, axis=1)
+                m = nb > 0
+                return (np.linalg.norm(a - b, axis=1)[m] / nb[m])
+
+            parity = {"cam_max": float(brel(gcam, o_ref["cam"]).max()),
+                      "pt_max": float(brel(gpt, o_ref["pt"]).max()),
+                      "noise_floor_cam_max": float(brel(o_lu["cam"], o_ref["cam"]).max()),
+                      "noise_floor_pt_max": float(brel(o_lu["pt"], o_ref["pt"]).max()),
+                      "contract": 1e-9, "terms_equal": terms == o_ref["n_terms"],
+                      "against": "C oracle (refined FP64 inverse); noise floor = unrefined "
+                                 "equilibrated-LU oracle vs refined oracle"}
 
     # the main measurement's context and tensors are released first: the
     # request server's streams then have the device's hardware queues to
@@ -396,6 +479,7 @@ def main() -> int:
                      "per_launch_flops": flops, "avg_launch_ms": round(avg_gemm_s * 1e3, 4),
                      "launches": gemm_n},
         "cpu_baseline": cpu,
+        "parity": parity,
         "stages_ms": {names[i]: round(float(stage[i]), 4) for i in range(12) if stage[i] > 0},
         "hbm": {"assemble_GBps": round(asm_bytes / (stage[3] * 1e-3) / 1e9, 1) if stage[3] > 0 else None,
                 "schur_GBps": round(schur_bytes / (stage[5] * 1e-3) / 1e9, 1) if stage[5] > 0 else None,
