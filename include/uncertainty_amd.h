@@ -214,6 +214,14 @@ typedef struct unc_scene unc_scene;
  * 123-157): images by id, those without observations dropped; points by id;
  * observations image by image, each image's by point id. */
 int unc_scene_read_colmap(const char *dir, unc_scene **out);
+/* A BAL problem file (replaces BALProblem::BALProblem(filename),
+ * bal_problem.cpp:111-172): "nc np nobs", nobs lines "cam pt x y", 9 nc camera
+ * numbers, 3 np point numbers.  File order throughout.  The camera numbers mean
+ * what the reference's BAL functor makes of them (SnavelyReprojectionError,
+ * snavely_reprojection_error.h:309-347, bundle type 0 with unit weights):
+ * aa[3], the camera centre C[3], f, k1, k2 -- the UNC_CAM_RADIAL model.
+ * A truncated or malformed file or an index out of range: UNC_ERR_BAD_INPUT. */
+int unc_scene_read_bal(const char *path, unc_scene **out);
 int unc_scene_counts(const unc_scene *sc, int *num_cams, int *num_points, int *num_obs);
 /* cams: num_cams x 9 (aa[3], C[3], f, k1, k2); points: num_points x 3;
  * obs_cam / obs_pt: num_obs indices.  Any pointer may be NULL. */

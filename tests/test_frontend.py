@@ -105,3 +105,86 @@ def test_cli_reads_colmap_until_the_device(tmp_path):
     r = subprocess.run([u.CLI_PATH, f"-in={tmp_path}", "-cam=FISHEYE"], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "unknown camera model" in r.stderr
+
+
+# ---- BAL problem files (unc_scene_read_bal; BALProblem, bal_problem.cpp:111-172)
+
+_BAL_TEXT = """2 3 4
+0 0 -1.5 2.5
+1 0\t3e1 -4
+0 1 0.5 0.5
+1 2 1 2
+0.01 -0.02 0.03
+1 2 3
+500 -1e-7 2.5E-9
+-0.04 0.05 -0.06
+-1 -2 -3
+600.5 0 0
+1 2 3
+4 5 6
+7 8 9.25
+"""
+
+
+def test_bal_reads_a_hand_written_file(tmp_path):
+    """Format rules fixed by bal_problem.cpp:121-143 on a file written by hand:
+    header, four "cam pt x y" lines (a tab among the separators), 9 numbers per
+    camera and 3 per point in mixed notations, file order kept."""
+    p = tmp_path / "tiny.txt"
+    p.write_text(_BAL_TEXT)
+    cams, pts, oc, op = u.read_bal(str(p))
+    assert np.array_equal(cams, [[0.01, -0.02, 0.03, 1, 2, 3, 500, -1e-7, 2.5e-9],
+                                 [-0.04, 0.05, -0.06, -1, -2, -3, 600.5, 0, 0]])
+    assert np.array_equal(pts, [[1, 2, 3], [4, 5, 6], [7, 8, 9.25]])
+    assert np.array_equal(oc, [0, 1, 0, 1]) and np.array_equal(op, [0, 0, 1, 2])
+
+
+@pytest.mark.parametrize("order", ["point", "camera", "shuffle"])
+def test_bal_roundtrip_keeps_file_order(tmp_path, order):
+    sc = scene.ring_scene(7, 150, seed=5)
+    p = tmp_path / "problem.txt"
+    sel = scene.write_bal(str(p), sc, order)
+    cams, pts, oc, op = u.read_bal(str(p))
+    assert np.array_equal(cams, sc.cams) and np.array_equal(pts, sc.points)
+    assert np.array_equal(oc, sc.obs_cam[sel]) and np.array_equal(op, sc.obs_pt[sel])
+
+
+@pytest.mark.parametrize("edit", ["missing", "truncated", "cam_range", "pt_range", "negative",
+                                  "word", "empty"])
+def test_bal_errors(tmp_path, edit):
+    p = tmp_path / "bad.txt"
+    lines = _BAL_TEXT.splitlines()
+    if edit == "truncated":          # the last point number is missing
+        lines[-1] = "7 8"
+    elif edit == "cam_range":        # camera index == num_cameras
+        lines[2] = "2 0 3e1 -4"
+    elif edit == "pt_range":
+        lines[4] = "1 3 1 2"
+    elif edit == "negative":
+        lines[0] = "2 -3 4"
+    elif edit == "word":
+        lines[7] = "500 nothing 2.5E-9"
+    elif edit == "empty":
+        lines = []
+    if edit != "missing":
+        p.write_text("\n".join(lines) + "\n")
+    with pytest.raises(u.UncError) as e:
+        u.read_bal(str(p))
+    assert e.value.code == u.UNC_ERR_BAD_INPUT
+
+
+def test_cli_reads_bal_until_the_device(tmp_path):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except Exception:
+        pass
+    sc = scene.ring_scene(4, 40, seed=2)
+    scene.write_bal(str(tmp_path / "p.txt"), sc)
+    r = subprocess.run([u.CLI_PATH, f"-in={tmp_path}/p.txt", "-in_form=BAL",
+                        f"-out={tmp_path}/o.cov"], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot compose the Jacobian" in r.stderr
+    r = subprocess.run([u.CLI_PATH, f"-in={tmp_path}/none.txt", "-in_form=BAL"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot read the BAL problem" in r.stderr

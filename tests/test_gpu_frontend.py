@@ -1,6 +1,6 @@
 """GPU: the scene Jacobian kernel (unc_scene_jacobian; JacobianComposer::
 scene2Jacobian, JacobianComposer.cpp:84-194, with analytic derivatives of
-snavely_reprojection_error.h:148-302) and the COLMAP path end to end.
+snavely_reprojection_error.h:148-347) and the COLMAP and BAL paths end to end.
 
 Checks: for each of the four camera models, the CRS structure exactly (two rows
 per observation, cp + 3 columns: camera block then point block) and the values
@@ -8,8 +8,9 @@ against the oracle's central differences of the reference's residual functors
 (oracle/scene_fd.py, 1e-6 of each row's largest entry; the differencing error
 is ~1e-9); for RADIAL also against the generator's analytic blocks at 1e-12.
 Out-of-range indices come back as UNC_ERR_BAD_INPUT from the kernel's own
-bounds check.  End to end: a COLMAP text model through the CLI gives the .cov
-that unc_compute gives on the same Jacobian and fixed points."""
+bounds check.  End to end: a COLMAP text model and a BAL problem file through
+the CLI give the .cov that unc_compute gives on the same Jacobian and fixed
+points."""
 import os
 import subprocess
 
@@ -103,3 +104,56 @@ def test_colmap_to_cov_through_the_cli(tmp_path):
     # the .cov holds 6 significant digits
     assert np.all(np.abs(got_c - want_c) <= 1e-5 * np.abs(want_c) + 1e-300)
     assert np.all(np.abs(got_p - want_p) <= 1e-5 * np.abs(want_p) + 1e-300)
+
+
+def _block_rel(a, b):
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    return np.where(nb > 0, d / np.where(nb > 0, nb, 1), d)
+
+
+def test_bal_file_to_covariances(tmp_path):
+    """A BAL problem (observations point by point, as the published BAL files
+    list them) read by unc_scene_read_bal, its Jacobian by the device kernel in
+    the file's row order against the oracle's central differences of the
+    reference's functor, the covariances against the oracle (bit-exact Schur
+    complement), the CLI's -in_form=BAL .cov against unc_compute, and the same
+    problem with its observations in other orders to 1e-9."""
+    import oracle
+    sc = scene.ring_scene(12, 600, seed=9)
+    path = tmp_path / "problem.txt"
+    scene.write_bal(str(path), sc, "point")
+    cams, pts, oc, op = u.read_bal(str(path))
+    print("bal ring12x600: jacobian", flush=True)
+    J = u.scene_jacobian(cams, pts, oc, op, u.CAM_RADIAL)
+    fd = scene_fd.jacobian_fd(cams, pts, oc, op, u.CAM_RADIAL)
+    got = J.vals.reshape(len(oc), 2, 12)
+    assert np.all(np.abs(got - fd) <= 1e-6 * np.abs(fd).max(axis=2, keepdims=True))
+    print("bal ring12x600: covariances", flush=True)
+    o = oracle.pipeline(J, debug=True)
+    r = u.compute_covariances(J, debug=True)
+    assert np.array_equal(r.debug["U_s"], o["U_s"]) and np.array_equal(r.debug["Z"], o["Z"])
+    assert r.n_terms == o["n_terms"]
+    assert _block_rel(r.cam, o["cam"]).max() <= 1e-9
+    assert _block_rel(r.pt, o["pt"]).max() <= 1e-9
+    print("bal ring12x600: cli", flush=True)
+    out = tmp_path / "bal.cov"
+    p = subprocess.run([u.CLI_PATH, f"-in={path}", "-in_form=BAL", f"-out={out}"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    hdr, cam, pt = jacob.read_cov(str(out))
+    assert int(hdr["Number of cameras"]) == sc.num_cams
+    got_c, got_p = np.asarray(cam, np.float64).reshape(-1), np.asarray(pt, np.float64).reshape(-1)
+    want_c, want_p = r.cam.reshape(-1), r.pt.reshape(-1)
+    assert np.all(np.abs(got_c - want_c) <= 1e-5 * np.abs(want_c) + 1e-300)
+    assert np.all(np.abs(got_p - want_p) <= 1e-5 * np.abs(want_p) + 1e-300)
+    # the same problem with its observations camera by camera and shuffled:
+    # only the row order differs, so only the rounding may
+    for order in ("camera", "shuffle"):
+        p2 = tmp_path / f"{order}.txt"
+        scene.write_bal(str(p2), sc, order)
+        r2 = u.compute_covariances(u.scene_jacobian(*u.read_bal(str(p2)), u.CAM_RADIAL, fix=J.fix))
+        assert _block_rel(r2.cam, r.cam).max() <= 1e-9, order
+        assert _block_rel(r2.pt, r.pt).max() <= 1e-9, order

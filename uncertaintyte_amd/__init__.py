@@ -126,6 +126,7 @@ def lib():
     L.unc_server_destroy.restype = None
     L.unc_device_lds_bytes.argtypes = [ctypes.c_int]
     L.unc_scene_read_colmap.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.unc_scene_read_bal.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     L.unc_scene_counts.argtypes = [vp, ip, ip, ip]
     L.unc_scene_arrays.argtypes = [vp, vp, vp, vp, vp]
     L.unc_scene_destroy.argtypes = [vp]
@@ -404,12 +405,10 @@ CAM_MODELS = {"SIMPLE_PINHOLE": CAM_SIMPLE_PINHOLE, "PINHOLE": CAM_PINHOLE,
 CAM_PARAMS = {CAM_SIMPLE_PINHOLE: 6, CAM_PINHOLE: 7, CAM_SIMPLE_RADIAL: 8, CAM_RADIAL: 9}
 
 
-def read_colmap(path: str):
-    """(cams (n_cams, 9) [aa, C, f, k1, k2], points (n_points, 3), obs_cam,
-    obs_pt) of a COLMAP text model, in the reference's parameter-block order."""
+def _read_scene(reader: str, path: str):
     L = lib()
     h = ctypes.c_void_p()
-    _check(L.unc_scene_read_colmap(os.fsencode(path), ctypes.byref(h)), "unc_scene_read_colmap")
+    _check(getattr(L, reader)(os.fsencode(path), ctypes.byref(h)), reader)
     try:
         nc, npt, no = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(L.unc_scene_counts(h, ctypes.byref(nc), ctypes.byref(npt), ctypes.byref(no)),
@@ -423,6 +422,19 @@ def read_colmap(path: str):
     finally:
         L.unc_scene_destroy(h)
     return cams, pts, oc, op
+
+
+def read_colmap(path: str):
+    """(cams (n_cams, 9) [aa, C, f, k1, k2], points (n_points, 3), obs_cam,
+    obs_pt) of a COLMAP text model, in the reference's parameter-block order."""
+    return _read_scene("unc_scene_read_colmap", path)
+
+
+def read_bal(path: str):
+    """(cams (n_cams, 9) [aa, C, f, k1, k2], points (n_points, 3), obs_cam,
+    obs_pt) of a BAL problem file, in file order (unc_scene_read_bal); the
+    Jacobian of these is scene_jacobian(..., CAM_RADIAL)."""
+    return _read_scene("unc_scene_read_bal", path)
 
 
 def find_pts2fix(points: np.ndarray) -> tuple:

@@ -1,9 +1,10 @@
 // cli.cpp -- `uncertainty` command line tool with the reference's flags
-// (src/software/main_uncertainty.cpp:33-93): read a .jacob or a COLMAP text
-// model, compute the covariances on the MI355X, write a .cov.
+// (src/software/main_uncertainty.cpp:33-93): read a .jacob, a COLMAP text
+// model or a BAL problem file, compute the covariances on the MI355X, write a .cov.
 //
 //   uncertainty -in=scene.jacob [-in_form=JACOBIAN] -out=scene.cov
 //   uncertainty -in=colmap_dir [-in_form=COLMAP] [-cam=SIMPLE_RADIAL] -out=scene.cov
+//   uncertainty -in=problem.txt -in_form=BAL -out=scene.cov
 //               [-alg=TAYLOR_EXPANSION] [-debug] [-legacy_time_units]
 //
 // As in the reference, a .jacob input always runs TAYLOR_EXPANSION
@@ -228,6 +229,55 @@ bool load_colmap(const std::string &dir, const std::string &cam, const std::stri
   return true;
 }
 
+// BAL problem file -> Jacobian (the reference's Matlab route: BALProblem,
+// BuildProblem with bundle type 0 and unit weights, buildJacobian,
+// matlabInterface.cpp:19-213): all nine camera parameters free
+// (UNC_CAM_RADIAL), fixed points by unc_find_pts2fix (setPts2Fix,
+// auxCmd.cpp:43-66), TAYLOR_EXPANSION whatever -alg says and no SVD mode count,
+// as buildJacobian sets them (matlabInterface.cpp:203-212).
+bool load_bal(const std::string &path, ceres::CRSMatrix &J, cov::Options &opt) {
+  unc_scene *sc = nullptr;
+  int rc = unc_scene_read_bal(path.c_str(), &sc);
+  if (rc != UNC_OK) {
+    std::cerr << "cannot read the BAL problem " << path << ": " << unc_error_string(rc) << "\n";
+    return false;
+  }
+  int nc = 0, np = 0, nobs = 0;
+  unc_scene_counts(sc, &nc, &np, &nobs);
+  std::vector<double> cams((size_t)nc * 9), pts((size_t)np * 3);
+  std::vector<int> oc(nobs), op(nobs);
+  unc_scene_arrays(sc, cams.data(), pts.data(), oc.data(), op.data());
+  unc_scene_destroy(sc);
+  int fix[3];
+  rc = unc_find_pts2fix(np, pts.data(), fix);
+  const int cp = 9;
+  if (rc == UNC_OK) {
+    J.num_rows = 2 * nobs;
+    J.num_cols = nc * cp + 3 * np;
+    J.rows.resize(2 * (size_t)nobs + 1);
+    J.cols.resize(2 * (size_t)nobs * (cp + 3));
+    J.values.resize(J.cols.size());
+    rc = unc_scene_jacobian(NULL, UNC_CAM_RADIAL, nc, np, nobs, cams.data(), pts.data(),
+                            oc.data(), op.data(), J.rows.data(), J.cols.data(), J.values.data());
+  }
+  if (rc != UNC_OK) {
+    std::cerr << "cannot compose the Jacobian: " << unc_error_string(rc) << "\n";
+    return false;
+  }
+  opt._algorithm = TAYLOR_EXPANSION;
+  opt._epsilon = 1e-10;
+  opt._lambda = -1;
+  opt._numCams = nc;
+  opt._camParams = cp;
+  opt._numPoints = np;
+  opt._numObs = nobs;
+  opt._svdRemoveN = -1;
+  opt._maxIterTE = -1;
+  opt._pts2fix = (int *)malloc(3 * sizeof(int));
+  for (int i = 0; i < 3; ++i) opt._pts2fix[i] = fix[i];
+  return true;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -257,8 +307,10 @@ int main(int argc, char **argv) {
     if (!load_jacobian(in, J, opt)) return 1;
   } else if (in_form == "COLMAP") {
     if (!load_colmap(in, cam, alg, J, opt)) return 1;
+  } else if (in_form == "BAL") {
+    if (!load_bal(in, J, opt)) return 1;
   } else {
-    std::cerr << "unsupported input format " << in_form << " (JACOBIAN or COLMAP)\n";
+    std::cerr << "unsupported input format " << in_form << " (JACOBIAN, COLMAP or BAL)\n";
     return 2;
   }
   const int per = opt._camParams * (opt._camParams + 1) / 2;

@@ -11,10 +11,22 @@
 // ColmapIO.cpp:56-59 reads radial terms for the wrong models), an observation
 // of an unknown point is an error (the reference creates a point at the
 // origin), and an identity rotation gives aa = 0 (the reference divides by 0).
+//
+// The BAL text reader (BALProblem::BALProblem(filename), bal_problem.cpp:
+// 111-172) keeps the file's order: cameras, points and observations as they
+// stand.  Its nine camera numbers are taken as the reference's functor takes
+// them (SnavelyReprojectionError, snavely_reprojection_error.h:309-347, the
+// bundle type 0 of BuildProblem, matlabInterface.cpp:66-68): angle-axis,
+// camera centre C, f, k1, k2, with unit weights since a BAL file carries none.
+// Unlike the reference (which dies on a short file, fscanf in FscanfOrDie) a
+// malformed or truncated file, or an index out of range, is an error code.
 #include <algorithm>
 #include <array>
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <map>
 #include <random>
@@ -177,6 +189,80 @@ int unc_scene_read_colmap(const char *dir, unc_scene **out) {
       sc->obs_cam.push_back(ci);
       sc->obs_pt.push_back(p);
     }
+  }
+  *out = sc;
+  return UNC_OK;
+}
+
+int unc_scene_read_bal(const char *path, unc_scene **out) {
+  if (!path || !out) return UNC_ERR_BAD_INPUT;
+  *out = nullptr;
+  // The whole file at once, then a forward scan with strtol / strtod: BAL
+  // problems run to millions of observations and a stream extractor per value
+  // is several times slower.
+  std::string text;
+  {
+    std::FILE *f = std::fopen(path, "rb");
+    if (!f) return UNC_ERR_BAD_INPUT;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return UNC_ERR_BAD_INPUT;
+  }
+  const char *p = text.c_str();
+  bool ok = true;
+  auto next_long = [&](long long &v) {
+    char *e;
+    v = std::strtoll(p, &e, 10);
+    if (e == p) ok = false;
+    p = e;
+  };
+  auto next_double = [&](double &v) {
+    char *e;
+    v = std::strtod(p, &e);
+    if (e == p) ok = false;
+    p = e;
+  };
+  // header: num_cameras num_points num_observations (bal_problem.cpp:121-123)
+  long long nc, np, nobs;
+  next_long(nc);
+  next_long(np);
+  next_long(nobs);
+  if (!ok || nc < 1 || np < 1 || nobs < 1 || nc > INT32_MAX / 9 || np > INT32_MAX / 3 ||
+      nobs > INT32_MAX / 2)
+    return UNC_ERR_BAD_INPUT;
+  unc_scene *sc = new unc_scene();
+  sc->obs_cam.resize(nobs);
+  sc->obs_pt.resize(nobs);
+  // observations: camera_index point_index x y (bal_problem.cpp:133-139); the
+  // Jacobian does not depend on x, y, which are parsed and dropped
+  for (long long i = 0; i < nobs && ok; ++i) {
+    long long c, q;
+    double x, y;
+    next_long(c);
+    next_long(q);
+    next_double(x);
+    next_double(y);
+    if (c < 0 || c >= nc || q < 0 || q >= np) ok = false;
+    sc->obs_cam[i] = (int)c;
+    sc->obs_pt[i] = (int)q;
+  }
+  // 9 per camera, then 3 per point (bal_problem.cpp:141-143)
+  sc->cams.resize(9 * nc);
+  sc->points.resize(3 * np);
+  for (auto &v : sc->cams) {
+    if (!ok) break;
+    next_double(v);
+  }
+  for (auto &v : sc->points) {
+    if (!ok) break;
+    next_double(v);
+  }
+  if (!ok) {
+    delete sc;
+    return UNC_ERR_BAD_INPUT;
   }
   *out = sc;
   return UNC_OK;

@@ -414,3 +414,29 @@ def write_colmap(path: str, sc: Scene, model: str = "RADIAL", image_id=None, poi
             X = sc.points[p]
             f.write(f"{int(pid[p])} {float(X[0])!r} {float(X[1])!r} {float(X[2])!r} "
                     "128 128 128 0.5\n")
+
+
+def write_bal(path: str, sc: Scene, order: str = "point"):
+    """Write sc as a BAL problem file (bal_problem.cpp:111-143 reads it):
+    "nc np nobs", one "cam pt x y" line per observation, then the 9 numbers of
+    each camera [aa, C, f, k1, k2] and the 3 of each point, one per line.  x, y
+    are the projections.  order "point" lists the observations point by point
+    (the order of the published BAL problems), "camera" camera by camera, and
+    "shuffle" in a fixed random order."""
+    uv = project(sc.cams[sc.obs_cam], sc.points[sc.obs_pt])
+    if order == "point":
+        sel = np.lexsort((sc.obs_cam, sc.obs_pt))
+    elif order == "camera":
+        sel = np.lexsort((sc.obs_pt, sc.obs_cam))
+    elif order == "shuffle":
+        sel = np.random.default_rng(17).permutation(sc.num_obs)
+    else:
+        raise ValueError(order)
+    with open(path, "w") as f:
+        f.write(f"{sc.num_cams} {sc.num_points} {sc.num_obs}\n")
+        for o in sel:
+            f.write(f"{int(sc.obs_cam[o])} {int(sc.obs_pt[o])} "
+                    f"{float(uv[o, 0])!r} {float(uv[o, 1])!r}\n")
+        for v in np.concatenate([sc.cams.reshape(-1), sc.points.reshape(-1)]):
+            f.write(f"{float(v)!r}\n")
+    return sel
