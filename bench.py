@@ -246,9 +246,9 @@ def copy_calibration() -> int:
 def gemm_traffic():
     """HBM bytes per Taylor-chain product (GEMM + epilogue) measured offline with
     rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE, each scaled by the copy
-    calibration); recorded in profiles/gemm_rings200_traffic.md as
+    calibration); recorded in profiles/gemm_batch_rings200_traffic.md as
     'traffic_bytes_per_launch: <int>'."""
-    path = os.path.join(ROOT, "profiles", "gemm_rings200_traffic.md")
+    path = os.path.join(ROOT, "profiles", "gemm_batch_rings200_traffic.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -374,6 +374,8 @@ def main() -> int:
     gemm_ms = 0.0
     gemm_flops = 0.0
     gemm_n = 0
+    batch_ms = batch_flops = 0.0
+    batch_n = 0
     stage = np.zeros(12)
     terms = 0
     t0 = time.perf_counter()
@@ -382,17 +384,32 @@ def main() -> int:
         gemm_ms += st.gemm_ms
         gemm_flops += st.gemm_flops
         gemm_n += st.gemm_count
+        batch_ms += st.batch_ms
+        batch_flops += st.batch_flops
+        batch_n += st.batch_count
         stage += np.array(st.stage_ms[:])
         terms = st.num_terms
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = args.steps / el
     N = J.num_cams * J.cam_params
-    # algorithmic flops per chain product: N^2 (N + 1) for the symmetric form
-    # (lower tiles only), 2 N^3 where the plain product runs
-    flops = gemm_flops / max(gemm_n, 1)
-    avg_gemm_s = gemm_ms / max(gemm_n, 1) * 1e-3
+    # The dominant kernel: the chain's launches of four products each
+    # (k_gemm_sym_batch, where the chain is batched), algorithmic flops
+    # 4 N^2 (N + 1) per launch (the symmetric form forms the lower tiles only).
+    # Where no such launch ran, all of the chain's product launches.
+    if batch_n > 0:
+        flops = batch_flops / batch_n
+        avg_gemm_s = batch_ms / batch_n * 1e-3
+        launches = batch_n
+        kernel = ("Taylor-chain powers, four per launch: k_gemm_sym_batch (X0^4j times X0^1..4, "
+                  "lower 128x128 tiles, whole K per workgroup, mirrored in the epilogue)")
+    else:
+        flops = gemm_flops / max(gemm_n, 1)
+        avg_gemm_s = gemm_ms / max(gemm_n, 1) * 1e-3
+        launches = gemm_n
+        kernel = "Taylor-chain product launches (k_gemm_f64w8 + epilogue)"
     achieved = flops / avg_gemm_s / 1e12 if avg_gemm_s > 0 else 0.0
+    chain_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     stage /= args.steps
     names = [u.lib().unc_stage_name(i).decode() for i in range(12)]
     nC, nO, nP = J.num_cams, J.num_obs, J.num_points
@@ -473,11 +490,12 @@ def main() -> int:
         "config": {"workload": WORKLOADS.get(args.workload, args.workload) + ", " + PATHS[args.precision],
                    "cameras": nC, "points": nP, "observations": nO,
                    "jacobian_nnz": nnz, "N": N, "taylor_terms": terms, "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "Taylor-chain product T X0: k_gemm_f64w8 (lower tiles in XCD-contiguous blocks, 4 K chunks) + k_split_reduce_sym", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": gemm_traffic(),
                      "per_launch_flops": flops, "avg_launch_ms": round(avg_gemm_s * 1e3, 4),
-                     "launches": gemm_n},
+                     "launches": launches, "chain_launches": gemm_n,
+                     "chain_TFLOPs": round(chain_tflops, 3)},
         "cpu_baseline": cpu,
         "parity": parity,
         "stages_ms": {names[i]: round(float(stage[i]), 4) for i in range(12) if stage[i] > 0},

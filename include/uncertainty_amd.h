@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define UNC_ABI_VERSION 1
+#define UNC_ABI_VERSION 2  /* 2: unc_stats gained batch_ms, batch_flops, batch_count */
 
 /* error codes */
 #define UNC_OK 0
@@ -93,16 +93,23 @@ typedef struct unc_stats {
   int num_terms;           /* accepted Taylor terms = cycle_change size            */
   double cycle_change[20];
   /* device-side measurements */
-  double gemm_ms;          /* summed duration of the Taylor-chain products (HIP events) */
-  double gemm_flops;       /* their algorithmic flops: 2 N^3 each, N^2 (N + 1) for the  */
-                           /* symmetric form that computes the lower tiles only         */
-  int gemm_count;
+  double gemm_ms;          /* summed duration of the Taylor chain's product launches    */
+                           /* that did work (HIP events; one launch forms 1-4 products) */
+  double gemm_flops;       /* their algorithmic flops: 2 N^3 per product, N^2 (N + 1)   */
+                           /* for the symmetric form that computes the lower tiles only */
+  int gemm_count;          /* launches                                                */
   int num_obs_kept;        /* observations left after fixPts                          */
                            /* time_norm_j and time_split_jj are 0: the column norms are
                               the assembled blocks' diagonals and the blocks are
                               written split; time_multiply_jj = camera blocks + W_s,
                               time_inv_v = point blocks with their 3x3 inverses     */
   double stage_ms[12];     /* per device stage, see unc_stage_name()                  */
+  /* the chain's launches of four products each (where the chain is batched,
+     N >= 1024): their summed duration, algorithmic flops and count; included
+     in gemm_ms / gemm_flops / gemm_count, which count launches */
+  double batch_ms;
+  double batch_flops;
+  int batch_count;
 } unc_stats;
 
 /* Optional host copies of intermediates (any pointer may be NULL).

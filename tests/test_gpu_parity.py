@@ -91,6 +91,30 @@ def test_gemm_f64_symmetric_product(n):
     assert np.array_equal(C, C2) and m == m2
 
 
+@pytest.mark.parametrize("n,nb", [(1800, 4), (1800, 2), (1283, 3), (2100, 4), (300, 1)])
+def test_gemm_f64_symmetric_batch(n, nb):
+    """The batched chain's products X0^a X0^(1..nb) in one launch: each C_b exactly
+    symmetric (lower tiles formed, mirrored), equal to the full product to FP64
+    rounding, max|C_b| from the epilogue, the same on a second launch."""
+    say(f"gemm sym batch {n} x {nb}")
+    rng = np.random.default_rng(n + nb)
+    Q = rng.standard_normal((n, n)) / np.sqrt(n)
+    S = Q + Q.T
+    P = [S]
+    for _ in range(3):
+        T = P[-1] @ S
+        P.append(0.5 * (T + T.T))
+    A = P[3]
+    Cs, mx, _ = diag.gemm_sym_batch(A, P[:nb])
+    for b in range(nb):
+        ref = A @ P[b]
+        assert np.array_equal(Cs[b], Cs[b].T), b
+        assert np.abs(Cs[b] - ref).max() <= 1e-13 * np.abs(A).max() * np.abs(P[b]).max() * n, b
+        assert mx[b] == np.abs(Cs[b]).max(), b
+    Cs2, mx2, _ = diag.gemm_sym_batch(A, P[:nb])
+    assert all(np.array_equal(x, y) for x, y in zip(Cs, Cs2)) and mx == mx2
+
+
 def test_gemm_f64_split_k_declines_small_shapes():
     A = np.ones((256, 1024))
     with pytest.raises(u.UncError) as e:
@@ -156,6 +180,8 @@ def _cases():
         "rings40x4000_shuffled": lambda: _shuffled(scene.scene_jacobian(scene.rings_scene(40, 4000, 4.0, seed=11))),
         # ~8000 observations per camera: the Schur kernel's global-table form
         "ring10x20000": lambda: scene.scene_jacobian(scene.ring_scene(10, 20000, seed=4)),
+        # N = 1080: the Taylor chain in batches of four products (from N = 1024)
+        "rings120x12000": lambda: scene.scene_jacobian(scene.rings_scene(120, 12000, 4.0, seed=13)),
     }
 
 

@@ -60,7 +60,9 @@ class Stats(ctypes.Structure):
         "lambda_")] + [("fixed_pts", ctypes.c_int * 3), ("num_terms", ctypes.c_int),
                        ("cycle_change", ctypes.c_double * 20), ("gemm_ms", ctypes.c_double),
                        ("gemm_flops", ctypes.c_double), ("gemm_count", ctypes.c_int),
-                       ("num_obs_kept", ctypes.c_int), ("stage_ms", ctypes.c_double * 12)]
+                       ("num_obs_kept", ctypes.c_int), ("stage_ms", ctypes.c_double * 12),
+                       ("batch_ms", ctypes.c_double), ("batch_flops", ctypes.c_double),
+                       ("batch_count", ctypes.c_int)]
 
 
 class Debug(ctypes.Structure):

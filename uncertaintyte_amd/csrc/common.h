@@ -116,6 +116,15 @@ size_t gemm_sym_split_work_doubles(int n);
 hipError_t gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                         double *work, size_t work_doubles, unsigned long long *absmax,
                         const int *active, hipStream_t st, int parts = 0);
+// nb <= 4 symmetric products C_b = A B_b in one launch (A and every B_b
+// symmetric, every product symmetric: powers of one matrix), on the 128 x 128
+// tiles on and below the diagonal, each workgroup forming one tile over the
+// whole K range, lower triangle mirrored in the epilogue; max|C_b| folded into
+// absmax[b].  The grid (nb x lower tiles) is dealt to the XCDs product by
+// product.  Exits at once when *active == 0.
+hipError_t gemm_f64_sym_batch(int n, int nb, const double *A, const double *const B[4],
+                              double *const C[4], unsigned long long *absmax, const int *active,
+                              hipStream_t st);
 hipError_t gemm_f64_split(int M, int N, int K, double alpha, const double *A, int lda,
                           const double *B, int ldb, double *C, int ldc, double *work,
                           unsigned long long *absmax, const int *active, hipStream_t st);
@@ -130,6 +139,12 @@ constexpr int GEMM_LOWER_LIST = 16;
 // all tiles (times the K parts) in 4 x 4 tile blocks, each XCD taking a
 // contiguous range of them
 constexpr int GEMM_BLOCKED = 32;
+// with GEMM_LOWER_LIST: the tile index is the lower-list position itself (the
+// caller has already spread its grid over the XCDs); K is not split
+constexpr int GEMM_TILE_DIRECT = 64;
+// C is symmetric: of a lower tile only the elements on or below the diagonal
+// are formed, each written at (row, col) and (col, row); max|C| folds over them
+constexpr int GEMM_SYM_OUT = 128;
 hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                        int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
                        hipStream_t st);
@@ -176,6 +191,11 @@ hipError_t sym_update(int n, const double *X0, const double *D, double *out, hip
 // out = X0 + D when both are exactly symmetric (the same value as sym_update)
 hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st);
 hipError_t te_init(TeCtrl *ctrl, hipStream_t st);
+// Terms i0 .. i0+m-1 (m <= 4, coefficients k[]) of the chain at once: the
+// accept / stop decisions of te_step taken in term order on one thread, then
+// X += k_j T_j over the accepted terms, in term order.
+hipError_t te_steps(int n, TeCtrl *ctrl, int i0, int m, const double k[4], double *X,
+                    const double *const T[4], hipStream_t st);
 hipError_t te_step(int n, TeCtrl *ctrl, int i, double k, double *X, const double *T,
                    hipStream_t st);
 // A[0, n) = 0 (A 16-byte aligned)

@@ -30,6 +30,12 @@ int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, 
  * mirrored; UNC_ERR_UNSUPPORTED for shapes the library would not take this way */
 int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                           double *absmax_out, int parts);
+/* C_b = A B_b (b < nb <= 4; A and every B_b n x n symmetric, products symmetric)
+ * by the batched lower-tile kernel of the Taylor chain in one launch.  B and C
+ * hold nb matrices one after the other; absmax_out[b] = max|C_b| from the
+ * fused epilogue.  avg_ms (may be NULL): average of `reps` further launches. */
+int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B, double *C,
+                                double *absmax_out, int reps, double *avg_ms);
 /* X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels, plus the Newton step
  * with the error-free residual when refine != 0 */
 int unc_diag_inverse(int n, const double *A, double *X, int refine);

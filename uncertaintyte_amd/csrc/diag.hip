@@ -126,6 +126,46 @@ int unc_diag_gemm_f64_split(int M, int N, int K, double alpha, const double *A, 
   return UNC_OK;
 }
 
+int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B, double *C,
+                                double *absmax_out, int reps, double *avg_ms) {
+  if (n <= 0 || nb < 1 || nb > 4 || !A || !B || !C || reps < 0) return UNC_ERR_BAD_INPUT;
+  DSTREAM(s);
+  const size_t nn = (size_t)n * n;
+  DevBuf buf;
+  if (buf.alloc(sizeof(double) * (1 + 2 * (size_t)nb) * nn + 256)) return UNC_ERR_OOM;
+  double *dA = (double *)buf.p, *dB = dA + nn, *dC = dB + nb * nn;
+  unsigned long long *dm = (unsigned long long *)(dC + nb * nn);
+  DC(hipMemcpyAsync(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  DC(hipMemcpyAsync(dB, B, sizeof(double) * nb * nn, hipMemcpyHostToDevice, s));
+  DC(hipMemsetAsync(dC, 0, sizeof(double) * nb * nn, s));
+  DC(hipMemsetAsync(dm, 0, sizeof(unsigned long long) * 4, s));
+  const double *bq[4];
+  double *cq[4];
+  for (int q = 0; q < 4; ++q) {
+    bq[q] = dB + (size_t)(q < nb ? q : 0) * nn;
+    cq[q] = dC + (size_t)(q < nb ? q : 0) * nn;
+  }
+  DC(gemm_f64_sym_batch(n, nb, dA, bq, cq, dm, nullptr, s));
+  unsigned long long mb[4] = {0, 0, 0, 0};
+  DC(hipMemcpyAsync(C, dC, sizeof(double) * nb * nn, hipMemcpyDeviceToHost, s));
+  DC(hipMemcpyAsync(mb, dm, sizeof(mb), hipMemcpyDeviceToHost, s));
+  DC(hipStreamSynchronize(s));
+  if (absmax_out) std::memcpy(absmax_out, mb, sizeof(double) * nb);
+  if (avg_ms && reps > 0) {
+    Events ev;
+    DC(hipEventCreate(&ev.a));
+    DC(hipEventCreate(&ev.b));
+    DC(hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r) DC(gemm_f64_sym_batch(n, nb, dA, bq, cq, dm, nullptr, s));
+    DC(hipEventRecord(ev.b, s));
+    DC(hipEventSynchronize(ev.b));
+    float ms = 0.0f;
+    DC(hipEventElapsedTime(&ms, ev.a, ev.b));
+    *avg_ms = ms / reps;
+  }
+  return UNC_OK;
+}
+
 int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B, double *C,
                           double *absmax_out, int parts) {
   if (n <= 0 || !A || !B || !C || parts < 0 || parts > 4) return UNC_ERR_BAD_INPUT;

@@ -91,7 +91,9 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves of 64 x 32
   const int tiles_m = (M + GB_M - 1) / GB_M, tiles_n = (N + GB_N - 1) / GB_N;
   int bid = tile_bid, part = 0, tm, tn;
-  if (tri & GEMM_LOWER_LIST) {  // grid = parts x lower tiles, chunk-major
+  if (tri & GEMM_TILE_DIRECT) {  // bid is the lower-list position
+    lower_tile_blocked(bid, tiles_m, tm, tn);
+  } else if (tri & GEMM_LOWER_LIST) {  // grid = parts x lower tiles, chunk-major
     const int nl = tiles_m * (tiles_m + 1) / 2;
     const int w = xcd_contiguous(bid, parts * nl);
     part = w / nl;
@@ -197,6 +199,19 @@ __device__ __forceinline__ void gemm_w8_tile(int tile_bid, int M, int N, int K, 
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn * 32 + j * 16 + (lane & 15);
       const bool col_ok = col < N && (col < skip0 || col >= skip1);
+      if (tri & GEMM_SYM_OUT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+          if (row < M && col < N && row >= col) {
+            const double v = alpha * acc[i][j][r];
+            C[(size_t)row + (size_t)col * ldc] = v;
+            if (row != col) C[(size_t)col + (size_t)row * ldc] = v;
+            lmax = fmax(lmax, fabs(v));
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;

@@ -57,6 +57,12 @@ struct Layout {
 constexpr int GJ_BLOCK = 64;
 size_t zplus_work_doubles(int n);
 
+// The Taylor chain forms its powers four at a time where one product alone
+// would leave the device half idle (from N = 1024; below, products are small
+// and launch-bound either way).
+constexpr int TE_BATCH_MIN_N = 1024;
+bool te_batched(int N) { return N >= TE_BATCH_MIN_N; }
+
 Layout make_layout(const Sizes &z) {
   size_t sz[B_COUNT];
   sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
@@ -78,7 +84,10 @@ Layout make_layout(const Sizes &z) {
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
   sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
   sz[B_DE] = sizeof(double) * z.nn;
-  sz[B_SPLIT] = sizeof(double) * gemm_sym_split_work_doubles(z.N);  // Taylor chain K chunks
+  // Taylor chain: the first product's K chunks, and the five power slots of
+  // the batched chain where it runs (the chunks alias the first three)
+  sz[B_SPLIT] = sizeof(double) * std::max(gemm_sym_split_work_doubles(z.N),
+                                          te_batched(z.N) ? 5 * z.nn : (size_t)0);
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
   Layout L;
   size_t o = 0;
@@ -576,27 +585,76 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   }
   HC(hipEventRecord(E[7], s));
   if (dbg && dbg->X0) HC(hipMemcpyAsync(dbg->X0, X0, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-  // ---- Taylor chain (compute.cpp:142-160); X in dA (A is dead), T ping-pongs
+  // ---- Taylor chain (compute.cpp:142-160); X in dA (A is dead)
   double *X = dA;
-  double *Tb0 = dC, *Tb1 = dD;
-  // X0^(i+2) = T_i X0 is symmetric: only its lower tiles are computed
   HC(hipMemcpyAsync(X, X0, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
   HC(te_init(ctrl, s));
+  // chain launches: events and the first term each produces (a launch runs iff
+  // every earlier term was accepted), products per launch
+  int lfirst[24], lcount[24];
   const int first_chain_pair = gemm_pairs;
-  HC(hipEventRecord(GE[2 * gemm_pairs], s));
-  HC(gemm_f64_sym(N, 1.0, X0, X0, Tb0, sym_work, sym_work_doubles, &ctrl->maxbits[1], nullptr, s));
-  HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
-  ++gemm_pairs;
-  double *Tc = Tb0, *Tn = Tb1;
-  for (int i = 1; i < 20; ++i) {
-    const double k = pow(lambda, i) / (double)fact32(i - 1);
-    HC(te_step(N, ctrl, i, k, X, Tc, s));
-    HC(hipEventRecord(GE[2 * gemm_pairs], s));
-    HC(gemm_f64_sym(N, 1.0, Tc, X0, Tn, sym_work, sym_work_doubles, &ctrl->maxbits[i + 1],
-                    &ctrl->active, s));
-    HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));
+  auto chain_launch_begin = [&](int first_term, int products) {
+    lfirst[gemm_pairs] = first_term;
+    lcount[gemm_pairs] = products;
+    return hipEventRecord(GE[2 * gemm_pairs], s);
+  };
+  auto chain_launch_end = [&]() {
+    const hipError_t e = hipEventRecord(GE[2 * gemm_pairs + 1], s);
     ++gemm_pairs;
-    std::swap(Tc, Tn);
+    return e;
+  };
+  auto coef = [&](int i) { return pow(lambda, i) / (double)fact32(i - 1); };
+  // T_1 = X0^2; X0^(i+1) = T_i is symmetric: only lower tiles are computed
+  double *T1 = dC;
+  HC(chain_launch_begin(1, 1));
+  HC(gemm_f64_sym(N, 1.0, X0, X0, T1, sym_work, sym_work_doubles, &ctrl->maxbits[1], nullptr, s));
+  HC(chain_launch_end());
+  const bool batched = te_batched(N);
+  if (!batched) {
+    double *Tc = T1, *Tn = dD;
+    for (int i = 1; i < 20; ++i) {
+      HC(te_step(N, ctrl, i, coef(i), X, Tc, s));
+      HC(chain_launch_begin(i + 1, 1));
+      HC(gemm_f64_sym(N, 1.0, Tc, X0, Tn, sym_work, sym_work_doubles, &ctrl->maxbits[i + 1],
+                      &ctrl->active, s));
+      HC(chain_launch_end());
+      std::swap(Tc, Tn);
+    }
+  } else {
+    // powers X0^1..4 stay resident; X0^(4j+1..4j+4) = X0^(4j) X0^(1..4) fill
+    // five rotating slots (the base X0^(4j) of a batch is the last product of
+    // the one before, so four fresh slots are always free)
+    double *pw[4] = {X0, T1, dD, X0 == dB ? dE : dB};
+    double *slot = sym_work;
+    HC(te_step(N, ctrl, 1, coef(1), X, T1, s));
+    {
+      const double *Bq[4] = {pw[0], pw[1], pw[0], pw[0]};
+      double *Cq[4] = {pw[2], pw[3], pw[2], pw[2]};
+      HC(chain_launch_begin(2, 2));
+      HC(gemm_f64_sym_batch(N, 2, T1, Bq, Cq, &ctrl->maxbits[2], &ctrl->active, s));
+      HC(chain_launch_end());
+      const double kq[4] = {coef(2), coef(3), 0.0, 0.0};
+      const double *Tq[4] = {pw[2], pw[3], pw[2], pw[2]};
+      HC(te_steps(N, ctrl, 2, 2, kq, X, Tq, s));
+    }
+    const double *base = pw[3];
+    for (int j = 1; 4 * j < 20; ++j) {
+      const int i0 = 4 * j, m = std::min(4, 20 - i0);
+      double *Cq[4];
+      const double *Tq[4];
+      double kq[4];
+      for (int q = 0; q < 4; ++q) {
+        Cq[q] = slot + (size_t)((4 * (j - 1) + std::min(q, m - 1)) % 5) * nn;
+        Tq[q] = Cq[q];
+        kq[q] = q < m ? coef(i0 + q) : 0.0;
+      }
+      const double *Bq[4] = {pw[0], pw[1], pw[2], pw[3]};
+      HC(chain_launch_begin(i0, m));
+      HC(gemm_f64_sym_batch(N, m, base, Bq, Cq, &ctrl->maxbits[i0], &ctrl->active, s));
+      HC(chain_launch_end());
+      HC(te_steps(N, ctrl, i0, m, kq, X, Tq, s));
+      base = Cq[m - 1];
+    }
   }
   HC(hipEventRecord(E[8], s));
   if (dbg && dbg->X) HC(hipMemcpyAsync(dbg->X, X, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
@@ -618,20 +676,33 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
 
   if (st) {
     fill_stats(lambda, hc.nterms, hc.change);
-    double gms = 0.0;
-    int gcount = 0;
-    // executed products of the Taylor chain: T1 and one per accepted term
-    const int executed = first_chain_pair + 1 + hc.nterms;
-    for (int p = first_chain_pair; p < executed && p < gemm_pairs; ++p) {
-      gms += ev_ms(GE[2 * p], GE[2 * p + 1]);
+    double gms = 0.0, gflops = 0.0, bms = 0.0, bflops = 0.0;
+    int gcount = 0, bcount = 0;
+    // a symmetric product computed on its lower tiles needs N (N + 1) / 2
+    // entries of 2 N flops each; the first product falls back to the full
+    // 2 N^3 where its split form does not apply
+    const double nd = (double)N, sym_flops = nd * nd * (nd + 1.0);
+    for (int p = first_chain_pair; p < gemm_pairs; ++p) {
+      if (hc.nterms < lfirst[p] - 1) continue;  // exited at once: the series had stopped
+      const double ms = ev_ms(GE[2 * p], GE[2 * p + 1]);
+      const bool first = p == first_chain_pair;
+      const double fl = (first || !batched) ? (sym_work_doubles > 0 ? sym_flops : 2.0 * nd * nd * nd)
+                                            : lcount[p] * sym_flops;
+      gms += ms;
+      gflops += fl;
       ++gcount;
+      if (lcount[p] == 4) {
+        bms += ms;
+        bflops += fl;
+        ++bcount;
+      }
     }
     st->gemm_ms = gms;
     st->gemm_count = gcount;
-    // a symmetric product computed on its lower tiles needs N (N + 1) / 2
-    // entries of 2 N flops each
-    const double nd = (double)N;
-    st->gemm_flops = (sym_work_doubles > 0 ? nd * nd * (nd + 1.0) : 2.0 * nd * nd * nd) * gcount;
+    st->gemm_flops = gflops;
+    st->batch_ms = bms;
+    st->batch_flops = bflops;
+    st->batch_count = bcount;
   }
   return UNC_OK;
 }

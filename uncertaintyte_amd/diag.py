@@ -31,6 +31,7 @@ def dlib():
     L.unc_diag_gemm_f64.argtypes = [i, i, i, ctypes.c_double, vp, i, vp, i, ctypes.c_double, vp, i, dp]
     L.unc_diag_gemm_f64_split.argtypes = [i, i, i, ctypes.c_double, vp, i, vp, i, vp, i, dp]
     L.unc_diag_gemm_f64_sym.argtypes = [i, ctypes.c_double, vp, vp, vp, dp, i]
+    L.unc_diag_gemm_f64_sym_batch.argtypes = [i, i, vp, vp, vp, dp, i, dp]
     L.unc_diag_inverse.argtypes = [i, vp, vp, i]
     L.unc_diag_residual.argtypes = [i, vp, vp, vp, i]
     L.unc_diag_chol_inverse.argtypes = [i, vp, vp, vp]
@@ -83,6 +84,24 @@ def gemm_sym(A: np.ndarray, B: np.ndarray, alpha: float = 1.0, parts: int = 0):
     _check(dlib().unc_diag_gemm_f64_sym(n, alpha, Af.ctypes.data, Bf.ctypes.data, Cf.ctypes.data,
                                         ctypes.byref(m), parts), "unc_diag_gemm_f64_sym")
     return np.asarray(Cf), m.value
+
+
+def gemm_sym_batch(A: np.ndarray, Bs, reps: int = 0):
+    """([C_b], [max|C_b|], avg_ms) with C_b = A B_b for up to four symmetric B_b
+    by the Taylor chain's batched lower-tile kernel (one launch); avg_ms over
+    `reps` further launches (None when reps == 0)."""
+    n = A.shape[0]
+    nb = len(Bs)
+    Af = np.asfortranarray(A, dtype=np.float64)
+    Bf = np.concatenate([np.asfortranarray(B, dtype=np.float64).reshape(-1, order="F") for B in Bs])
+    Cf = np.zeros(nb * n * n)
+    mx = (ctypes.c_double * nb)()
+    ms = ctypes.c_double()
+    _check(dlib().unc_diag_gemm_f64_sym_batch(n, nb, Af.ctypes.data, Bf.ctypes.data, Cf.ctypes.data,
+                                              mx, reps, ctypes.byref(ms) if reps else None),
+           "unc_diag_gemm_f64_sym_batch")
+    Cs = [Cf[b * n * n:(b + 1) * n * n].reshape(n, n, order="F") for b in range(nb)]
+    return Cs, list(mx), (ms.value if reps else None)
 
 
 def inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:
