@@ -190,6 +190,8 @@ hipError_t add_diag(int n, double *A, int lda, double v, hipStream_t st);
 hipError_t sym_update(int n, const double *X0, const double *D, double *out, hipStream_t st);
 // out = X0 + D when both are exactly symmetric (the same value as sym_update)
 hipError_t add_symmetric(int n, const double *X0, const double *D, double *out, hipStream_t st);
+// mx[j] = max_i |A_ij| (n x n, column-major) as IEEE bits
+hipError_t col_absmax(int n, const double *A, unsigned long long *mx, hipStream_t st);
 hipError_t te_init(TeCtrl *ctrl, hipStream_t st);
 // Terms i0 .. i0+m-1 (m <= 4, coefficients k[]) of the chain at once: the
 // accept / stop decisions of te_step taken in term order on one thread, then

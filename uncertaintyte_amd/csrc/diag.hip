@@ -8,6 +8,13 @@
 #include "common.h"
 #include "diag.h"
 
+namespace unc {
+// int8 Ozaki residual (k_ozaki.hip, this library only)
+size_t residual_oz_work_bytes(int n);
+hipError_t residual_oz(int n, const double *A, const double *X, double *R, void *work,
+                       hipStream_t st);
+}  // namespace unc
+
 using namespace unc;
 
 namespace {
@@ -231,6 +238,35 @@ int unc_diag_residual(int n, const double *A, const double *X, double *R, int ti
   DC(residual_ex(n, b, b + nn, b + 2 * nn, b + 3 * nn, s, tile_cols));
   DC(hipMemcpyAsync(R, b + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   DC(hipStreamSynchronize(s));
+  return UNC_OK;
+}
+
+int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int reps,
+                         double *avg_ms) {
+  if (n <= 0 || !A || !X || !R || reps < 0) return UNC_ERR_BAD_INPUT;
+  DSTREAM(s);
+  const size_t nn = (size_t)n * n, wb = residual_oz_work_bytes(n);
+  DevBuf buf;
+  if (buf.alloc(sizeof(double) * 3 * nn + wb + 256)) return UNC_ERR_OOM;
+  double *b = (double *)buf.p;
+  void *work = (void *)(((uintptr_t)(b + 3 * nn) + 255) & ~(uintptr_t)255);
+  DC(hipMemcpyAsync(b, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  DC(hipMemcpyAsync(b + nn, X, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+  DC(residual_oz(n, b, b + nn, b + 2 * nn, work, s));
+  DC(hipMemcpyAsync(R, b + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+  DC(hipStreamSynchronize(s));
+  if (avg_ms && reps > 0) {
+    Events ev;
+    DC(hipEventCreate(&ev.a));
+    DC(hipEventCreate(&ev.b));
+    DC(hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r) DC(residual_oz(n, b, b + nn, b + 2 * nn, work, s));
+    DC(hipEventRecord(ev.b, s));
+    DC(hipEventSynchronize(ev.b));
+    float ms = 0.0f;
+    DC(hipEventElapsedTime(&ms, ev.a, ev.b));
+    *avg_ms = ms / reps;
+  }
   return UNC_OK;
 }
 

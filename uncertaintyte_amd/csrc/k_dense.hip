@@ -464,6 +464,12 @@ __global__ __launch_bounds__(256) void k_col_absmax(int n, const double *__restr
   if (lane == 0) mx[j] = (unsigned long long)__double_as_longlong(m);
 }
 
+hipError_t col_absmax(int n, const double *A, unsigned long long *mx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_absmax, dim3(cdiv(n, 4)), dim3(256), 0, st, n, A, mx);
+  return hipGetLastError();
+}
+
 template <int QN>
 __global__ __launch_bounds__(256, QN == 2 ? 5 : 7) void k_residual_ex(
     int n, int M, const double *__restrict__ A, const double *__restrict__ X,
