@@ -80,6 +80,19 @@ def test_workspace_bytes_scales_with_n2():
     assert 5 * 8 * N * N <= b < 5 * 8 * N * N + 64 * 1024 * 1024
 
 
+def test_workspace_chain_slots_only_on_the_fp64_path():
+    """From N = 1024 the FP64 TE path forms the Taylor chain's powers in batches
+    and holds five more N x N buffers for them; the Z+ paths never run the chain
+    and must not reserve them (their footprint drives serving admission)."""
+    from uncertaintyte_amd import scene
+    j = scene.scene_jacobian(scene.ring_scene(120, 600, seed=3))
+    N = j.num_cams * j.cam_params
+    assert N >= 1024
+    te = u.workspace_bytes(u.make_options(j, u.PREC_FP64), j.num_rows)
+    zp = u.workspace_bytes(u.make_options(j, u.PREC_MIXED), j.num_rows)
+    assert 5 * 8 * N * N <= te - zp < 5 * 8 * N * N + (1 << 20)
+
+
 def test_no_gpu_is_a_clean_error():
     try:
         import torch

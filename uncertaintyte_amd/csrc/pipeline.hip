@@ -40,6 +40,7 @@ const char *kStageNames[12] = {"upload",   "decode",   "point_lists", "assemble"
 struct Sizes {
   int nc, cp, np, npc, nobs, N;
   size_t nn;
+  bool te;  // the plain FP64 TE path (its Taylor chain needs the chain slots)
 };
 
 enum {
@@ -86,8 +87,9 @@ Layout make_layout(const Sizes &z) {
   sz[B_DE] = sizeof(double) * z.nn;
   // Taylor chain: the first product's K chunks, and the five power slots of
   // the batched chain where it runs (the chunks alias the first three)
-  sz[B_SPLIT] = sizeof(double) * std::max(gemm_sym_split_work_doubles(z.N),
-                                          te_batched(z.N) ? 5 * z.nn : (size_t)0);
+  sz[B_SPLIT] = z.te ? sizeof(double) * std::max(gemm_sym_split_work_doubles(z.N),
+                                                 te_batched(z.N) ? 5 * z.nn : (size_t)0)
+                     : 0;
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
   Layout L;
   size_t o = 0;
@@ -142,6 +144,7 @@ int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[
   z->nobs = nrows / 2;
   z->N = (int)N;
   z->nn = (size_t)N * (size_t)N;
+  z->te = o->precision == UNC_PREC_FP64;
   return UNC_OK;
 }
 
@@ -1136,6 +1139,7 @@ unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   Sizes z;
   z.nc = o->num_cams; z.cp = o->cam_params; z.np = o->num_points; z.npc = z.np - 3;
   z.nobs = num_rows / 2; z.N = z.nc * z.cp; z.nn = (size_t)z.N * z.N;
+  z.te = o->precision == UNC_PREC_FP64;
   return (unsigned long long)make_layout(z).total;
 }
 
