@@ -77,44 +77,84 @@ def _timed(call, torch, reps):
     return (time.perf_counter() - t0) / reps * 1e3, st
 
 
-def measure_zplus(u, scene, torch, dev, check=True):
-    """Z+ path vs plain FP64 TE path on walk1500, inputs resident in HBM."""
+def _zplus_errors(J, cams, zcam_h, zpt_h, with_points, log=None):
+    """Camera blocks of `cams` (and, with_points, the point blocks of every point
+    seen only from `cams`) against the sampled FP64 oracle D pinv_7(Z_s) D."""
     import numpy as np
+    import oracle
+    nth = min(os.cpu_count() or 1, 16)
+    out = {}
+    if with_points:
+        ref, _, blk = oracle.zplus_sampled(J, cams, nthreads=nth, blocks=True, log=log)
+        pref, done = oracle.points_from_blocks(J, blk)
+        mask = done & (np.linalg.norm(pref, axis=1) > 0)
+    else:
+        ref, _ = oracle.zplus_sampled(J, cams, nthreads=nth, log=log)
+    for name, cam_h, pt_h in zip(("mixed", "zplus_fp64"), zcam_h, zpt_h):
+        rel = np.linalg.norm(cam_h[cams] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        o = {"cam_err_max": float(rel.max()), "cam_err_median": float(np.median(rel))}
+        if with_points:
+            pr = np.linalg.norm(pt_h[mask] - pref[mask], axis=1) / np.linalg.norm(pref[mask], axis=1)
+            o.update(pt_err_max=float(pr.max()), pt_err_median=float(np.median(pr)),
+                     points_checked=int(mask.sum()))
+        out[name] = o
+    return out
+
+
+def _zplus_pair(u, torch, dev, J, reps):
+    """Time the mixed (int8 CRT products) and the FP64 Z+ paths on device-resident
+    inputs; returns ms, stats and host copies of both outputs."""
+    res = {}
+    for name, prec in (("mixed", u.PREC_MIXED), ("zplus_fp64", u.PREC_ZPLUS_FP64)):
+        call, ctx, cam, pt = _device_solver(u, torch, dev, J, prec)
+        call()
+        ms, st = _timed(call, torch, reps)
+        res[name] = (ms, st, cam.cpu().numpy().reshape(J.num_cams, -1),
+                     pt.cpu().numpy().reshape(J.num_points, -1))
+        ctx.close()
+    return res
+
+
+def measure_zplus(u, scene, torch, dev, check=True):
+    """Config 3, walk1500: the mixed-precision Z+ path (int8 CRT products in the
+    Cholesky recursion) against the FP64 Z+ path and the plain FP64 TE path,
+    inputs resident in HBM; camera and point-block errors against the sampled
+    FP64 oracle."""
     J = scene.make_workload("walk1500")
     out = {"workload": "walk1500: 1500 cameras on a random walk, 1M points, track Poisson(5) "
                        "min 2, 10% short-baseline points", "N": J.num_cams * J.cam_params,
            "observations": J.num_obs}
-    zcall, zctx, zcam, _ = _device_solver(u, torch, dev, J, u.PREC_MIXED)
-    zcall()
-    z_ms, zst = _timed(zcall, torch, 3)
-    zcam_h = zcam.cpu().numpy().reshape(J.num_cams, -1)
-    zctx.close()
+    res = _zplus_pair(u, torch, dev, J, 3)
+    (z_ms, zst, zc, zp), (g_ms, gst, gc, gp) = res["mixed"], res["zplus_fp64"]
     fcall, fctx, _, _ = _device_solver(u, torch, dev, J, u.PREC_FP64)
     fcall()
     f_ms, fst = _timed(fcall, torch, 1)
     fctx.close()
-    out.update(zplus_ms=round(z_ms, 2), fp64_te_ms=round(f_ms, 2),
-               speedup=round(f_ms / z_ms, 2), zplus_inverse_ms=round(zst.stage_ms[6], 2),
+    out.update(zplus_ms=round(z_ms, 2), zplus_fp64_ms=round(g_ms, 2),
+               speedup_vs_fp64_zplus=round(g_ms / z_ms, 3),
+               fp64_te_ms=round(f_ms, 2), speedup=round(f_ms / z_ms, 2),
+               zplus_inverse_ms=round(zst.stage_ms[6], 2),
+               zplus_fp64_inverse_ms=round(gst.stage_ms[6], 2),
                fp64_inverse_plus_taylor_ms=round(fst.stage_ms[6] + fst.stage_ms[7], 2),
-               arithmetic="FP64 Cholesky selected inversion (MFMA FP64 GEMMs)")
+               moduli=zst.zplus_moduli, cond_lb=float(zst.zplus_cond_lb),
+               arithmetic="mixed: Cholesky selected inversion, products of the recursion nodes "
+                          f">= 2048 rows exact on int8 MFMA modulo {zst.zplus_moduli} moduli "
+                          "(CRT, 46-bit scaled operands), the rest FP64 MFMA; zplus_fp64: all FP64")
     if check:
-        import oracle
-        cams = list(range(0, J.num_cams, 8))
-        ref, _ = oracle.zplus_sampled(J, cams, nthreads=min(os.cpu_count() or 1, 16))
-        a = zcam_h[cams]
-        rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        out.update(cam_err_max=float(rel.max()), cam_err_median=float(np.median(rel)),
-                   err_sample=f"{len(cams)} cameras vs D pinv_7(Z_s) D (ARPACK + LAPACK oracle)",
+        cams = sorted(set(range(0, J.num_cams, 8)) | set(range(600, 800)))
+        err = _zplus_errors(J, cams, (zc, gc), (zp, gp), True)
+        out.update(errors=err, cam_err_max=err["mixed"]["cam_err_max"],
+                   pt_err_max=err["mixed"]["pt_err_max"],
+                   err_sample=f"{len(cams)} cameras (every 8th + 600..799) and the points seen "
+                              "only from them vs D pinv_7(Z_s) D (ARPACK + LAPACK oracle)",
                    contract=1e-5)
     return out
 
 
 def measure_large(u, scene, torch, dev, fp64=False, check=True):
-    """walk4000 (4000 cameras, 3M points, track Poisson(6) min 2; N ~ 36000):
-    the Z+ path, optionally the plain FP64 TE path, and the Z+ camera error on
-    200 sampled cameras against the oracle."""
-    import numpy as np
-
+    """Config 5, walk4000 (4000 cameras, 3M points, track Poisson(6) min 2;
+    N = 36000): the mixed and FP64 Z+ paths, optionally the plain FP64 TE path,
+    and the camera errors on 200 sampled cameras against the oracle."""
     def log(msg):
         print(f"[large] {msg}", file=sys.stderr, flush=True)
     t0 = time.time()
@@ -125,16 +165,17 @@ def measure_large(u, scene, torch, dev, fp64=False, check=True):
            "scene_gen_s": round(time.time() - t0, 1),
            "workspace_GiB": round(u.workspace_bytes(u.make_options(J, u.PREC_MIXED), J.num_rows)
                                   / 2 ** 30, 2)}
-    zcall, zctx, zcam, _ = _device_solver(u, torch, dev, J, u.PREC_MIXED)
-    zcall()
-    z_ms, zst = _timed(zcall, torch, 2)
-    zcam_h = zcam.cpu().numpy().reshape(J.num_cams, -1)
-    zctx.close()
-    log(f"Z+ path {z_ms:.0f} ms per solve")
-    out.update(zplus_ms=round(z_ms, 1), zplus_stages_ms={k: round(float(zst.stage_ms[i]), 2)
-               for i, k in enumerate(["upload", "decode", "point_lists", "assemble", "r", "schur",
-                                      "inverse", "taylor", "cam_out", "point_out", "d", "total"])
-               if zst.stage_ms[i] > 0})
+    res = _zplus_pair(u, torch, dev, J, 2)
+    (z_ms, zst, zc, zp), (g_ms, gst, gc, gp) = res["mixed"], res["zplus_fp64"]
+    log(f"Z+ mixed {z_ms:.0f} ms, FP64 {g_ms:.0f} ms per solve")
+    out.update(zplus_ms=round(z_ms, 1), zplus_fp64_ms=round(g_ms, 1),
+               speedup_vs_fp64_zplus=round(g_ms / z_ms, 3), moduli=zst.zplus_moduli,
+               cond_lb=float(zst.zplus_cond_lb),
+               zplus_stages_ms={k: round(float(zst.stage_ms[i]), 2)
+                                for i, k in enumerate(["upload", "decode", "point_lists", "assemble",
+                                                       "r", "schur", "inverse", "taylor", "cam_out",
+                                                       "point_out", "d", "total"])
+                                if zst.stage_ms[i] > 0})
     if fp64:
         fcall, fctx, _, _ = _device_solver(u, torch, dev, J, u.PREC_FP64)
         f_ms, fst = _timed(fcall, torch, 1)
@@ -143,11 +184,9 @@ def measure_large(u, scene, torch, dev, fp64=False, check=True):
         out.update(fp64_te_ms=round(f_ms, 1), speedup=round(f_ms / z_ms, 2),
                    fp64_taylor_terms=fst.num_terms)
     if check:
-        import oracle
         cams = list(range(0, J.num_cams, 20))
-        ref, _ = oracle.zplus_sampled(J, cams, nthreads=min(os.cpu_count() or 1, 16), log=log)
-        rel = np.linalg.norm(zcam_h[cams] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        out.update(cam_err_max=float(rel.max()), cam_err_median=float(np.median(rel)),
+        err = _zplus_errors(J, cams, (zc, gc), (zp, gp), False, log=log)
+        out.update(errors=err, cam_err_max=err["mixed"]["cam_err_max"],
                    err_sample=f"{len(cams)} cameras vs D pinv_7(Z_s) D (ARPACK + LAPACK oracle)")
     return out
 

@@ -32,7 +32,9 @@
 extern "C" {
 #endif
 
-#define UNC_ABI_VERSION 2  /* 2: unc_stats gained batch_ms, batch_flops, batch_count */
+#define UNC_ABI_VERSION 3  /* 2: unc_stats gained batch_ms, batch_flops, batch_count;
+                              3: unc_options.crt_moduli, unc_stats.zplus_moduli and
+                                 zplus_cond_lb */
 
 /* error codes */
 #define UNC_OK 0
@@ -53,8 +55,10 @@ extern "C" {
 /* arithmetic of the Z inverse (the only stage whose precision is selectable) */
 #define UNC_PREC_FP64 0        /* plain FP64 TE path: reference output, <=1e-9 per block  */
 #define UNC_PREC_MIXED 1       /* gauge-fixed Z^+ (contract: <=1e-5 per block vs the FP64
-                                  pseudo-inverse); computed by the FP64 Z^+ kernels, as
-                                  FP32 inverses fail on Z's condition (1e7-1e9)        */
+                                  pseudo-inverse): the large products of the Cholesky
+                                  recursion on the int8 matrix cores (exact integer
+                                  products modulo crt_moduli coprime moduli, rebuilt
+                                  in FP64), the rest as ZPLUS_FP64                     */
 #define UNC_PREC_ZPLUS_FP64 2  /* gauge-fixed Z^+ with FP64 inverses (the FP64 Z^+)     */
 /* Output semantics: UNC_PREC_FP64 reproduces the reference TAYLOR output
  * (including its (X[j,k] + X[k,j]/2) camera packing).  The two Z^+ modes return
@@ -82,6 +86,9 @@ typedef struct unc_options {
                          basis N; 0 disables refinement                           */
   int device;         /* HIP device ordinal                                           */
   int flags;          /* reserved, 0                                                  */
+  int crt_moduli;     /* UNC_PREC_MIXED: moduli of the int8 CRT products, 6..13 (more
+                         moduli = more bits kept: q = 23, 27, 31, 35, 39, 43, 46, 50);
+                         0 = automatic (12)                                         */
 } unc_options;
 
 typedef struct unc_stats {
@@ -110,6 +117,12 @@ typedef struct unc_stats {
   double batch_ms;
   double batch_flops;
   int batch_count;
+  /* Z^+ paths: moduli of the CRT products (0 = FP64 products throughout), and
+     1 / min_i L_ii^2 for the Cholesky factor L of Z~ = Z_s + N0 N0^T, a lower
+     bound of ||Z~^-1||_2 (each pivot L_ii^2 is at least lambda_min(Z~)); Z_s has
+     a unit-scale diagonal, so this is a lower estimate of cond(Z~) */
+  int zplus_moduli;
+  double zplus_cond_lb;
 } unc_stats;
 
 /* Optional host copies of intermediates (any pointer may be NULL).

@@ -66,7 +66,7 @@ def test_default_options_and_strings():
     u.lib().unc_default_options(ctypes.byref(o))
     assert o.algorithm == 2 and o.lambda_ == -1 and o.precision == 0 and o.refine == -1
     assert list(o.pts2fix) == [-1, -1, -1]
-    assert u.lib().unc_abi_version() == 2
+    assert u.lib().unc_abi_version() == 3
     for c in range(8):
         assert u.error_string(c)
     assert u.lib().unc_stage_name(5).decode() == "schur"

@@ -84,6 +84,33 @@ def test_zplus_modes(zcase, prec):
     assert np.all(r.pt[~mask] == 0)
 
 
+def test_mixed_uses_int8_crt_products():
+    """walk300 (N = 2700): the mixed path forms the top node's products on the int8
+    CRT kernels (12 moduli by default, or as asked), the FP64 Z+ path does not, and
+    the two differ by far less than the mixed contract."""
+    j = CASES["walk300"]()
+    print("mixed walk300: default moduli, 10 moduli, FP64", flush=True)
+    m = u.compute_covariances(j, precision=u.PREC_MIXED)
+    m10 = u.compute_covariances(j, precision=u.PREC_MIXED, moduli=10)
+    f = u.compute_covariances(j, precision=u.PREC_ZPLUS_FP64)
+    assert m.stats.zplus_moduli == 12 and m10.stats.zplus_moduli == 10
+    assert f.stats.zplus_moduli == 0
+    # the conditioning bound comes from each path's own Cholesky factor
+    assert m.stats.zplus_cond_lb > 1.0
+    assert abs(m.stats.zplus_cond_lb / f.stats.zplus_cond_lb - 1) < 1e-4
+    d12 = block_rel(m.cam, f.cam).max()
+    d10 = block_rel(m10.cam, f.cam).max()
+    print(f"  12 moduli vs FP64 {d12:.2e}; 10 moduli {d10:.2e}", flush=True)
+    assert 0 < d12 < 1e-7 and d12 < d10 < 1e-5
+
+
+def test_mixed_rejects_bad_moduli(daliborka):
+    for m in (5, 14, -1):
+        with pytest.raises(u.UncError) as e:
+            u.compute_covariances(daliborka, precision=u.PREC_MIXED, moduli=m)
+        assert e.value.code == u.UNC_ERR_BAD_INPUT
+
+
 def test_zplus_repeatable_and_context_reuse(daliborka):
     ctx = u.Context()
     a = u.compute_covariances(daliborka, precision=u.PREC_ZPLUS_FP64, ctx=ctx.handle)

@@ -50,7 +50,7 @@ class Options(ctypes.Structure):
                 ("num_obs", ctypes.c_int), ("svd_remove_n", ctypes.c_int),
                 ("max_iter_te", ctypes.c_int), ("pts2fix", ctypes.c_int * 3),
                 ("precision", ctypes.c_int), ("refine", ctypes.c_int),
-                ("device", ctypes.c_int), ("flags", ctypes.c_int)]
+                ("device", ctypes.c_int), ("flags", ctypes.c_int), ("crt_moduli", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
                        ("gemm_flops", ctypes.c_double), ("gemm_count", ctypes.c_int),
                        ("num_obs_kept", ctypes.c_int), ("stage_ms", ctypes.c_double * 12),
                        ("batch_ms", ctypes.c_double), ("batch_flops", ctypes.c_double),
-                       ("batch_count", ctypes.c_int)]
+                       ("batch_count", ctypes.c_int), ("zplus_moduli", ctypes.c_int),
+                       ("zplus_cond_lb", ctypes.c_double)]
 
 
 class Debug(ctypes.Structure):
@@ -168,7 +169,7 @@ class ServerStats(ctypes.Structure):
 
 def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
                  fix=None, lam: Optional[float] = None, algorithm: int = TAYLOR_EXPANSION,
-                 svd_remove_n: int = -1, epsilon: float = 1e-10) -> Options:
+                 svd_remove_n: int = -1, epsilon: float = 1e-10, moduli: int = 0) -> Options:
     o = Options()
     lib().unc_default_options(ctypes.byref(o))
     o.algorithm = algorithm
@@ -184,6 +185,7 @@ def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: i
     # True -> automatic number of Newton steps, False -> none, int -> that many
     o.refine = -1 if refine is True else (0 if refine is False else int(refine))
     o.device = device
+    o.crt_moduli = moduli
     return o
 
 
@@ -206,12 +208,14 @@ def _ptr(a: np.ndarray):
 def compute_covariances(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,
                         fix=None, lam: Optional[float] = None, debug: bool = False,
                         ctx=None, algorithm: int = TAYLOR_EXPANSION, svd_remove_n: int = -1,
-                        epsilon: float = 1e-10) -> Result:
+                        epsilon: float = 1e-10, moduli: int = 0) -> Result:
     """Run the pipeline on a host-resident Jacobian (jacob.Jacobian).  algorithm
     SVD_QR_ITERATION / SVD_DEVIDE_AND_CONQUER selects the dense pseudo-inverse
-    modes (small problems only) with svd_remove_n / epsilon as in cov::Options."""
+    modes (small problems only) with svd_remove_n / epsilon as in cov::Options.
+    moduli: PREC_MIXED's int8 CRT moduli (6..13, 0 = the library's default)."""
     L = lib()
-    o = make_options(jac, precision, refine, device, fix, lam, algorithm, svd_remove_n, epsilon)
+    o = make_options(jac, precision, refine, device, fix, lam, algorithm, svd_remove_n, epsilon,
+                     moduli)
     cp = jac.cam_params
     per = cp * (cp + 1) // 2
     cam = np.zeros(jac.num_cams * per, np.float64)

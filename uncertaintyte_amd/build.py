@@ -23,7 +23,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libuncertainty_amd.so")
 CLI = os.path.join(OUT, "uncertainty")
 
-HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "k_zplus.hip", "k_eig.hip", "k_scene.hip", "k_svd.hip", "pipeline.hip"]
+HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_output.hip", "k_zplus.hip", "k_eig.hip", "k_scene.hip", "k_svd.hip", "k_crt_gemm.hip", "pipeline.hip"]
 # MFMA accumulators in VGPR form: the loop-carried accumulators of the GEMM
 # then stay in place instead of being copied between AGPRs and VGPRs each slab.
 PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],

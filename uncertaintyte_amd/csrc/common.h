@@ -148,6 +148,37 @@ constexpr int GEMM_SYM_OUT = 128;
 hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                        int ldb, bool trans_b, double beta, double *C, int ldc, int tri,
                        hipStream_t st);
+// C = alpha op(A) op(B) + beta C from the exact integer product of op(A) and
+// op(B) scaled to q-bit rows/columns, formed modulo nmod coprime moduli on the
+// int8 matrix cores and rebuilt by the Chinese remainder theorem
+// (k_crt_gemm.hip); same operand flags as gemm_f64_ex.  work: 256-byte aligned,
+// crt_gemm_work_bytes(M, N, K, nmod) bytes.
+constexpr int CRT_MIN_MOD = 6, CRT_MAX_MOD = 13, CRT_MAX_K = 131071;
+int crt_bits(int nmod);  // q of nmod moduli (0 when out of range)
+size_t crt_gemm_work_bytes(int M, int N, int K, int nmod);
+// depth: LDS buffers of the product kernel's pipeline, 3..5 (0 = the library's)
+hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                    int ldb, bool trans_b, double beta, double *C, int ldc, int tri, int nmod,
+                    void *work, size_t work_bytes, hipStream_t st, int depth = 0);
+// The same in two parts, so that one operand's residues can serve several
+// products: crt_split cuts the R rows of an operand (element (r, k) at
+// P[r + k ld] when gather, else at P[k + r ld]; mask 1 keeps k <= r, 2 keeps
+// k >= r) into residue planes in `area` (crt_operand_bytes, 256-byte aligned);
+// crt_product forms C = alpha A B^T + beta C of two such operands (rows of A
+// are C's rows, rows of B its columns; tri as for gemm_f64_ex, consistent with
+// the operands' masks) with its residue planes in `cplanes` (crt_product_bytes).
+struct CrtOperand {
+  signed char *planes = nullptr;
+  int *expo = nullptr;
+  int R = 0, K = 0, Rp = 0, Kp = 0, nmod = 0;
+};
+size_t crt_operand_bytes(int R, int K, int nmod);
+size_t crt_product_bytes(int M, int N, int nmod);
+hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mask, int nmod,
+                     void *area, CrtOperand *out, hipStream_t st);
+hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
+                       double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
+                       hipStream_t st, int depth = 0);
 // gauge-fixed Z^+ path (k_zplus.hip)
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st);
@@ -175,6 +206,8 @@ hipError_t small_apply(int n, int k1, int k2, double alpha, const double *A, dou
                        const double *B, const double *T, double *out, hipStream_t st);
 hipError_t fill_hash(size_t cnt, double *V, unsigned long long seed, hipStream_t st);
 hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st);
+// *out = max_i X_ii^2 (one workgroup; X n x n, leading dimension ldx)
+hipError_t diag_max_sq(int n, const double *X, int ldx, double *out, hipStream_t st);
 hipError_t cam_gram(const Structure &s, const double *X, const double *MU, const double *S,
                     const double *Nb, double *Xd, hipStream_t st);
 hipError_t cam_output_cov(const Structure &s, const double *Xd, const double *d, double *out,

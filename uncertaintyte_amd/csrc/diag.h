@@ -36,6 +36,15 @@ int unc_diag_gemm_f64_sym(int n, double alpha, const double *A, const double *B,
  * fused epilogue.  avg_ms (may be NULL): average of `reps` further launches. */
 int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B, double *C,
                                 double *absmax_out, int reps, double *avg_ms);
+/* C = alpha A op(B) + beta C (op(B) = B^T when trans_b, B then N x K) with the
+ * operand flags of the library's general product (1 A lower, 2 B lower, 4 B^T
+ * lower, 8 lower C only): nmod = 0 by the FP64 MFMA product, 6..13 by the int8
+ * CRT product with that many moduli, its pipeline `depth` LDS buffers deep (3..5,
+ * 0 = the library's).  avg_ms (may be NULL): average of `reps` further calls on
+ * the device-resident operands. */
+int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,
+                     const double *B, int ldb, int trans_b, double beta, double *C, int ldc,
+                     int tri, int nmod, int depth, int reps, double *avg_ms);
 /* X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels, plus the Newton step
  * with the error-free residual when refine != 0 */
 int unc_diag_inverse(int n, const double *A, double *X, int refine);

@@ -270,6 +270,48 @@ int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int
   return UNC_OK;
 }
 
+int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,
+                     const double *B, int ldb, int trans_b, double beta, double *C, int ldc,
+                     int tri, int nmod, int depth, int reps, double *avg_ms) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || lda < M || ldc < M ||
+      ldb < (trans_b ? N : K) || reps < 0 || (nmod != 0 && crt_bits(nmod) == 0) ||
+      (depth != 0 && (depth < 3 || depth > 5)))
+    return UNC_ERR_BAD_INPUT;
+  DSTREAM(s);
+  const size_t na = (size_t)lda * K, nb = (size_t)ldb * (trans_b ? K : N), nc = (size_t)ldc * N;
+  const size_t wb = nmod ? crt_gemm_work_bytes(M, N, K, nmod) : 0;
+  DevBuf buf;
+  if (buf.alloc(sizeof(double) * (na + nb + 2 * nc) + wb + 512)) return UNC_ERR_OOM;
+  double *dA = (double *)buf.p, *dB = dA + na, *dC = dB + nb, *dC0 = dC + nc;
+  void *work = (void *)(((uintptr_t)(dC0 + nc) + 255) & ~(uintptr_t)255);
+  DC(hipMemcpyAsync(dA, A, sizeof(double) * na, hipMemcpyHostToDevice, s));
+  DC(hipMemcpyAsync(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  DC(hipMemcpyAsync(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+  DC(hipMemcpyAsync(dC0, dC, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
+  auto call = [&]() -> hipError_t {
+    return nmod ? gemm_crt(M, N, K, alpha, dA, lda, dB, ldb, trans_b != 0, beta, dC, ldc, tri,
+                           nmod, work, wb, s, depth)
+                : gemm_f64_ex(M, N, K, alpha, dA, lda, dB, ldb, trans_b != 0, beta, dC, ldc, tri,
+                              s);
+  };
+  DC(call());
+  DC(hipMemcpyAsync(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  DC(hipStreamSynchronize(s));
+  if (avg_ms && reps > 0) {
+    Events ev;
+    DC(hipEventCreate(&ev.a));
+    DC(hipEventCreate(&ev.b));
+    DC(hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r) DC(call());
+    DC(hipEventRecord(ev.b, s));
+    DC(hipEventSynchronize(ev.b));
+    float ms = 0.f;
+    DC(hipEventElapsedTime(&ms, ev.a, ev.b));
+    *avg_ms = ms / reps;
+  }
+  return UNC_OK;
+}
+
 int unc_diag_chol_inverse(int n, const double *A, double *L, double *X) {
   if (n <= 0 || !A || !L || !X) return UNC_ERR_BAD_INPUT;
   DSTREAM(s);

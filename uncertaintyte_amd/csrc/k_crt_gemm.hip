@@ -1,0 +1,541 @@
+// k_crt_gemm.hip -- C = alpha op(A) op(B) + beta C in FP64 from one exact integer
+// product, formed on the int8 matrix cores (v_mfma_i32_16x16x64_i8) modulo a set
+// of coprime moduli and rebuilt by the Chinese remainder theorem.
+//
+// Method (Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
+// floating-point matrix multiplication using an integer modular technique",
+// 2025):
+//  1. every row i of op(A) is scaled by 2^a_i and every column j of op(B) by
+//     2^b_j so that their 2-norms stay below 2^(q-1), and rounded to integers
+//     A', B' (|A'_ik|, |B'_kj| < 2^(q-1));
+//  2. by Cauchy-Schwarz |(A'B')_ij| <= |A'_i.| |B'_.j| < 2^(2q-2) < P/2 with
+//     P = m_1 ... m_s, so A'B' is fixed by its residues modulo the m_l;
+//  3. for each m_l the residues of A' and B' (symmetric, in [-128, 127]) are
+//     multiplied exactly in int32 (K 128^2 < 2^31 for K < 131072) and reduced
+//     modulo m_l;
+//  4. (A'B')_ij = sum_l r_l E_l mod P in 128-bit integers (E_l = P_l (P_l^-1 mod
+//     m_l), P_l = P / m_l), taken in the symmetric range, converted to FP64 and
+//     scaled back by 2^-(a_i + b_j).
+// The only rounding is the cut of step 1: |C - A B|_ij <= 2^(-q+2) sqrt(K)
+// |A_i.| |B_.j| (plus the FP64 rounding of the result).  With s moduli,
+// q = floor((log2 P - 1) / 2): 10 moduli give q = 39, 13 give q = 50.
+//
+// Operands are cut into residue planes laid out in the order the MFMA reads
+// them: a plane holds blocks of 16 rows x 64 k, each a contiguous 1 KB in which
+// lane l's 16 bytes (row l % 16, k 16 (l / 16) .. + 15) sit at 16 l, so every
+// global load, LDS store and LDS read of a wave is one contiguous 1 KB.
+#include <cmath>
+
+#include "common.h"
+#include "gemm_tile.h"
+
+namespace unc {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int CRT_T = 256;      // output tile (rows and columns), 4 waves of 128 x 128
+constexpr int CRT_KB = 64;      // k per block and per pipeline step
+constexpr int CRT_BLK = 1024;   // bytes of one 16 x 64 residue block
+constexpr int CRT_NCH = 16;     // K chunks of the row-norm partial sums (gathered operands)
+constexpr int CRT_THREADS = 256;
+constexpr int CRT_DEPTH = 4;    // LDS buffers of the product's pipeline (32 KB each)
+
+// pairwise coprime moduli <= 256; their residues fit int8 in the symmetric range
+constexpr int kModuli[CRT_MAX_MOD] = {256, 255, 253, 251, 247, 241, 239,
+                                      233, 229, 227, 223, 217, 211};
+
+struct CrtConst {
+  int nmod, q;
+  int m[CRT_MAX_MOD];
+  double minv[CRT_MAX_MOD];
+  unsigned long long elo[CRT_MAX_MOD], ehi[CRT_MAX_MOD];  // E_l
+  unsigned long long plo, phi;                            // P
+  double pinv;                                            // 1 / P (rounded)
+};
+
+typedef unsigned __int128 u128;
+
+inline unsigned long long inv_mod(unsigned long long a, unsigned long long m) {
+  for (unsigned long long x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+
+CrtConst make_const(int nmod) {
+  CrtConst c{};
+  c.nmod = nmod;
+  u128 P = 1;
+  for (int l = 0; l < nmod; ++l) P *= (u128)kModuli[l];
+  for (int l = 0; l < nmod; ++l) {
+    const unsigned long long m = (unsigned long long)kModuli[l];
+    const u128 Pl = P / m;
+    const unsigned long long y = inv_mod((unsigned long long)(Pl % m), m);
+    const u128 E = (Pl * y) % P;
+    c.m[l] = (int)m;
+    c.minv[l] = 1.0 / (double)m;
+    c.elo[l] = (unsigned long long)E;
+    c.ehi[l] = (unsigned long long)(E >> 64);
+  }
+  c.plo = (unsigned long long)P;
+  c.phi = (unsigned long long)(P >> 64);
+  const double pd = (double)c.phi * 18446744073709551616.0 + (double)c.plo;
+  c.pinv = 1.0 / pd;
+  c.q = (int)std::floor((std::log2(pd) - 1.0) / 2.0);
+  return c;
+}
+
+__device__ __forceinline__ bool crt_keep(int mask, int r, int k) {
+  return mask == 0 || (mask == 1 ? k <= r : k >= r);
+}
+
+// part[ch * R + r] = sum over the ch-th of CRT_NCH chunks of k of P(r, k)^2 for an
+// operand whose element (r, k) sits at P[r + k ld] (rows gathered across k);
+// 64 rows x 4 interleaved k phases per workgroup, phases added in order.
+__global__ __launch_bounds__(256) void k_crt_norm_gather(int R, int K, const double *__restrict__ P,
+                                                          int ld, int mask, double *part) {
+  __shared__ double s4[4][64];
+  const int rr = threadIdx.x & 63, ph = threadIdx.x >> 6, r = blockIdx.x * 64 + rr;
+  const int per = (K + CRT_NCH - 1) / CRT_NCH;
+  const int kb = blockIdx.y * per, ke = min(K, kb + per);
+  double s = 0.0;
+  if (r < R)
+    for (int k = kb + ph; k < ke; k += 4)
+      if (crt_keep(mask, r, k)) {
+        const double v = P[(size_t)r + (size_t)k * ld];
+        s = fma(v, v, s);
+      }
+  s4[ph][rr] = s;
+  __syncthreads();
+  if (ph == 0 && r < R)
+    part[(size_t)blockIdx.y * R + r] = ((s4[0][rr] + s4[1][rr]) + s4[2][rr]) + s4[3][rr];
+}
+
+// part[r] = sum_k P(r, k)^2 for an operand whose element (r, k) sits at
+// P[k + r ld] (rows contiguous in k): one wave per row, a fixed butterfly.
+__global__ __launch_bounds__(256) void k_crt_norm_contig(int R, int K, const double *__restrict__ P,
+                                                          int ld, int mask, double *part) {
+  const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= R) return;
+  double s = 0.0;
+  for (int k = lane; k < K; k += 64)
+    if (crt_keep(mask, r, k)) {
+      const double v = P[(size_t)k + (size_t)r * ld];
+      s = fma(v, v, s);
+    }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) part[r] = s;
+}
+
+// Symmetric residue of the integer x (|x| < 2^52, held exactly as a double).
+__device__ __forceinline__ int crt_residue(double x, int m, double minv) {
+  if (m == 256) {
+    const int r = (int)((long long)x & 255);
+    return r >= 128 ? r - 256 : r;
+  }
+  const double t = rint(x * minv);
+  int r = (int)fma(-(double)m, t, x);  // exact: m t < 2^50 and the result is a small integer
+  const int h = m >> 1;
+  if (r > h) r -= m;
+  else if (r < -h) r += m;
+  return r;
+}
+
+// Residue planes of an operand with R rows over K (element (r, k) at
+// P[r + k ld] when GATHER, else at P[k + r ld]; `mask` keeps k <= r (1) or
+// k >= r (2)): one wave per 16 x 64 block, lane l taking row l % 16 and
+// k 16 (l / 16) .. + 15.  expo[r] = the row's scale exponent, from its norm
+// partials (nch of them, summed in order).
+template <bool GATHER>
+__global__ __launch_bounds__(256) void k_crt_split(int R, int K, int Rp, int Kp,
+                                                   const double *__restrict__ P, int ld, int mask,
+                                                   const double *__restrict__ part, int nch,
+                                                   CrtConst cc, signed char *__restrict__ planes,
+                                                   size_t plane_bytes, int *__restrict__ expo) {
+  const int lane = threadIdx.x & 63;
+  const int kblocks = Kp / CRT_KB;
+  const long long bi = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (bi >= (long long)(Rp / 16) * kblocks) return;
+  const int rb = (int)(bi / kblocks), kb = (int)(bi - (long long)rb * kblocks);
+  const int r = rb * 16 + (lane & 15), k0 = kb * CRT_KB + 16 * (lane >> 4);
+  int e = 0;
+  if (r < R) {
+    double ss = 0.0;
+    for (int c = 0; c < nch; ++c) ss += part[(size_t)c * R + r];
+    if (ss > 0.0) {
+      int E;
+      (void)frexp(ss, &E);                 // ss < 2^E, so |row| < 2^ceil(E/2)
+      e = cc.q - 2 - ((E + 1) >> 1);       // |row| 2^e < 2^(q-2)
+    }
+  }
+  if (kb == 0 && lane < 16) expo[r] = e;
+  double x[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int k = k0 + t;
+    double v = 0.0;
+    if (r < R && k < K && crt_keep(mask, r, k))
+      v = GATHER ? P[(size_t)r + (size_t)k * ld] : P[(size_t)k + (size_t)r * ld];
+    x[t] = rint(ldexp(v, e));
+  }
+  const size_t at = ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane;
+  for (int l = 0; l < cc.nmod; ++l) {
+    const int m = cc.m[l];
+    const double mi = cc.minv[l];
+    int w[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      unsigned int u = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        u |= (unsigned int)(unsigned char)(signed char)crt_residue(x[4 * g + b], m, mi) << (8 * b);
+      w[g] = (int)u;
+    }
+    *(v4i *)(planes + (size_t)l * plane_bytes + at) = (v4i){w[0], w[1], w[2], w[3]};
+  }
+}
+
+// Residue (in [0, m)) of an int32 product sum.
+__device__ __forceinline__ unsigned int crt_reduce(int c, int m, double minv) {
+  if (m == 256) return (unsigned int)c & 255u;
+  const int t = (int)floor((double)c * minv);
+  int r = c - m * t;
+  if (r < 0) r += m;
+  else if (r >= m) r -= m;
+  return (unsigned int)r;
+}
+
+// One 256 x 256 tile of one modulus' product per workgroup: 4 waves (one per
+// SIMD), wave (wm, wn) = (w & 1, w >> 1) owning rows 128 wm .. + 127 and columns
+// 128 wn .. + 127 as 8 x 8 blocks of 16 x 16 (the int32 sums in the accumulation
+// registers).  A k step (64) is 32 residue blocks of 1 KB (16 of A, 16 of B),
+// copied straight from global memory into one of NBUF LDS buffers (each wave
+// moves 8 blocks, one global_load_lds_dwordx4 each).  Each step reads its 16
+// fragments per wave from LDS once, while the 64 MFMAs of the step before run.  The residues of the int32 sums go to the C plane of
+// the modulus (column-major, leading dimension Mp), four rows per 32-bit store.
+// Grid: nmod x the tiles (the lower ones only when GEMM_C_LOWER), modulus-major,
+// each XCD taking a contiguous range so that the workgroups sharing a modulus'
+// panels share its L2.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int NBUF>
+__global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
+    int Mp, int Np, int Kp, int tiles_m, int tiles_n, int ntiles, int tri, CrtConst cc,
+    const signed char *__restrict__ PA, const signed char *__restrict__ PB, size_t plane_a,
+    size_t plane_b, unsigned char *__restrict__ PC, size_t plane_c) {
+  __shared__ __attribute__((aligned(16))) signed char L[NBUF][32 * CRT_BLK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int w = xcd_contiguous((int)blockIdx.x, cc.nmod * ntiles);
+  const int mod = w / ntiles;
+  int t = w - mod * ntiles, tm, tn;
+  if (tri & GEMM_C_LOWER) {  // lower tiles row by row
+    tm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
+    while (tm * (tm + 1) / 2 > t) --tm;
+    tn = t - tm * (tm + 1) / 2;
+  } else {  // groups of four tile rows, column by column inside a group
+    const int per = 4 * tiles_n, g0 = (t / per) * 4, gs = min(4, tiles_m - g0);
+    tm = g0 + (t % per) % gs;
+    tn = (t % per) / gs;
+  }
+  const int m0 = tm * CRT_T, n0 = tn * CRT_T;
+  int kbeg = 0, kend = Kp;
+  if (tri & GEMM_A_LOWER) kend = min(kend, m0 + CRT_T);
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + CRT_T);
+  if (tri & GEMM_B_LOWER) kbeg = n0;
+  const int kblocks = Kp / CRT_KB;
+  const int s0 = kbeg / CRT_KB, nsteps = max(0, (kend - kbeg + CRT_KB - 1) / CRT_KB);
+  // this wave's 8 blocks of a step: A blocks 8 w .. 8 w + 7 (w < 2), else B blocks
+  const signed char *src = (wave < 2 ? PA + (size_t)mod * plane_a +
+                                           ((size_t)(m0 / 16) + 8 * wave) * kblocks * CRT_BLK
+                                     : PB + (size_t)mod * plane_b +
+                                           ((size_t)(n0 / 16) + 8 * (wave - 2)) * kblocks * CRT_BLK) +
+                           16 * lane;
+  auto issue = [&](int q) {
+    const signed char *g = src + (size_t)(s0 + q) * CRT_BLK;
+    signed char *dst = L[q % NBUF] + 8 * wave * CRT_BLK;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      __builtin_amdgcn_global_load_lds((const void *)(g + (size_t)b * kblocks * CRT_BLK),
+                                       (lds_void *)(dst + b * CRT_BLK), 16, 0, 0);
+  };
+  v4i acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
+  // Steps run through NBUF LDS buffers: while step q is multiplied, steps up to
+  // q + NBUF - 1 are being copied in; step r's copy is issued at the barrier
+  // before step r - NBUF + 1 is read, into the buffer of step r - NBUF, whose
+  // fragment reads every wave finished before that barrier.
+  auto sync_step = [&](int r) {
+    const int later = min(nsteps - 1 - r, NBUF - 2);
+    // this wave's fragment reads are done and its copies of step r have landed
+    // (vector memory operations complete in order; the 8 of each later step
+    // issued may be in flight); the barrier then makes every wave's visible.
+    // (gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+    __builtin_amdgcn_s_waitcnt(0xC07F);                                    // lgkmcnt(0)
+    if (NBUF > 4 && later >= 3) __builtin_amdgcn_s_waitcnt(0x4F78);       // vmcnt(24)
+    else if (NBUF > 3 && later >= 2) __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16)
+    else if (later >= 1) __builtin_amdgcn_s_waitcnt(0x0F78);              // vmcnt(8)
+    else __builtin_amdgcn_s_waitcnt(0x0F70);                              // vmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    if (r + NBUF - 1 < nsteps) issue(r + NBUF - 1);
+  };
+#pragma unroll
+  for (int q = 0; q < NBUF - 1; ++q)
+    if (q < nsteps) issue(q);
+  for (int q = 0; q < nsteps; ++q) {
+    sync_step(q);
+    const signed char *Lb = L[q % NBUF];
+    // the A fragments first, then each B fragment read while the eight MFMAs
+    // of the one before it run
+    v4i af[8], bq[2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) af[i] = *(const v4i *)(Lb + (wm * 8 + i) * CRT_BLK + 16 * lane);
+    bq[0] = *(const v4i *)(Lb + (16 + wn * 8) * CRT_BLK + 16 * lane);
+    __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);  // 9 LDS reads
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (j + 1 < 8) {
+        bq[(j + 1) & 1] = *(const v4i *)(Lb + (16 + wn * 8 + j + 1) * CRT_BLK + 16 * lane);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bq[j & 1], acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);  // 8 MFMAs
+    }
+  }
+  const int m = cc.m[mod];
+  const double mi = cc.minv[mod];
+  unsigned char *pc = PC + (size_t)mod * plane_c;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = n0 + wn * 128 + j * 16 + (lane & 15);
+      const int row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4);
+      const unsigned int u = crt_reduce(acc[i][j][0], m, mi) |
+                             crt_reduce(acc[i][j][1], m, mi) << 8 |
+                             crt_reduce(acc[i][j][2], m, mi) << 16 |
+                             crt_reduce(acc[i][j][3], m, mi) << 24;
+      *(unsigned int *)(pc + (size_t)row + (size_t)col * Mp) = u;
+    }
+}
+
+// 128-bit helpers on two 64-bit limbs
+__device__ __forceinline__ void u128_addmul(unsigned long long &lo, unsigned long long &hi,
+                                            unsigned long long r, unsigned long long blo,
+                                            unsigned long long bhi) {
+  const unsigned long long plo = r * blo;
+  const unsigned long long phi = __umul64hi(r, blo) + r * bhi;
+  lo += plo;
+  hi += phi + (lo < plo ? 1ull : 0ull);
+}
+
+__device__ __forceinline__ bool u128_ge(unsigned long long alo, unsigned long long ahi,
+                                        unsigned long long blo, unsigned long long bhi) {
+  return ahi > bhi || (ahi == bhi && alo >= blo);
+}
+
+__device__ __forceinline__ void u128_sub(unsigned long long &alo, unsigned long long &ahi,
+                                         unsigned long long blo, unsigned long long bhi) {
+  const unsigned long long borrow = alo < blo ? 1ull : 0ull;
+  alo -= blo;
+  ahi -= bhi + borrow;
+}
+
+// C_ij = alpha 2^-(ea_i + eb_j) crt(r_1 .. r_s) + beta C_ij for the rows i of
+// column j = blockIdx.y, four consecutive rows per thread (one 32-bit load per
+// plane); with `lower` only i >= j is written.
+__global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int lower, double alpha,
+                                                     double beta, CrtConst cc,
+                                                     const unsigned char *__restrict__ PC,
+                                                     size_t plane_c, const int *__restrict__ ea,
+                                                     const int *__restrict__ eb, double *C,
+                                                     int ldc) {
+  const int j = blockIdx.y, i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (i0 >= M || j >= N) return;
+  if (lower && i0 + 3 < j) return;
+  unsigned long long lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+  for (int l = 0; l < cc.nmod; ++l) {
+    const unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + (size_t)i0 +
+                                                   (size_t)j * Mp);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      u128_addmul(lo[b], hi[b], (u >> (8 * b)) & 255u, cc.elo[l], cc.ehi[l]);
+  }
+  const int ej = eb[j];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int i = i0 + b;
+    if (i >= M || (lower && i < j)) continue;
+    // x mod P: the quotient (< 2^12) from FP64, one below to stay non-negative
+    const double xd = (double)hi[b] * 18446744073709551616.0 + (double)lo[b];
+    double qd = floor(xd * cc.pinv) - 1.0;
+    if (qd > 0.0) {
+      unsigned long long qlo = 0, qhi = 0;
+      u128_addmul(qlo, qhi, (unsigned long long)qd, cc.plo, cc.phi);
+      u128_sub(lo[b], hi[b], qlo, qhi);
+    }
+    while (u128_ge(lo[b], hi[b], cc.plo, cc.phi)) u128_sub(lo[b], hi[b], cc.plo, cc.phi);
+    // symmetric range: x >= P / 2 stands for x - P
+    const unsigned long long hlo = (cc.plo >> 1) | (cc.phi << 63), hhi = cc.phi >> 1;
+    double v;
+    if (u128_ge(lo[b], hi[b], hlo, hhi)) {
+      unsigned long long nlo = cc.plo, nhi = cc.phi;
+      u128_sub(nlo, nhi, lo[b], hi[b]);
+      v = -((double)nhi * 18446744073709551616.0 + (double)nlo);
+    } else {
+      v = (double)hi[b] * 18446744073709551616.0 + (double)lo[b];
+    }
+    double out = alpha * ldexp(v, -(ea[i] + ej));
+    double *cp = C + (size_t)i + (size_t)j * ldc;
+    if (beta != 0.0) out = fma(beta, *cp, out);
+    *cp = out;
+  }
+}
+
+size_t r256(size_t b) { return (b + 255) / 256 * 256; }
+
+const CrtConst &crt_const(int nmod) {
+  static const CrtConst table[CRT_MAX_MOD + 1] = {
+      make_const(1),  make_const(1),  make_const(2),  make_const(3),  make_const(4),
+      make_const(5),  make_const(6),  make_const(7),  make_const(8),  make_const(9),
+      make_const(10), make_const(11), make_const(12), make_const(13)};
+  return table[nmod];
+}
+
+// an operand's area: nmod residue planes (Rp x Kp), Rp exponents, norm partials
+struct OperandLayout {
+  size_t Rp, Kp, plane, planes, expo, part, total;
+};
+
+OperandLayout operand_layout(int R, int K, int nmod) {
+  OperandLayout L{};
+  L.Rp = (size_t)cdiv(R, CRT_T) * CRT_T;
+  L.Kp = (size_t)cdiv(K, CRT_KB) * CRT_KB;
+  L.plane = L.Rp * L.Kp;
+  size_t o = 0;
+  L.planes = o; o += r256(nmod * L.plane);
+  L.expo = o; o += r256(sizeof(int) * L.Rp);
+  L.part = o; o += r256(sizeof(double) * CRT_NCH * (size_t)R);
+  L.total = o;
+  return L;
+}
+
+size_t cplanes_bytes(int M, int N, int nmod) {
+  return r256((size_t)nmod * (size_t)cdiv(M, CRT_T) * CRT_T * (size_t)cdiv(N, CRT_T) * CRT_T);
+}
+
+}  // namespace
+
+int crt_bits(int nmod) {
+  if (nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD) return 0;
+  return crt_const(nmod).q;
+}
+
+size_t crt_operand_bytes(int R, int K, int nmod) {
+  if (R <= 0 || K <= 0 || nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD) return 0;
+  return operand_layout(R, K, nmod).total;
+}
+
+size_t crt_product_bytes(int M, int N, int nmod) {
+  if (M <= 0 || N <= 0 || nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD) return 0;
+  return cplanes_bytes(M, N, nmod);
+}
+
+hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mask, int nmod,
+                     void *area, CrtOperand *out, hipStream_t st) {
+  if (R <= 0 || K <= 0 || K > CRT_MAX_K || nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD || !area ||
+      ((uintptr_t)area & 255) != 0 || mask < 0 || mask > 2)
+    return hipErrorInvalidValue;
+  const OperandLayout Ly = operand_layout(R, K, nmod);
+  const CrtConst &cc = crt_const(nmod);
+  char *a = (char *)area;
+  CrtOperand o;
+  o.planes = (signed char *)(a + Ly.planes);
+  o.expo = (int *)(a + Ly.expo);
+  o.R = R; o.K = K; o.Rp = (int)Ly.Rp; o.Kp = (int)Ly.Kp; o.nmod = nmod;
+  double *part = (double *)(a + Ly.part);
+  const unsigned sg = cdiv((long long)(Ly.Rp / 16) * (Ly.Kp / CRT_KB), 4);
+  if (gather) {
+    hipLaunchKernelGGL(k_crt_norm_gather, dim3(cdiv(R, 64), CRT_NCH), dim3(256), 0, st, R, K, P,
+                       ld, mask, part);
+    hipLaunchKernelGGL(k_crt_split<true>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                       (int)Ly.Kp, P, ld, mask, part, CRT_NCH, cc, o.planes, Ly.plane, o.expo);
+  } else {
+    hipLaunchKernelGGL(k_crt_norm_contig, dim3(cdiv(R, 4)), dim3(256), 0, st, R, K, P, ld, mask,
+                       part);
+    hipLaunchKernelGGL(k_crt_split<false>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                       (int)Ly.Kp, P, ld, mask, part, 1, cc, o.planes, Ly.plane, o.expo);
+  }
+  *out = o;
+  return hipGetLastError();
+}
+
+hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
+                       double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
+                       hipStream_t st, int depth) {
+  const int M = A.R, N = B.R, nmod = A.nmod;
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (A.K != B.K || A.Kp != B.Kp || A.nmod != B.nmod || nmod < CRT_MIN_MOD ||
+      nmod > CRT_MAX_MOD || !cplanes || ((uintptr_t)cplanes & 255) != 0 ||
+      cplanes_size < cplanes_bytes(M, N, nmod) || ((tri & GEMM_C_LOWER) && M != N))
+    return hipErrorInvalidValue;
+  const CrtConst &cc = crt_const(nmod);
+  unsigned char *pc = (unsigned char *)cplanes;
+  const size_t plane_a = (size_t)A.Rp * A.Kp, plane_b = (size_t)B.Rp * B.Kp;
+  const size_t plane_c = (size_t)A.Rp * B.Rp;
+  const int tm = A.Rp / CRT_T, tn = B.Rp / CRT_T;
+  const int ntiles = (tri & GEMM_C_LOWER) ? tm * (tm + 1) / 2 : tm * tn;
+  if (depth == 0) depth = CRT_DEPTH;
+#define UNC_CRT_GEMM(NB)                                                                      \
+  hipLaunchKernelGGL(k_crt_gemm<NB>, dim3(nmod * ntiles), dim3(CRT_THREADS), 0, st, A.Rp,     \
+                     B.Rp, A.Kp, tm, tn, ntiles, tri, cc, A.planes, B.planes, plane_a, plane_b, \
+                     pc, plane_c)
+  if (depth == 3) UNC_CRT_GEMM(3);
+  else if (depth == 4) UNC_CRT_GEMM(4);
+  else if (depth == 5) UNC_CRT_GEMM(5);
+  else return hipErrorInvalidValue;
+#undef UNC_CRT_GEMM
+  hipLaunchKernelGGL(k_crt_rebuild, dim3(cdiv(M, 1024), N), dim3(256), 0, st, M, N, A.Rp,
+                     (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, cc, pc, plane_c, A.expo, B.expo,
+                     C, ldc);
+  return hipGetLastError();
+}
+
+size_t crt_gemm_work_bytes(int M, int N, int K, int nmod) {
+  if (M <= 0 || N <= 0 || K <= 0 || nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD) return 0;
+  return crt_operand_bytes(M, K, nmod) + crt_operand_bytes(N, K, nmod) +
+         crt_product_bytes(M, N, nmod);
+}
+
+hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
+                    int ldb, bool trans_b, double beta, double *C, int ldc, int tri, int nmod,
+                    void *work, size_t work_bytes, hipStream_t st, int depth) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD || K <= 0 || K > CRT_MAX_K)
+    return hipErrorInvalidValue;
+  const size_t ba = crt_operand_bytes(M, K, nmod), bb = crt_operand_bytes(N, K, nmod);
+  const size_t bc = crt_product_bytes(M, N, nmod);
+  if (!work || work_bytes < ba + bb + bc || ((uintptr_t)work & 255) != 0)
+    return hipErrorInvalidValue;
+  char *w = (char *)work;
+  CrtOperand oa, ob;
+  // op(A): rows i, element (i, k) at A[i + k lda]
+  hipError_t e = crt_split(M, K, A, lda, true, (tri & GEMM_A_LOWER) ? 1 : 0, nmod, w, &oa, st);
+  if (e != hipSuccess) return e;
+  // op(B) by columns j: B^T -> element (j, k) at B[j + k ldb]; B -> at B[k + j ldb]
+  if (trans_b)
+    e = crt_split(N, K, B, ldb, true, (tri & GEMM_BT_LOWER) ? 1 : 0, nmod, w + ba, &ob, st);
+  else
+    e = crt_split(N, K, B, ldb, false, (tri & GEMM_B_LOWER) ? 2 : 0, nmod, w + ba, &ob, st);
+  if (e != hipSuccess) return e;
+  return crt_product(alpha, oa, ob, beta, C, ldc, tri, w + ba + bb, bc, st, depth);
+}
+
+}  // namespace unc

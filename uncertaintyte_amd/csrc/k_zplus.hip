@@ -864,6 +864,28 @@ __global__ void k_rank_copy(int n, const double *A, const double *V, double *out
   out[x] = A[x] + s;
 }
 
+// max_i X_ii^2 in one workgroup: a strided pass, then a fixed tree in LDS
+__global__ __launch_bounds__(256) void k_diag_max_sq(int n, const double *X, int ldx, double *out) {
+  __shared__ double m[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double x = X[(size_t)i + (size_t)i * ldx];
+    v = fmax(v, x * x);
+  }
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = m[0];
+}
+
+hipError_t diag_max_sq(int n, const double *X, int ldx, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_max_sq, dim3(1), dim3(256), 0, st, n, X, ldx, out);
+  return hipGetLastError();
+}
+
 hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st) {
   if (k != 7 || n > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rank_copy, dim3(cdiv(n, 256), n), dim3(256), 0, st, n, A, V, out);

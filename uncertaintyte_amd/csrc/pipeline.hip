@@ -40,14 +40,16 @@ const char *kStageNames[12] = {"upload",   "decode",   "point_lists", "assemble"
 struct Sizes {
   int nc, cp, np, npc, nobs, N;
   size_t nn;
-  bool te;  // the plain FP64 TE path (its Taylor chain needs the chain slots)
+  bool te;     // the plain FP64 TE path (its Taylor chain needs the chain slots)
+  bool mixed;  // the Z^+ path with CRT products (needs their residue planes)
+  int moduli;  // their moduli
 };
 
 enum {
   B_OBS_CAM, B_OBS_PT, B_CAM_OFF, B_CAM_LIST, B_SORT_K0, B_SORT_K1, B_SORT_V,
   B_SORT_TMP, B_PT_CNT, B_PT_OFF, B_PT_LIST, B_PT_CURSOR, B_ERR, B_SCAN, B_CAM_PTS, B_PT_CAMS,
   B_GHASH, B_XD, B_US, B_D, B_VI, B_WS, B_PT_HAS, B_CTRL, B_P, B_TP, B_RP, B_NULL, B_DA, B_DB,
-  B_DC, B_DD, B_DE, B_SPLIT, B_COUNT
+  B_DC, B_DD, B_DE, B_SPLIT, B_CRT, B_COUNT
 };
 
 struct Layout {
@@ -57,6 +59,13 @@ struct Layout {
 
 constexpr int GJ_BLOCK = 64;
 size_t zplus_work_doubles(int n);
+size_t zplus_crt_bytes(int n, int nmod);
+// Mixed precision: the Cholesky recursion's nodes of at least ZPLUS_CRT_MIN_N
+// rows form their products on the int8 matrix cores, by default with 12
+// moduli (exact products of operands scaled to 46-bit rows and columns); the
+// smaller nodes stay FP64.
+constexpr int ZPLUS_CRT_MOD = 12;
+constexpr int ZPLUS_CRT_MIN_N = 2048;
 
 // The Taylor chain forms its powers four at a time where one product alone
 // would leave the device half idle (from N = 1024; below, products are small
@@ -91,6 +100,7 @@ Layout make_layout(const Sizes &z) {
                                                  te_batched(z.N) ? 5 * z.nn : (size_t)0)
                      : 0;
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
+  sz[B_CRT] = z.mixed ? zplus_crt_bytes(z.N, z.moduli) : 0;
   Layout L;
   size_t o = 0;
   for (int i = 0; i < B_COUNT; ++i) {
@@ -122,6 +132,8 @@ int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[
     return UNC_ERR_BAD_INPUT;
   if (o->cam_params < 1 || o->cam_params > 9) return UNC_ERR_UNSUPPORTED;
   if (o->num_cams <= 0 || o->num_points < 4) return UNC_ERR_BAD_INPUT;
+  if (o->crt_moduli != 0 && (o->crt_moduli < CRT_MIN_MOD || o->crt_moduli > CRT_MAX_MOD))
+    return UNC_ERR_BAD_INPUT;
   const long long N = (long long)o->num_cams * o->cam_params;
   if ((long long)ncols != N + 3LL * o->num_points) return UNC_ERR_BAD_INPUT;
   if (nrows <= 0 || (nrows & 1)) return UNC_ERR_BAD_INPUT;
@@ -145,6 +157,8 @@ int check_options(const unc_options *o, int nrows, int ncols, Sizes *z, int fix[
   z->N = (int)N;
   z->nn = (size_t)N * (size_t)N;
   z->te = o->precision == UNC_PREC_FP64;
+  z->mixed = o->precision == UNC_PREC_MIXED;
+  z->moduli = o->crt_moduli ? o->crt_moduli : ZPLUS_CRT_MOD;
   return UNC_OK;
 }
 
@@ -207,6 +221,49 @@ int gj_inverse(int n, double *W, double *P3, double *T, double *R, unsigned int 
   return UNC_OK;
 }
 
+// Products of the Z^+ recursion: FP64 MFMA, or (mixed precision) the int8 CRT
+// product with `nmod` moduli for every node of at least `min_n` rows.  The CRT
+// area holds one product's scratch (operands and C residues at the top node's
+// size) and, per recursion depth, the residues of that depth's L21, which serve
+// three products of the node (A22 -= L21 L21^T as both operands, then L21 X11
+// after the recursion into A22).
+constexpr int REC_MAX_DEPTH = 40;
+struct RecProducts {
+  int nmod = 0;          // 0: FP64 throughout
+  int min_n = 0;
+  char *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  char *level[REC_MAX_DEPTH] = {};
+};
+
+// The halves of a recursion node of n rows: n1 a multiple of 64, n1 >= n2.
+inline int rec_split(int n) { return ((n / 2 + 63) / 64) * 64; }
+
+// CRT area of an n x n recursion (0 below ZPLUS_CRT_MIN_N); with `P` also its
+// carve-up from `base`.  At depth d the largest node is the d-th of the chain
+// of first halves (rec_split is monotone).
+size_t zplus_crt_layout(int n, int nmod, char *base, RecProducts *P) {
+  if (n < ZPLUS_CRT_MIN_N) return 0;
+  auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  const int n1 = rec_split(n);
+  size_t o = r256(crt_gemm_work_bytes(n1, n1, n1, nmod));
+  if (P) {
+    P->nmod = nmod;
+    P->min_n = ZPLUS_CRT_MIN_N;
+    P->scratch = base;
+    P->scratch_bytes = o;
+  }
+  int d = 0;
+  for (int m = n; m >= ZPLUS_CRT_MIN_N && d < REC_MAX_DEPTH; m = rec_split(m), ++d) {
+    const int m1 = rec_split(m);
+    if (P) P->level[d] = base + o;
+    o += r256(crt_operand_bytes(m - m1, m1, nmod));
+  }
+  return o;
+}
+
+size_t zplus_crt_bytes(int n, int nmod) { return zplus_crt_layout(n, nmod, nullptr, nullptr); }
+
 // Recursive Cholesky of the SPD block A[off.., off..] (n x n, ld N) with the
 // inverse of its factor: L overwrites A's lower triangle, X = L^-1 goes to the
 // same block of X.  With A = [A11 .; A21 A22]:
@@ -214,23 +271,43 @@ int gj_inverse(int n, double *W, double *P3, double *T, double *R, unsigned int 
 //   (L22, X22) = chol_inv(A22);  X21 = -X22 (L21 X11).
 // W is scratch of (n/2)^2 doubles at least.  Leaves are 64 x 64.
 int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned int *err,
-             hipStream_t s) {
+             const RecProducts &P, int depth, hipStream_t s) {
   if (n <= 64) {
     HC(chol_inv_leaf(n, A + off + (size_t)off * N, N, X + off + (size_t)off * N, N, err, s));
     return UNC_OK;
   }
-  const int n1 = ((n / 2 + 63) / 64) * 64, n2 = n - n1;
-  int rc = chol_inv(N, A, X, W, off, n1, err, s);
+  const int n1 = rec_split(n), n2 = n - n1;
+  int rc = chol_inv(N, A, X, W, off, n1, err, P, depth + 1, s);
   if (rc) return rc;
   double *A21 = A + (off + n1) + (size_t)off * N;
   double *A22 = A + (off + n1) + (size_t)(off + n1) * N;
   const double *X11 = X + off + (size_t)off * N;
   const double *X22 = X + (off + n1) + (size_t)(off + n1) * N;
   double *X21 = X + (off + n1) + (size_t)off * N;
+  if (P.nmod > 0 && n >= P.min_n && depth < REC_MAX_DEPTH && P.level[depth]) {
+    const int nm = P.nmod;
+    const size_t sb = P.scratch_bytes;
+    HC(gemm_crt(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, nm, P.scratch,
+                sb, s));
+    HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
+    CrtOperand L21;
+    HC(crt_split(n2, n1, A21, N, true, 0, nm, P.level[depth], &L21, s));
+    HC(crt_product(-1.0, L21, L21, 1.0, A22, N, GEMM_C_LOWER, P.scratch, sb, s));
+    rc = chol_inv(N, A, X, W, off + n1, n2, err, P, depth + 1, s);
+    if (rc) return rc;
+    // W = L21 X11: the columns of X11 (lower) as the second operand
+    CrtOperand Xc;
+    const size_t bx = crt_operand_bytes(n1, n1, nm);
+    HC(crt_split(n1, n1, X11, N, false, 2, nm, P.scratch, &Xc, s));
+    HC(crt_product(1.0, L21, Xc, 0.0, W, n2, GEMM_B_LOWER, P.scratch + bx, sb - bx, s));
+    HC(gemm_crt(n2, n1, n2, -1.0, X22, N, W, n2, false, 0.0, X21, N, GEMM_A_LOWER, nm, P.scratch,
+                sb, s));
+    return UNC_OK;
+  }
   HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, s));
   HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
   HC(gemm_f64_ex(n2, n2, n1, -1.0, A21, N, A21, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
-  rc = chol_inv(N, A, X, W, off + n1, n2, err, s);
+  rc = chol_inv(N, A, X, W, off + n1, n2, err, P, depth + 1, s);
   if (rc) return rc;
   HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, false, 0.0, W, n2, GEMM_B_LOWER, s));
   HC(gemm_f64_ex(n2, n1, n2, -1.0, X22, N, W, n2, false, 0.0, X21, N, GEMM_A_LOWER, s));
@@ -248,7 +325,8 @@ size_t zplus_work_doubles(int n) {
 // Gauge-fixed Z^+: Z in dA on entry (kept), dB/dC/dD are N x N scratch.  Leaves
 // the cameras' own blocks of Z^+ in Xd (row-major cp x cp per camera).
 int zplus(int N, int newton_steps, double *dA, double *dB, double *dC, double *dD, double *work,
-          const Structure &S, double *Xd, unsigned int *err, hipStream_t s) {
+          const RecProducts &P, const Structure &S, double *Xd, unsigned int *err,
+          hipStream_t s) {
   const size_t nn = (size_t)N * N;
   const int k = GAUGE;
   double *U = work, *Pm = U + 2 * k * (size_t)N, *MU = Pm + 2 * k * (size_t)N;
@@ -265,7 +343,7 @@ int zplus(int N, int newton_steps, double *dA, double *dB, double *dC, double *d
   // Z~ = Z + N0 N0^T, Cholesky with the inverse factor X (upper part zero)
   HC(rank_copy(N, k, dA, N0, dB, s));
   HC(hipMemsetAsync(dC, 0, sizeof(double) * nn, s));
-  int rc = chol_inv(N, dB, dC, dD, 0, N, err, s);
+  int rc = chol_inv(N, dB, dC, dD, 0, N, err, P, 0, s);
   if (rc) return rc;
   const double *X = dC;
   // N = orth(Z~^-1 N0) = orth(X^T X N0)
@@ -536,8 +614,12 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     // gauge-fixed Z^+ path
     const int steps = o->refine >= 0 ? o->refine : 1;
     double *Xd = (double *)ptr(B_XD);
-    rc = zplus(N, steps, dA, dB, dC, dD, (double *)ptr(B_NULL), S, Xd, S.err, s);
+    RecProducts P;
+    if (z.mixed) zplus_crt_layout(N, z.moduli, (char *)ptr(B_CRT), &P);
+    rc = zplus(N, steps, dA, dB, dC, dD, (double *)ptr(B_NULL), P, S, Xd, S.err, s);
     if (rc) return rc;
+    double *d_condlb = (double *)((char *)ptr(B_ERR) + 8);
+    HC(diag_max_sq(N, dC, N, d_condlb, s));
     HC(hipEventRecord(E[7], s));
     HC(hipEventRecord(E[8], s));
     HC(cam_output_cov(S, Xd, Bk.d, d_cam, s));
@@ -551,9 +633,15 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     if (dbg && dbg->X) HC(hipMemcpyAsync(dbg->X, dC, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
     HC(hipMemcpyAsync(&herr, S.err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     HC(hipMemcpyAsync(&kept, S.pt_off + z.npc, sizeof(int), hipMemcpyDeviceToHost, s));
+    double condlb = 0.0;
+    HC(hipMemcpyAsync(&condlb, d_condlb, sizeof(double), hipMemcpyDeviceToHost, s));
     HC(hipStreamSynchronize(s));
     if (herr) return map_err(herr);
-    if (st) fill_stats(0.0, 0, nullptr);
+    if (st) {
+      fill_stats(0.0, 0, nullptr);
+      st->zplus_moduli = P.nmod;
+      st->zplus_cond_lb = condlb;  // X = L^-1: 1 / min L_ii^2 = max X_ii^2
+    }
     return UNC_OK;
   }
   // ---- (d) A = Z + lambda I ; X0 = A^-1
@@ -1140,6 +1228,8 @@ unsigned long long unc_workspace_bytes(const unc_options *o, int num_rows) {
   z.nc = o->num_cams; z.cp = o->cam_params; z.np = o->num_points; z.npc = z.np - 3;
   z.nobs = num_rows / 2; z.N = z.nc * z.cp; z.nn = (size_t)z.N * z.N;
   z.te = o->precision == UNC_PREC_FP64;
+  z.mixed = o->precision == UNC_PREC_MIXED;
+  z.moduli = o->crt_moduli ? o->crt_moduli : ZPLUS_CRT_MOD;
   return (unsigned long long)make_layout(z).total;
 }
 
@@ -1183,7 +1273,7 @@ int dense_inverse_gj(int n, double *W, double *P3, double *T, double *R, unsigne
   return gj_inverse(n, W, P3, T, R, err, s);
 }
 int cholesky_inverse(int n, double *A, double *X, double *W, unsigned int *err, hipStream_t s) {
-  return chol_inv(n, A, X, W, 0, n, err, s);
+  return chol_inv(n, A, X, W, 0, n, err, RecProducts(), 0, s);
 }
 int error_code(unsigned int device_error_bits) { return map_err(device_error_bits); }
 }  // namespace unc
