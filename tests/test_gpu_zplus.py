@@ -104,6 +104,20 @@ def test_mixed_uses_int8_crt_products():
     assert 0 < d12 < 1e-7 and d12 < d10 < 1e-5
 
 
+def test_mixed_odd_split_node():
+    """441 cameras (N = 3969): the top node's halves are 1984 and 1985 rows, the
+    second half the larger; the mixed path sizes its residue areas for it."""
+    from uncertaintyte_amd import serve
+    j = serve.request_scene(441, 60, 77)
+    print("mixed 441 cameras: mixed vs FP64 Z+", flush=True)
+    m = u.compute_covariances(j, precision=u.PREC_MIXED)
+    f = u.compute_covariances(j, precision=u.PREC_ZPLUS_FP64)
+    assert m.stats.zplus_moduli == 12
+    d = block_rel(m.cam, f.cam).max()
+    print(f"  mixed vs FP64 {d:.2e}", flush=True)
+    assert d < 1e-6
+
+
 def test_mixed_rejects_bad_moduli(daliborka):
     for m in (5, 14, -1):
         with pytest.raises(u.UncError) as e:

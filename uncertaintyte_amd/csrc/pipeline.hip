@@ -28,6 +28,10 @@ struct unc_context {
   int max_lds = 65536;
   hipEvent_t stage_ev[16];
   hipEvent_t gemm_ev[2 * 24];
+  // the Z^+ recursion's second stream (default priority) and its fork / join
+  // events, one pair per recursion depth
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev[40], join_ev[40];
   std::mutex mu;
 };
 
@@ -234,32 +238,104 @@ struct RecProducts {
   char *scratch = nullptr;
   size_t scratch_bytes = 0;
   char *level[REC_MAX_DEPTH] = {};
+  // Second stream: a node of at least fork_min_n rows forms L21 X11 there while
+  // the main stream descends into A22 (that product needs only L21 and X11); its
+  // result goes to the depth's own region wside[depth] of the recursion scratch,
+  // its CRT scratch to side_scratch.
+  hipStream_t side = nullptr;
+  const hipEvent_t *fork = nullptr, *join = nullptr;
+  int fork_min_n = 0;
+  double *wside[REC_MAX_DEPTH] = {};
+  char *side_scratch = nullptr;
+  size_t side_scratch_bytes = 0;
 };
 
-// The halves of a recursion node of n rows: n1 a multiple of 64, n1 >= n2.
+// Nodes of at least this many rows fork their L21 X11 onto the second stream.
+constexpr int REC_FORK_MIN_N = 1024;
+
+// The first half of a recursion node of n rows: the multiple of 64 at or just
+// above n / 2 (the second half, n - n1, can be one row larger when n is odd).
 inline int rec_split(int n) { return ((n / 2 + 63) / 64) * 64; }
 
 // CRT area of an n x n recursion (0 below ZPLUS_CRT_MIN_N); with `P` also its
-// carve-up from `base`.  At depth d the largest node is the d-th of the chain
-// of first halves (rec_split is monotone).
+// carve-up from `base`.  Walks the distinct node sizes depth by depth (a node's
+// halves differ by less than 64, in either order): the scratch covers the
+// largest node's products, each depth's L21 area its largest node's L21.
 size_t zplus_crt_layout(int n, int nmod, char *base, RecProducts *P) {
   if (n < ZPLUS_CRT_MIN_N) return 0;
   auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
-  const int n1 = rec_split(n);
-  size_t o = r256(crt_gemm_work_bytes(n1, n1, n1, nmod));
+  size_t scratch = 0, side = 0, lvl[REC_MAX_DEPTH] = {};
+  std::vector<int> cur{n};
+  int depths = 0;
+  for (; depths < REC_MAX_DEPTH && !cur.empty(); ++depths) {
+    std::vector<int> next;
+    for (int m : cur) {
+      if (m < ZPLUS_CRT_MIN_N) continue;
+      const int m1 = rec_split(m), m2 = m - m1, h = std::max(m1, m2);
+      scratch = std::max(scratch, r256(crt_gemm_work_bytes(h, h, h, nmod)));
+      // the forked L21 X11: X11's residues and the product's
+      if (m >= REC_FORK_MIN_N)
+        side = std::max(side, r256(crt_operand_bytes(m1, m1, nmod)) +
+                                  r256(crt_product_bytes(m2, m1, nmod)));
+      lvl[depths] = std::max(lvl[depths], r256(crt_operand_bytes(m2, m1, nmod)));
+      next.push_back(m1);
+      next.push_back(m2);
+    }
+    std::sort(next.begin(), next.end());
+    next.erase(std::unique(next.begin(), next.end()), next.end());
+    cur.swap(next);
+  }
+  size_t o = scratch + side;
   if (P) {
     P->nmod = nmod;
     P->min_n = ZPLUS_CRT_MIN_N;
     P->scratch = base;
-    P->scratch_bytes = o;
+    P->scratch_bytes = scratch;
+    P->side_scratch = base + scratch;
+    P->side_scratch_bytes = side;
   }
-  int d = 0;
-  for (int m = n; m >= ZPLUS_CRT_MIN_N && d < REC_MAX_DEPTH; m = rec_split(m), ++d) {
-    const int m1 = rec_split(m);
+  for (int d = 0; d < depths; ++d) {
+    if (!lvl[d]) continue;
     if (P) P->level[d] = base + o;
-    o += r256(crt_operand_bytes(m - m1, m1, nmod));
+    o += lvl[d];
   }
   return o;
+}
+
+// Regions of the recursion scratch W (n x n doubles): the main stream's
+// products use W from its start (at most max n2 n1 doubles), and each depth's
+// forked L21 X11 its own region after that.  Returns false when they would not
+// fit (then nothing forks).
+bool rec_fork_layout(int n, double *W, RecProducts *P) {
+  size_t main = 0, wsz[REC_MAX_DEPTH] = {};
+  std::vector<int> cur{n};
+  int depths = 0;
+  for (; depths < REC_MAX_DEPTH && !cur.empty(); ++depths) {
+    std::vector<int> next;
+    for (int m : cur) {
+      if (m <= 64) continue;
+      const int m1 = rec_split(m), m2 = m - m1;
+      main = std::max(main, (size_t)m2 * m1);
+      if (m >= REC_FORK_MIN_N) wsz[depths] = std::max(wsz[depths], (size_t)m2 * m1);
+      next.push_back(m1);
+      next.push_back(m2);
+    }
+    std::sort(next.begin(), next.end());
+    next.erase(std::unique(next.begin(), next.end()), next.end());
+    cur.swap(next);
+  }
+  size_t o = (main + 31) / 32 * 32;
+  for (int d = 0; d < depths; ++d) {
+    if (!wsz[d]) continue;
+    P->wside[d] = W + o;
+    o += (wsz[d] + 31) / 32 * 32;
+  }
+  if (o > (size_t)n * n) {
+    for (auto &w : P->wside) w = nullptr;
+    return false;
+  }
+  P->fork_min_n = REC_FORK_MIN_N;
+  return true;
 }
 
 size_t zplus_crt_bytes(int n, int nmod) { return zplus_crt_layout(n, nmod, nullptr, nullptr); }
@@ -284,33 +360,53 @@ int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned in
   const double *X11 = X + off + (size_t)off * N;
   const double *X22 = X + (off + n1) + (size_t)(off + n1) * N;
   double *X21 = X + (off + n1) + (size_t)off * N;
-  if (P.nmod > 0 && n >= P.min_n && depth < REC_MAX_DEPTH && P.level[depth]) {
-    const int nm = P.nmod;
-    const size_t sb = P.scratch_bytes;
+  const bool crt = P.nmod > 0 && n >= P.min_n && depth < REC_MAX_DEPTH && P.level[depth];
+  const bool fork = P.side && n >= P.fork_min_n && depth < REC_MAX_DEPTH && P.wside[depth] &&
+                    (!crt || P.side_scratch);
+  const int nm = P.nmod;
+  const size_t sb = P.scratch_bytes;
+  // L21 = A21 X11^T (into W, then over A21)
+  if (crt)
     HC(gemm_crt(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, nm, P.scratch,
                 sb, s));
-    HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
-    CrtOperand L21;
-    HC(crt_split(n2, n1, A21, N, true, 0, nm, P.level[depth], &L21, s));
-    HC(crt_product(-1.0, L21, L21, 1.0, A22, N, GEMM_C_LOWER, P.scratch, sb, s));
-    rc = chol_inv(N, A, X, W, off + n1, n2, err, P, depth + 1, s);
-    if (rc) return rc;
-    // W = L21 X11: the columns of X11 (lower) as the second operand
-    CrtOperand Xc;
-    const size_t bx = crt_operand_bytes(n1, n1, nm);
-    HC(crt_split(n1, n1, X11, N, false, 2, nm, P.scratch, &Xc, s));
-    HC(crt_product(1.0, L21, Xc, 0.0, W, n2, GEMM_B_LOWER, P.scratch + bx, sb - bx, s));
-    HC(gemm_crt(n2, n1, n2, -1.0, X22, N, W, n2, false, 0.0, X21, N, GEMM_A_LOWER, nm, P.scratch,
-                sb, s));
-    return UNC_OK;
-  }
-  HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, s));
+  else
+    HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, s));
   HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
-  HC(gemm_f64_ex(n2, n2, n1, -1.0, A21, N, A21, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
+  CrtOperand L21;
+  if (crt) HC(crt_split(n2, n1, A21, N, true, 0, nm, P.level[depth], &L21, s));
+  // W3 = L21 X11, here or on the second stream
+  double *W3 = fork ? P.wside[depth] : W;
+  auto l21_x11 = [&](hipStream_t st, char *cs, size_t csb) -> hipError_t {
+    if (!crt) return gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, false, 0.0, W3, n2, GEMM_B_LOWER, st);
+    CrtOperand Xc;  // the columns of X11 (lower) as the second operand
+    const size_t bx = crt_operand_bytes(n1, n1, nm);
+    hipError_t e = crt_split(n1, n1, X11, N, false, 2, nm, cs, &Xc, st);
+    if (e != hipSuccess) return e;
+    return crt_product(1.0, L21, Xc, 0.0, W3, n2, GEMM_B_LOWER, cs + bx, csb - bx, st);
+  };
+  if (fork) {
+    HC(hipEventRecord(P.fork[depth], s));
+    HC(hipStreamWaitEvent(P.side, P.fork[depth], 0));
+    HC(l21_x11(P.side, P.side_scratch, P.side_scratch_bytes));
+    HC(hipEventRecord(P.join[depth], P.side));
+  }
+  // A22 -= L21 L21^T, then the recursion into A22
+  if (crt)
+    HC(crt_product(-1.0, L21, L21, 1.0, A22, N, GEMM_C_LOWER, P.scratch, sb, s));
+  else
+    HC(gemm_f64_ex(n2, n2, n1, -1.0, A21, N, A21, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
   rc = chol_inv(N, A, X, W, off + n1, n2, err, P, depth + 1, s);
   if (rc) return rc;
-  HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, false, 0.0, W, n2, GEMM_B_LOWER, s));
-  HC(gemm_f64_ex(n2, n1, n2, -1.0, X22, N, W, n2, false, 0.0, X21, N, GEMM_A_LOWER, s));
+  if (fork)
+    HC(hipStreamWaitEvent(s, P.join[depth], 0));
+  else
+    HC(l21_x11(s, P.scratch, sb));
+  // X21 = -X22 W3
+  if (crt)
+    HC(gemm_crt(n2, n1, n2, -1.0, X22, N, W3, n2, false, 0.0, X21, N, GEMM_A_LOWER, nm, P.scratch,
+                sb, s));
+  else
+    HC(gemm_f64_ex(n2, n1, n2, -1.0, X22, N, W3, n2, false, 0.0, X21, N, GEMM_A_LOWER, s));
   return UNC_OK;
 }
 
@@ -616,6 +712,11 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     double *Xd = (double *)ptr(B_XD);
     RecProducts P;
     if (z.mixed) zplus_crt_layout(N, z.moduli, (char *)ptr(B_CRT), &P);
+    if (rec_fork_layout(N, dD, &P)) {
+      P.side = ctx->side;
+      P.fork = ctx->fork_ev;
+      P.join = ctx->join_ev;
+    }
     rc = zplus(N, steps, dA, dB, dC, dD, (double *)ptr(B_NULL), P, S, Xd, S.err, s);
     if (rc) return rc;
     double *d_condlb = (double *)((char *)ptr(B_ERR) + 8);
@@ -838,6 +939,9 @@ int context_init(unc_context *c) {
     c->max_lds = lds;
   for (auto &e : c->stage_ev) HC(hipEventCreate(&e));
   for (auto &e : c->gemm_ev) HC(hipEventCreate(&e));
+  HC(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  for (auto &e : c->fork_ev) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto &e : c->join_ev) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return scratch_setup(c->device);
 }
 
@@ -967,6 +1071,8 @@ int unc_context_create(int device, unc_context **out) {
   c->device = device;
   for (auto &e : c->stage_ev) e = nullptr;
   for (auto &e : c->gemm_ev) e = nullptr;
+  for (auto &e : c->fork_ev) e = nullptr;
+  for (auto &e : c->join_ev) e = nullptr;
   const int rc = context_init(c);
   if (rc) {
     unc_context_destroy(c);
@@ -984,6 +1090,12 @@ void unc_context_destroy(unc_context *c) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->gemm_ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  for (auto &e : c->fork_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->join_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ws) (void)hipFreeAsync(c->ws, c->stream);
   if (c->io) (void)hipFreeAsync(c->io, c->stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
