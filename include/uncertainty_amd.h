@@ -271,6 +271,13 @@ int unc_abi_version(void);
 /* Largest LDS allocation of one workgroup on the device (bytes), -1 on error. */
 int unc_device_lds_bytes(int device);
 
+/* Page-locked host memory (hipHostMalloc) for Jacobians and outputs that travel
+ * over PCIe often, e.g. a server's request payloads: copies from it run at the
+ * DMA engines' rate instead of being staged through pageable buffers.  No
+ * counterpart in the reference.  UNC_ERR_OOM when the allocation fails. */
+int unc_host_alloc(size_t bytes, void **ptr);
+void unc_host_free(void *ptr);
+
 /* ---- request server: many reconstructions of different sizes -----------------
  * No counterpart in the reference (which handles one problem per call).
  * unc_server_submit queues a request and returns a ticket at once; the host

@@ -116,3 +116,21 @@ def test_config4_poisson_512_requests():
     s = res.summary()
     print(s, flush=True)
     assert s["requests"] == 512 and s["failed"] == 0
+
+
+def test_pinned_payloads_same_results(small_scenes):
+    """Request payloads in page-locked host memory (unc_host_alloc) give the same
+    results as pageable ones, through the server and the host API."""
+    j = small_scenes[0]
+    jp = j.pinned()
+    assert jp.vals.ctypes.data != j.vals.ctypes.data and np.array_equal(jp.vals, j.vals)
+    a = u.compute_covariances(j, precision=u.PREC_MIXED)
+    b = u.compute_covariances(jp, precision=u.PREC_MIXED)
+    assert np.array_equal(a.cam, b.cam) and np.array_equal(a.pt, b.pt)
+    srv = u.Server(slots=2)
+    tk = [srv.submit(s, precision=u.PREC_MIXED) for s in (jp, j, jp)]
+    outs = [srv.wait(t)[0] for t in tk]
+    srv.close()
+    for r in outs:
+        assert np.array_equal(r.cam, a.cam) and np.array_equal(r.pt, a.pt)
+    del jp  # the pinned block is freed with its last view

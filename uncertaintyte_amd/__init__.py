@@ -116,6 +116,9 @@ def lib():
     L.unc_stage_name.argtypes = [ctypes.c_int]
     L.unc_stage_name.restype = ctypes.c_char_p
     L.unc_abi_version.restype = ctypes.c_int
+    L.unc_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+    L.unc_host_free.argtypes = [ctypes.c_void_p]
+    L.unc_host_free.restype = None
     L.unc_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
     L.unc_server_default_config.restype = None
     L.unc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
@@ -165,6 +168,38 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_longlong), ("completed", ctypes.c_longlong),
                 ("admitted_out_of_order", ctypes.c_longlong), ("held_bytes", ctypes.c_size_t),
                 ("peak_held_bytes", ctypes.c_size_t), ("max_concurrent", ctypes.c_int)]
+
+
+class _PinnedBlock:
+    """Owner of one unc_host_alloc block; freed with the last array viewing it."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        self._free = lib().unc_host_free  # held: module globals may be gone at exit
+        _check(lib().unc_host_alloc(max(nbytes, 1), ctypes.byref(self.ptr)), "unc_host_alloc")
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if self.ptr:
+            self._free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in page-locked host memory (unc_host_alloc): uploads from it and
+    downloads into it run at the DMA engines' rate."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) if np.ndim(shape) else int(shape)
+    blk = _PinnedBlock(n * dt.itemsize)
+    buf = (ctypes.c_char * max(n * dt.itemsize, 1)).from_address(blk.ptr.value)
+    buf._owner = blk  # keeps the block alive as long as any view of the buffer
+    return np.frombuffer(buf, dtype=dt, count=n).reshape(shape)
+
+
+def pinned_copy(a: np.ndarray) -> np.ndarray:
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
 
 
 def make_options(jac, precision: int = PREC_FP64, refine: bool = True, device: int = 0,

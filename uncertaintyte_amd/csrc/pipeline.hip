@@ -1366,6 +1366,21 @@ const char *unc_stage_name(int stage) {
 
 int unc_abi_version(void) { return UNC_ABI_VERSION; }
 
+int unc_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr || bytes == 0) return UNC_ERR_BAD_INPUT;
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return UNC_ERR_OOM;
+  }
+  return UNC_OK;
+}
+
+void unc_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+}
+
 int unc_device_lds_bytes(int device) {
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {

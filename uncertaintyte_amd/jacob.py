@@ -47,6 +47,14 @@ class Jacobian:
     def nnz(self) -> int:
         return int(self.rows[-1]) if len(self.rows) else 0
 
+    def pinned(self) -> "Jacobian":
+        """The same Jacobian with its arrays in page-locked host memory
+        (uncertaintyte_amd.pinned_copy), for repeated fast uploads."""
+        from . import pinned_copy
+        return dataclasses.replace(self, rows=pinned_copy(np.ascontiguousarray(self.rows, np.int32)),
+                                   cols=pinned_copy(np.ascontiguousarray(self.cols, np.int32)),
+                                   vals=pinned_copy(np.ascontiguousarray(self.vals, np.float64)))
+
     def fix_array(self) -> np.ndarray:
         return np.array(self.fix if self.fix is not None else (-1, -1, -1), dtype=np.int32)
 

@@ -42,7 +42,10 @@ def _gen(args):
 
 
 def make_requests(n_requests: int = 512, n_distinct: int = 64, seed: int = 11,
-                  workers: int = 8, max_cams: int = 800):
+                  workers: int = 8, max_cams: int = 800, pinned: bool = False):
+    """(scenes, order): n_distinct seeded request scenes and the request order.
+    pinned: the scenes' arrays in page-locked host memory (a server's receive
+    buffers), so that their uploads run at the DMA engines' rate."""
     cams, ppc = request_sizes(n_distinct, seed)
     cams = np.minimum(cams, max_cams)
     args = [(int(c), int(p), seed * 1000 + i) for i, (c, p) in enumerate(zip(cams, ppc))]
@@ -51,6 +54,8 @@ def make_requests(n_requests: int = 512, n_distinct: int = 64, seed: int = 11,
             scenes = list(ex.map(_gen, args))
     else:
         scenes = [_gen(a) for a in args]
+    if pinned:
+        scenes = [s.pinned() for s in scenes]
     order = np.random.default_rng(seed + 1).integers(0, n_distinct, n_requests)
     return scenes, order
 
