@@ -312,30 +312,24 @@ __global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
   const int m = cc.m[mod];
   const double mi = cc.minv[mod];
   unsigned char *pc = PC + (size_t)mod * plane_c;
+  // C plane in 16 x 16 blocks (column-block major), each the 256 bytes of one
+  // MFMA output block in lane order: one contiguous store per block and wave
+  const size_t rblocks = (size_t)Mp / 16;
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int col = n0 + wn * 128 + j * 16 + (lane & 15);
-      const int row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4);
+      const size_t rb = (size_t)(m0 + wm * 128 + i * 16) / 16;
+      const size_t cb = (size_t)(n0 + wn * 128 + j * 16) / 16;
       const unsigned int u = crt_reduce(acc[i][j][0], m, mi) |
                              crt_reduce(acc[i][j][1], m, mi) << 8 |
                              crt_reduce(acc[i][j][2], m, mi) << 16 |
                              crt_reduce(acc[i][j][3], m, mi) << 24;
-      *(unsigned int *)(pc + (size_t)row + (size_t)col * Mp) = u;
+      *(unsigned int *)(pc + (cb * rblocks + rb) * 256 + 4 * lane) = u;
     }
 }
 
 // 128-bit helpers on two 64-bit limbs
-__device__ __forceinline__ void u128_addmul(unsigned long long &lo, unsigned long long &hi,
-                                            unsigned long long r, unsigned long long blo,
-                                            unsigned long long bhi) {
-  const unsigned long long plo = r * blo;
-  const unsigned long long phi = __umul64hi(r, blo) + r * bhi;
-  lo += plo;
-  hi += phi + (lo < plo ? 1ull : 0ull);
-}
-
 __device__ __forceinline__ bool u128_ge(unsigned long long alo, unsigned long long ahi,
                                         unsigned long long blo, unsigned long long bhi) {
   return ahi > bhi || (ahi == bhi && alo >= blo);
@@ -348,49 +342,76 @@ __device__ __forceinline__ void u128_sub(unsigned long long &alo, unsigned long 
   ahi -= bhi + borrow;
 }
 
-// C_ij = alpha 2^-(ea_i + eb_j) crt(r_1 .. r_s) + beta C_ij for the rows i of
-// column j = blockIdx.y, four consecutive rows per thread (one 32-bit load per
-// plane); with `lower` only i >= j is written.
+// C_ij = alpha 2^-(ea_i + eb_j) crt(r_1 .. r_s) + beta C_ij.  One thread per
+// four rows of one column, read from the blocked C planes as one 32-bit word
+// per plane (the lane's word of a 16 x 16 block); 256 threads take four row
+// blocks of one column block.  The CRT sum sum_l r_l E_l is formed on 32-bit
+// limbs of E_l (s_k = sum_l r_l e_lk < 13 * 2^40, one 32 x 32 -> 64-bit
+// multiply-add each), then carried into 128 bits.  With `lower` only i >= j is
+// written.
 __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int lower, double alpha,
                                                      double beta, CrtConst cc,
                                                      const unsigned char *__restrict__ PC,
                                                      size_t plane_c, const int *__restrict__ ea,
                                                      const int *__restrict__ eb, double *C,
                                                      int ldc) {
-  const int j = blockIdx.y, i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int lane = threadIdx.x & 63;
+  const int rb = blockIdx.x * 4 + (threadIdx.x >> 6), cb = blockIdx.y;
+  const int j = cb * 16 + (lane & 15), i0 = rb * 16 + 4 * (lane >> 4);
   if (i0 >= M || j >= N) return;
   if (lower && i0 + 3 < j) return;
-  unsigned long long lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+  const size_t at = ((size_t)cb * (Mp / 16) + rb) * 256 + 4 * lane;
+  unsigned long long sk[4][4] = {};  // [row][limb]
   for (int l = 0; l < cc.nmod; ++l) {
-    const unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + (size_t)i0 +
-                                                   (size_t)j * Mp);
+    const unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + at);
+    const unsigned int e0 = (unsigned int)cc.elo[l], e1 = (unsigned int)(cc.elo[l] >> 32);
+    const unsigned int e2 = (unsigned int)cc.ehi[l], e3 = (unsigned int)(cc.ehi[l] >> 32);
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      u128_addmul(lo[b], hi[b], (u >> (8 * b)) & 255u, cc.elo[l], cc.ehi[l]);
+    for (int b = 0; b < 4; ++b) {
+      const unsigned long long r = (u >> (8 * b)) & 255u;
+      sk[b][0] += r * e0;
+      sk[b][1] += r * e1;
+      sk[b][2] += r * e2;
+      sk[b][3] += r * e3;
+    }
   }
   const int ej = eb[j];
+  const unsigned long long hlo = (cc.plo >> 1) | (cc.phi << 63), hhi = cc.phi >> 1;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int i = i0 + b;
     if (i >= M || (lower && i < j)) continue;
+    // carry the limb sums into x = lo + 2^64 hi
+    unsigned long long c = sk[b][0];
+    unsigned long long lo = c & 0xffffffffull;
+    c = (c >> 32) + sk[b][1];
+    lo |= c << 32;
+    c = (c >> 32) + sk[b][2];
+    unsigned long long hi = c & 0xffffffffull;
+    c = (c >> 32) + sk[b][3];
+    hi |= c << 32;
     // x mod P: the quotient (< 2^12) from FP64, one below to stay non-negative
-    const double xd = (double)hi[b] * 18446744073709551616.0 + (double)lo[b];
-    double qd = floor(xd * cc.pinv) - 1.0;
+    const double xd = (double)hi * 18446744073709551616.0 + (double)lo;
+    const double qd = floor(xd * cc.pinv) - 1.0;
     if (qd > 0.0) {
-      unsigned long long qlo = 0, qhi = 0;
-      u128_addmul(qlo, qhi, (unsigned long long)qd, cc.plo, cc.phi);
-      u128_sub(lo[b], hi[b], qlo, qhi);
+      const unsigned long long q = (unsigned long long)qd;
+      const unsigned long long plo = cc.plo, p0 = plo & 0xffffffffull, p1 = plo >> 32;
+      unsigned long long t = q * p0;
+      unsigned long long qlo = t & 0xffffffffull;
+      t = (t >> 32) + q * p1;
+      qlo |= t << 32;
+      const unsigned long long qhi = (t >> 32) + q * cc.phi;
+      u128_sub(lo, hi, qlo, qhi);
     }
-    while (u128_ge(lo[b], hi[b], cc.plo, cc.phi)) u128_sub(lo[b], hi[b], cc.plo, cc.phi);
+    while (u128_ge(lo, hi, cc.plo, cc.phi)) u128_sub(lo, hi, cc.plo, cc.phi);
     // symmetric range: x >= P / 2 stands for x - P
-    const unsigned long long hlo = (cc.plo >> 1) | (cc.phi << 63), hhi = cc.phi >> 1;
     double v;
-    if (u128_ge(lo[b], hi[b], hlo, hhi)) {
+    if (u128_ge(lo, hi, hlo, hhi)) {
       unsigned long long nlo = cc.plo, nhi = cc.phi;
-      u128_sub(nlo, nhi, lo[b], hi[b]);
+      u128_sub(nlo, nhi, lo, hi);
       v = -((double)nhi * 18446744073709551616.0 + (double)nlo);
     } else {
-      v = (double)hi[b] * 18446744073709551616.0 + (double)lo[b];
+      v = (double)hi * 18446744073709551616.0 + (double)lo;
     }
     double out = alpha * ldexp(v, -(ea[i] + ej));
     double *cp = C + (size_t)i + (size_t)j * ldc;
@@ -502,7 +523,7 @@ hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, d
   else if (depth == 5) UNC_CRT_GEMM(5);
   else return hipErrorInvalidValue;
 #undef UNC_CRT_GEMM
-  hipLaunchKernelGGL(k_crt_rebuild, dim3(cdiv(M, 1024), N), dim3(256), 0, st, M, N, A.Rp,
+  hipLaunchKernelGGL(k_crt_rebuild, dim3(A.Rp / 64, B.Rp / 16), dim3(256), 0, st, M, N, A.Rp,
                      (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, cc, pc, plane_c, A.expo, B.expo,
                      C, ldc);
   return hipGetLastError();
