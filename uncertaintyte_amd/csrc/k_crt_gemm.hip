@@ -142,44 +142,24 @@ __device__ __forceinline__ int crt_residue(double x, int m, double minv) {
   return r;
 }
 
-// Residue planes of an operand with R rows over K (element (r, k) at
-// P[r + k ld] when GATHER, else at P[k + r ld]; `mask` keeps k <= r (1) or
-// k >= r (2)): one wave per 16 x 64 block, lane l taking row l % 16 and
-// k 16 (l / 16) .. + 15.  expo[r] = the row's scale exponent, from its norm
-// partials (nch of them, summed in order).
-template <bool GATHER>
-__global__ __launch_bounds__(256) void k_crt_split(int R, int K, int Rp, int Kp,
-                                                   const double *__restrict__ P, int ld, int mask,
-                                                   const double *__restrict__ part, int nch,
-                                                   CrtConst cc, signed char *__restrict__ planes,
-                                                   size_t plane_bytes, int *__restrict__ expo) {
-  const int lane = threadIdx.x & 63;
-  const int kblocks = Kp / CRT_KB;
-  const long long bi = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (bi >= (long long)(Rp / 16) * kblocks) return;
-  const int rb = (int)(bi / kblocks), kb = (int)(bi - (long long)rb * kblocks);
-  const int r = rb * 16 + (lane & 15), k0 = kb * CRT_KB + 16 * (lane >> 4);
-  int e = 0;
-  if (r < R) {
-    double ss = 0.0;
-    for (int c = 0; c < nch; ++c) ss += part[(size_t)c * R + r];
-    if (ss > 0.0) {
-      int E;
-      (void)frexp(ss, &E);                 // ss < 2^E, so |row| < 2^ceil(E/2)
-      e = cc.q - 2 - ((E + 1) >> 1);       // |row| 2^e < 2^(q-2)
-    }
-  }
-  if (kb == 0 && lane < 16) expo[r] = e;
-  double x[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const int k = k0 + t;
-    double v = 0.0;
-    if (r < R && k < K && crt_keep(mask, r, k))
-      v = GATHER ? P[(size_t)r + (size_t)k * ld] : P[(size_t)k + (size_t)r * ld];
-    x[t] = rint(ldexp(v, e));
-  }
-  const size_t at = ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane;
+// The scale exponent of row r from its norm partials (nch of them, summed in
+// order): |row| 2^e < 2^(q-2).
+__device__ __forceinline__ int crt_row_exponent(int r, int R, const double *__restrict__ part,
+                                                int nch, int q) {
+  if (r >= R) return 0;
+  double ss = 0.0;
+  for (int c = 0; c < nch; ++c) ss += part[(size_t)c * R + r];
+  if (!(ss > 0.0)) return 0;
+  int E;
+  (void)frexp(ss, &E);              // ss < 2^E, so |row| < 2^ceil(E/2)
+  return q - 2 - ((E + 1) >> 1);
+}
+
+// The residues of the 16 integers x (a lane's row and 16 k of one 16 x 64 block)
+// for every modulus, 16 bytes per plane at the lane's slot of the block.
+__device__ __forceinline__ void crt_emit(const double (&x)[16], const CrtConst &cc,
+                                         signed char *__restrict__ planes, size_t plane_bytes,
+                                         size_t at) {
   for (int l = 0; l < cc.nmod; ++l) {
     const int m = cc.m[l];
     const double mi = cc.minv[l];
@@ -194,6 +174,72 @@ __global__ __launch_bounds__(256) void k_crt_split(int R, int K, int Rp, int Kp,
     }
     *(v4i *)(planes + (size_t)l * plane_bytes + at) = (v4i){w[0], w[1], w[2], w[3]};
   }
+}
+
+// Residue planes of an operand with R rows over K whose element (r, k) sits at
+// P[r + k ld] (rows gathered across k; `mask` keeps k <= r (1) or k >= r (2)):
+// one wave per 16 x 64 block, lane l taking row l % 16 and k 16 (l / 16) .. + 15,
+// so each load reads 16 consecutive rows.  expo[r] = the row's scale exponent.
+__global__ __launch_bounds__(256) void k_crt_split_gather(int R, int K, int Rp, int Kp,
+                                                          const double *__restrict__ P, int ld,
+                                                          int mask,
+                                                          const double *__restrict__ part, int nch,
+                                                          CrtConst cc,
+                                                          signed char *__restrict__ planes,
+                                                          size_t plane_bytes,
+                                                          int *__restrict__ expo) {
+  const int lane = threadIdx.x & 63;
+  const int kblocks = Kp / CRT_KB;
+  const long long bi = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (bi >= (long long)(Rp / 16) * kblocks) return;
+  const int rb = (int)(bi / kblocks), kb = (int)(bi - (long long)rb * kblocks);
+  const int r = rb * 16 + (lane & 15), k0 = kb * CRT_KB + 16 * (lane >> 4);
+  const int e = crt_row_exponent(r, R, part, nch, cc.q);
+  if (kb == 0 && lane < 16) expo[r] = e;
+  double x[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int k = k0 + t;
+    const double v = (r < R && k < K && crt_keep(mask, r, k)) ? P[(size_t)r + (size_t)k * ld] : 0.0;
+    x[t] = rint(ldexp(v, e));
+  }
+  crt_emit(x, cc, planes, plane_bytes, ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
+}
+
+// The same for an operand whose rows are contiguous in k (element (r, k) at
+// P[k + r ld]): a workgroup stages 16 rows x 256 k through LDS, each load
+// instruction reading 128 consecutive bytes of four rows, then each wave cuts one
+// 16 x 64 block from LDS.  One norm partial per row.
+__global__ __launch_bounds__(256) void k_crt_split_contig(int R, int K, int Rp, int Kp,
+                                                          const double *__restrict__ P, int ld,
+                                                          int mask,
+                                                          const double *__restrict__ part,
+                                                          CrtConst cc,
+                                                          signed char *__restrict__ planes,
+                                                          size_t plane_bytes,
+                                                          int *__restrict__ expo) {
+  __shared__ double T[16][4 * CRT_KB + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kblocks = Kp / CRT_KB, kgroups = (kblocks + 3) / 4;
+  const int rb = blockIdx.x / kgroups, kg = blockIdx.x - rb * kgroups;
+  {
+    const int lr = tid >> 4, c = tid & 15, r = rb * 16 + lr;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int kl = c + 16 * i, k = kg * 4 * CRT_KB + kl;
+      T[lr][kl] = (r < R && k < K && crt_keep(mask, r, k)) ? P[(size_t)k + (size_t)r * ld] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int kb = kg * 4 + wave;
+  if (kb >= kblocks) return;
+  const int r = rb * 16 + (lane & 15), kl0 = wave * CRT_KB + 16 * (lane >> 4);
+  const int e = crt_row_exponent(r, R, part, 1, cc.q);
+  if (kb == 0 && lane < 16) expo[r] = e;
+  double x[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) x[t] = rint(ldexp(T[lane & 15][kl0 + t], e));
+  crt_emit(x, cc, planes, plane_bytes, ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
 }
 
 // Residue (in [0, m)) of an int32 product sum.
@@ -482,17 +528,18 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
   o.expo = (int *)(a + Ly.expo);
   o.R = R; o.K = K; o.Rp = (int)Ly.Rp; o.Kp = (int)Ly.Kp; o.nmod = nmod;
   double *part = (double *)(a + Ly.part);
-  const unsigned sg = cdiv((long long)(Ly.Rp / 16) * (Ly.Kp / CRT_KB), 4);
   if (gather) {
+    const unsigned sg = cdiv((long long)(Ly.Rp / 16) * (Ly.Kp / CRT_KB), 4);
     hipLaunchKernelGGL(k_crt_norm_gather, dim3(cdiv(R, 64), CRT_NCH), dim3(256), 0, st, R, K, P,
                        ld, mask, part);
-    hipLaunchKernelGGL(k_crt_split<true>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+    hipLaunchKernelGGL(k_crt_split_gather, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
                        (int)Ly.Kp, P, ld, mask, part, CRT_NCH, cc, o.planes, Ly.plane, o.expo);
   } else {
+    const unsigned sg = (unsigned)(Ly.Rp / 16) * cdiv((long long)(Ly.Kp / CRT_KB), 4);
     hipLaunchKernelGGL(k_crt_norm_contig, dim3(cdiv(R, 4)), dim3(256), 0, st, R, K, P, ld, mask,
                        part);
-    hipLaunchKernelGGL(k_crt_split<false>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
-                       (int)Ly.Kp, P, ld, mask, part, 1, cc, o.planes, Ly.plane, o.expo);
+    hipLaunchKernelGGL(k_crt_split_contig, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                       (int)Ly.Kp, P, ld, mask, part, cc, o.planes, Ly.plane, o.expo);
   }
   *out = o;
   return hipGetLastError();
