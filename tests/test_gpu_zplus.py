@@ -30,7 +30,7 @@ def block_rel(a, b):
     return np.where(nb > 0, d / np.where(nb > 0, nb, 1), d)
 
 
-@pytest.mark.parametrize("n", [1, 37, 64, 65, 130, 200, 777])
+@pytest.mark.parametrize("n", [1, 37, 64, 65, 100, 128, 129, 130, 200, 777])
 def test_chol_inverse_matches_lapack(n):
     rng = np.random.default_rng(n)
     B = rng.standard_normal((n, n))

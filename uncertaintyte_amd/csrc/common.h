@@ -180,6 +180,9 @@ hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, d
                        double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
                        hipStream_t st, int depth = 0);
 // gauge-fixed Z^+ path (k_zplus.hip)
+// a recursion node of 64 < n <= 128 rows (halves 64 and n - 64) in one launch
+hipError_t chol_inv_node128(int n, double *A, int lda, double *X, int ldx, unsigned int *err,
+                            hipStream_t st);
 hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned int *err,
                          hipStream_t st);
 // P = W^-1 of a b x b SPD block (b <= 64) by Cholesky, L^-1 and X^T X (k_zplus.hip)

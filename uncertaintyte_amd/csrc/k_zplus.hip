@@ -290,6 +290,74 @@ __device__ __forceinline__ void store_xtx(int b, double (*Xs)[CL_B + 1], double 
   });
 }
 
+// A recursion node of n rows, 64 < n <= 128, in one workgroup: halves of 64
+// and n2 = n - 64 rows, every block in LDS (S, Xs: the diagonal blocks and their
+// inverse factors in turn; T: A21 -> L21; U: W = L21 X11), the four products on
+// the FP64 matrix cores:
+//   (L11, X11) = leaf(A11);  L21 = A21 X11^T;  W = L21 X11;
+//   (L22, X22) = leaf(A22 - L21 L21^T);  X21 = -X22 W.
+// The same results as the recursion's two leaves and four products (to
+// rounding), in one launch instead of seven.
+__global__ __launch_bounds__(CP_THREADS) void k_chol_inv128(int n, double *A, int lda, double *Xo,
+                                                            int ldx, unsigned int *err) {
+  __shared__ double S[CL_B][CL_B + 1];
+  __shared__ double Xs[CL_B][CL_B + 1];
+  __shared__ double T[CL_B][CL_B + 1];
+  __shared__ double U[CL_B][CL_B + 1];
+  const int n2 = n - CL_B;
+  double *A21 = A + CL_B, *A22 = A + CL_B + (size_t)CL_B * lda;
+  double *X21 = Xo + CL_B, *X22 = Xo + CL_B + (size_t)CL_B * ldx;
+  // (L11, X11)
+  load_spd64(CL_B, A, lda, S);
+  bool bad = chol_inv_lds(S, Xs);
+  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
+    const int i = x & 63, k = x >> 6;
+    A[(size_t)i + (size_t)k * lda] = S[i][k];
+    Xo[(size_t)i + (size_t)k * ldx] = Xs[i][k];
+  }
+  stage64(T, [&](int i, int k) { return i < n2 ? A21[(size_t)i + (size_t)k * lda] : 0.0; });
+  __syncthreads();
+  // L21 = A21 X11^T (X11 lower), over A21 in T
+  double4_t acc[2][2];
+  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
+                  [&](int k, int col) { return Xs[col][k]; }, acc);
+  __syncthreads();
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) { T[i][j] = v; });
+  __syncthreads();
+  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
+    const int i = x & 63, k = x >> 6;
+    if (i < n2) A21[(size_t)i + (size_t)k * lda] = T[i][k];
+  }
+  // W = L21 X11 into U; P = L21 L21^T subtracted from A22 staged in S
+  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
+                  [&](int k, int col) { return Xs[k][col]; }, acc);
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) { U[i][j] = v; });
+  double4_t pacc[2][2];
+  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
+                  [&](int k, int col) { return T[col][k]; }, pacc);
+  load_spd64(n2, A22, lda, S);  // S held L11, already stored
+  quadrant64_for_each(pacc, [&](int i, int j, double v, int) {
+    if (i < n2 && j <= i) S[i][j] -= v;
+  });
+  __syncthreads();
+  // (L22, X22); Xs held X11, no longer needed
+  bad = chol_inv_lds(S, Xs) || bad;
+  for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
+    const int i = x & 63, k = x >> 6;
+    if (i < n2 && k < n2) {
+      A22[(size_t)i + (size_t)k * lda] = S[i][k];
+      X22[(size_t)i + (size_t)k * ldx] = Xs[i][k];
+    }
+  }
+  // X21 = -X22 W (X22 lower)
+  mfma_quadrant64(CL_B, [&](int row, int k) { return Xs[row][k]; },
+                  [&](int k, int col) { return U[k][col]; }, acc);
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) {
+    if (i < n2) X21[(size_t)i + (size_t)j * ldx] = -v;
+  });
+  if (bad && threadIdx.x == 0) atomicOr(err, ERRBIT_GJ_PIVOT);
+}
+
 // P = W^-1 for the b x b SPD diagonal block of the blocked Gauss-Jordan
 // inverse (Cholesky, L^-1, X^T X); P is b x b with leading dimension b.
 __global__ __launch_bounds__(CP_THREADS) void k_spd_inv64(int b, const double *W, int ldw,
@@ -486,6 +554,13 @@ hipError_t chol_inv_leaf(int b, double *A, int lda, double *X, int ldx, unsigned
                          hipStream_t st) {
   if (b < 1 || b > CL_B) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_chol_inv64p, dim3(1), dim3(CP_THREADS), 0, st, b, A, lda, X, ldx, err);
+  return hipGetLastError();
+}
+
+hipError_t chol_inv_node128(int n, double *A, int lda, double *X, int ldx, unsigned int *err,
+                            hipStream_t st) {
+  if (n <= CL_B || n > 2 * CL_B) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_chol_inv128, dim3(1), dim3(CP_THREADS), 0, st, n, A, lda, X, ldx, err);
   return hipGetLastError();
 }
 

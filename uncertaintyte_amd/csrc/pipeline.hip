@@ -345,11 +345,16 @@ size_t zplus_crt_bytes(int n, int nmod) { return zplus_crt_layout(n, nmod, nullp
 // same block of X.  With A = [A11 .; A21 A22]:
 //   (L11, X11) = chol_inv(A11);  L21 = A21 X11^T;  A22 -= L21 L21^T;
 //   (L22, X22) = chol_inv(A22);  X21 = -X22 (L21 X11).
-// W is scratch of (n/2)^2 doubles at least.  Leaves are 64 x 64.
+// W is scratch of (n/2)^2 doubles at least.  Leaves are 64 x 64; a node of at
+// most 128 rows runs in one launch (chol_inv_node128).
 int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned int *err,
              const RecProducts &P, int depth, hipStream_t s) {
   if (n <= 64) {
     HC(chol_inv_leaf(n, A + off + (size_t)off * N, N, X + off + (size_t)off * N, N, err, s));
+    return UNC_OK;
+  }
+  if (n <= 128) {  // halves 64 and n - 64: the whole node in one workgroup
+    HC(chol_inv_node128(n, A + off + (size_t)off * N, N, X + off + (size_t)off * N, N, err, s));
     return UNC_OK;
   }
   const int n1 = rec_split(n), n2 = n - n1;
