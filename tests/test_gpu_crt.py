@@ -124,9 +124,10 @@ def test_crt_rejects_bad_moduli_count():
             d.gemm_ex(A, A, nmod=nmod)
 
 
-@pytest.mark.parametrize("depth", [3, 4, 5])
+@pytest.mark.parametrize("depth", [3, 4, 5, 13, 14])
 def test_crt_pipeline_depths_agree(depth):
-    """The product kernel's pipeline depth (LDS buffers in flight) changes no value."""
+    """The product kernel's pipeline depth (LDS buffers in flight) and workgroup tile
+    width (256 or 128 columns) change no value."""
     d = _diag()
     rng = np.random.default_rng(9)
     A = rng.standard_normal((700, 1300))
@@ -134,3 +135,8 @@ def test_crt_pipeline_depths_agree(depth):
     C0, _ = d.gemm_ex(A, B, tri=2, nmod=11)
     C, _ = d.gemm_ex(A, B, tri=2, nmod=11, depth=depth)
     assert np.array_equal(C, C0)
+    # a lower-only square product too (the tile list differs by tile width)
+    S = rng.standard_normal((700, 900))
+    L0, _ = d.gemm_ex(S, S, trans_b=True, tri=8, nmod=11)
+    L1, _ = d.gemm_ex(S, S, trans_b=True, tri=8, nmod=11, depth=depth)
+    assert np.array_equal(np.tril(L1), np.tril(L0))

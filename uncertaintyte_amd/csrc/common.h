@@ -156,7 +156,8 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 constexpr int CRT_MIN_MOD = 6, CRT_MAX_MOD = 13, CRT_MAX_K = 131071;
 int crt_bits(int nmod);  // q of nmod moduli (0 when out of range)
 size_t crt_gemm_work_bytes(int M, int N, int K, int nmod);
-// depth: LDS buffers of the product kernel's pipeline, 3..5 (0 = the library's)
+// depth: the product kernel's LDS buffers, 3..5 with 256-column workgroup tiles or
+// 13..14 for 3..4 buffers with 128-column tiles (0 = the library's)
 hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, bool trans_b, double beta, double *C, int ldc, int tri, int nmod,
                     void *work, size_t work_bytes, hipStream_t st, int depth = 0);

@@ -275,7 +275,7 @@ int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda
                      int tri, int nmod, int depth, int reps, double *avg_ms) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || lda < M || ldc < M ||
       ldb < (trans_b ? N : K) || reps < 0 || (nmod != 0 && crt_bits(nmod) == 0) ||
-      (depth != 0 && (depth < 3 || depth > 5)))
+      (depth != 0 && !(depth >= 3 && depth <= 5) && depth != 13 && depth != 14))
     return UNC_ERR_BAD_INPUT;
   DSTREAM(s);
   const size_t na = (size_t)lda * K, nb = (size_t)ldb * (trans_b ? K : N), nc = (size_t)ldc * N;

@@ -40,7 +40,10 @@ constexpr int CRT_KB = 64;      // k per block and per pipeline step
 constexpr int CRT_BLK = 1024;   // bytes of one 16 x 64 residue block
 constexpr int CRT_NCH = 16;     // K chunks of the row-norm partial sums (gathered operands)
 constexpr int CRT_THREADS = 256;
-constexpr int CRT_DEPTH = 4;    // LDS buffers of the product's pipeline (32 KB each)
+// product kernel variant: 128-column workgroup tiles (two workgroups per CU) and
+// three LDS buffers of 24 KB; 3.9 -> 3.4 ms against the 256-column tiles on the
+// 6784-row products of a 13500-row recursion
+constexpr int CRT_DEPTH = 13;
 
 // pairwise coprime moduli <= 256; their residues fit int8 in the symmetric range
 constexpr int kModuli[CRT_MAX_MOD] = {256, 255, 253, 251, 247, 241, 239,
@@ -265,53 +268,68 @@ __device__ __forceinline__ unsigned int crt_reduce(int c, int m, double minv) {
 // panels share its L2.
 typedef __attribute__((address_space(3))) void lds_void;
 
-template <int NBUF>
-__global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
+// s_waitcnt immediate for vmcnt(n), everything else unconstrained (gfx9:
+// vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | (((n >> 4) & 3) << 14); }
+
+// BN: columns of the workgroup tile, 256 (one workgroup per CU, 128 x 128 per
+// wave) or 128 (two per CU, 128 x 64 per wave: half the accumulators, so a
+// second workgroup covers one's barrier and load waits).
+template <int NBUF, int BN>
+__global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
     int Mp, int Np, int Kp, int tiles_m, int tiles_n, int ntiles, int tri, CrtConst cc,
     const signed char *__restrict__ PA, const signed char *__restrict__ PB, size_t plane_a,
     size_t plane_b, unsigned char *__restrict__ PC, size_t plane_c) {
-  __shared__ __attribute__((aligned(16))) signed char L[NBUF][32 * CRT_BLK];
+  constexpr int JN = BN / 32;          // 16-column blocks per wave
+  constexpr int NBL = 16 + BN / 16;    // residue blocks per k step (A then B)
+  constexpr int PER = NBL / 4;         // of them copied by each wave
+  constexpr int R = CRT_T / BN;        // tile columns per tile row
+  __shared__ __attribute__((aligned(16))) signed char L[NBUF][NBL * CRT_BLK];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int w = xcd_contiguous((int)blockIdx.x, cc.nmod * ntiles);
   const int mod = w / ntiles;
   int t = w - mod * ntiles, tm, tn;
-  if (tri & GEMM_C_LOWER) {  // lower tiles row by row
-    tm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
-    while (tm * (tm + 1) / 2 > t) --tm;
-    tn = t - tm * (tm + 1) / 2;
+  if (tri & GEMM_C_LOWER) {  // lower tiles row by row: row tm holds R (tm + 1) of them
+    auto before = [](int r) { return R * r * (r + 1) / 2; };
+    tm = (int)((sqrt(8.0 * t / R + 1.0) - 1.0) * 0.5);
+    while (before(tm + 1) <= t) ++tm;
+    while (before(tm) > t) --tm;
+    tn = t - before(tm);
   } else {  // groups of four tile rows, column by column inside a group
     const int per = 4 * tiles_n, g0 = (t / per) * 4, gs = min(4, tiles_m - g0);
     tm = g0 + (t % per) % gs;
     tn = (t % per) / gs;
   }
-  const int m0 = tm * CRT_T, n0 = tn * CRT_T;
+  const int m0 = tm * CRT_T, n0 = tn * BN;
   int kbeg = 0, kend = Kp;
   if (tri & GEMM_A_LOWER) kend = min(kend, m0 + CRT_T);
-  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + CRT_T);
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + BN);
   if (tri & GEMM_B_LOWER) kbeg = n0;
   const int kblocks = Kp / CRT_KB;
   const int s0 = kbeg / CRT_KB, nsteps = max(0, (kend - kbeg + CRT_KB - 1) / CRT_KB);
-  // this wave's 8 blocks of a step: A blocks 8 w .. 8 w + 7 (w < 2), else B blocks
-  const signed char *src = (wave < 2 ? PA + (size_t)mod * plane_a +
-                                           ((size_t)(m0 / 16) + 8 * wave) * kblocks * CRT_BLK
-                                     : PB + (size_t)mod * plane_b +
-                                           ((size_t)(n0 / 16) + 8 * (wave - 2)) * kblocks * CRT_BLK) +
-                           16 * lane;
-  auto issue = [&](int q) {
-    const signed char *g = src + (size_t)(s0 + q) * CRT_BLK;
-    signed char *dst = L[q % NBUF] + 8 * wave * CRT_BLK;
+  // this wave's blocks of a step: PER of the NBL (A blocks 0..15, then B)
+  const signed char *pa = PA + (size_t)mod * plane_a + (size_t)(m0 / 16) * kblocks * CRT_BLK;
+  const signed char *pb = PB + (size_t)mod * plane_b + (size_t)(n0 / 16) * kblocks * CRT_BLK;
+  const signed char *src[PER];
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
-      __builtin_amdgcn_global_load_lds((const void *)(g + (size_t)b * kblocks * CRT_BLK),
+  for (int b = 0; b < PER; ++b) {
+    const int blk = wave * PER + b;
+    src[b] = (blk < 16 ? pa + (size_t)blk * kblocks * CRT_BLK
+                       : pb + (size_t)(blk - 16) * kblocks * CRT_BLK) + 16 * lane;
+  }
+  auto issue = [&](int q) {
+    signed char *dst = L[q % NBUF] + PER * wave * CRT_BLK;
+#pragma unroll
+    for (int b = 0; b < PER; ++b)
+      __builtin_amdgcn_global_load_lds((const void *)(src[b] + (size_t)(s0 + q) * CRT_BLK),
                                        (lds_void *)(dst + b * CRT_BLK), 16, 0, 0);
   };
-  v4i acc[8][8];
+  v4i acc[8][JN];
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
+    for (int j = 0; j < JN; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
   // Steps run through NBUF LDS buffers: while step q is multiplied, steps up to
   // q + NBUF - 1 are being copied in; step r's copy is issued at the barrier
   // before step r - NBUF + 1 is read, into the buffer of step r - NBUF, whose
@@ -319,14 +337,13 @@ __global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
   auto sync_step = [&](int r) {
     const int later = min(nsteps - 1 - r, NBUF - 2);
     // this wave's fragment reads are done and its copies of step r have landed
-    // (vector memory operations complete in order; the 8 of each later step
-    // issued may be in flight); the barrier then makes every wave's visible.
-    // (gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-    __builtin_amdgcn_s_waitcnt(0xC07F);                                    // lgkmcnt(0)
-    if (NBUF > 4 && later >= 3) __builtin_amdgcn_s_waitcnt(0x4F78);       // vmcnt(24)
-    else if (NBUF > 3 && later >= 2) __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16)
-    else if (later >= 1) __builtin_amdgcn_s_waitcnt(0x0F78);              // vmcnt(8)
-    else __builtin_amdgcn_s_waitcnt(0x0F70);                              // vmcnt(0)
+    // (vector memory operations complete in order; the PER of each later step
+    // issued may be in flight); the barrier then makes every wave's visible
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    if (NBUF > 4 && later >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_imm(3 * PER));
+    else if (NBUF > 3 && later >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * PER));
+    else if (later >= 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
     __builtin_amdgcn_s_barrier();
     if (r + NBUF - 1 < nsteps) issue(r + NBUF - 1);
   };
@@ -341,12 +358,12 @@ __global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
     v4i af[8], bq[2];
 #pragma unroll
     for (int i = 0; i < 8; ++i) af[i] = *(const v4i *)(Lb + (wm * 8 + i) * CRT_BLK + 16 * lane);
-    bq[0] = *(const v4i *)(Lb + (16 + wn * 8) * CRT_BLK + 16 * lane);
+    bq[0] = *(const v4i *)(Lb + (16 + wn * JN) * CRT_BLK + 16 * lane);
     __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);  // 9 LDS reads
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if (j + 1 < 8) {
-        bq[(j + 1) & 1] = *(const v4i *)(Lb + (16 + wn * 8 + j + 1) * CRT_BLK + 16 * lane);
+    for (int j = 0; j < JN; ++j) {
+      if (j + 1 < JN) {
+        bq[(j + 1) & 1] = *(const v4i *)(Lb + (16 + wn * JN + j + 1) * CRT_BLK + 16 * lane);
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
       }
 #pragma unroll
@@ -364,9 +381,9 @@ __global__ __launch_bounds__(CRT_THREADS, 1) void k_crt_gemm(
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < JN; ++j) {
       const size_t rb = (size_t)(m0 + wm * 128 + i * 16) / 16;
-      const size_t cb = (size_t)(n0 + wn * 128 + j * 16) / 16;
+      const size_t cb = (size_t)(n0 + wn * (BN / 2) + j * 16) / 16;
       const unsigned int u = crt_reduce(acc[i][j][0], m, mi) |
                              crt_reduce(acc[i][j][1], m, mi) << 8 |
                              crt_reduce(acc[i][j][2], m, mi) << 16 |
@@ -558,16 +575,21 @@ hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, d
   unsigned char *pc = (unsigned char *)cplanes;
   const size_t plane_a = (size_t)A.Rp * A.Kp, plane_b = (size_t)B.Rp * B.Kp;
   const size_t plane_c = (size_t)A.Rp * B.Rp;
-  const int tm = A.Rp / CRT_T, tn = B.Rp / CRT_T;
-  const int ntiles = (tri & GEMM_C_LOWER) ? tm * (tm + 1) / 2 : tm * tn;
   if (depth == 0) depth = CRT_DEPTH;
-#define UNC_CRT_GEMM(NB)                                                                      \
-  hipLaunchKernelGGL(k_crt_gemm<NB>, dim3(nmod * ntiles), dim3(CRT_THREADS), 0, st, A.Rp,     \
+  // variant: pipeline depth 3..5 with 256-column tiles, or 13..14: depth 3..4 with
+  // 128-column tiles (two workgroups per CU)
+  const int bn = depth >= 10 ? 128 : 256, nbuf = depth % 10;
+  const int tm = A.Rp / CRT_T, tn = B.Rp / bn;
+  const int ntiles = (tri & GEMM_C_LOWER) ? (CRT_T / bn) * tm * (tm + 1) / 2 : tm * tn;
+#define UNC_CRT_GEMM(NB, BN)                                                                  \
+  hipLaunchKernelGGL((k_crt_gemm<NB, BN>), dim3(nmod * ntiles), dim3(CRT_THREADS), 0, st, A.Rp, \
                      B.Rp, A.Kp, tm, tn, ntiles, tri, cc, A.planes, B.planes, plane_a, plane_b, \
                      pc, plane_c)
-  if (depth == 3) UNC_CRT_GEMM(3);
-  else if (depth == 4) UNC_CRT_GEMM(4);
-  else if (depth == 5) UNC_CRT_GEMM(5);
+  if (bn == 256 && nbuf == 3) UNC_CRT_GEMM(3, 256);
+  else if (bn == 256 && nbuf == 4) UNC_CRT_GEMM(4, 256);
+  else if (bn == 256 && nbuf == 5) UNC_CRT_GEMM(5, 256);
+  else if (bn == 128 && nbuf == 3) UNC_CRT_GEMM(3, 128);
+  else if (bn == 128 && nbuf == 4) UNC_CRT_GEMM(4, 128);
   else return hipErrorInvalidValue;
 #undef UNC_CRT_GEMM
   hipLaunchKernelGGL(k_crt_rebuild, dim3(A.Rp / 64, B.Rp / 16), dim3(256), 0, st, M, N, A.Rp,
