@@ -105,6 +105,18 @@ def test_crt_operand_flags(M, N, K, trans_b, tri, beta):
         assert np.all(np.abs(C - ref) <= bnd), float(np.max(np.abs(C - ref) / bnd))
 
 
+def test_mfma_i8_32x32x32_lane_layout():
+    """The operand / accumulator lane layout of v_mfma_i32_32x32x32_i8 that the CRT
+    product relies on: row l % 32 of A and column l % 32 of B, k 16 (l / 32) ..,
+    accumulator r holding C's row 8 (r / 4) + 4 (l / 32) + r % 4."""
+    d = _diag()
+    rng = np.random.default_rng(32)
+    A = rng.integers(-128, 128, (32, 32)).astype(np.int8)
+    B = rng.integers(-128, 128, (32, 32)).astype(np.int8)
+    C = d.mfma_i8_32(A, B)
+    assert np.array_equal(C, A.astype(np.int64) @ B.astype(np.int64))
+
+
 def test_crt_run_to_run_identical():
     d = _diag()
     rng = np.random.default_rng(5)

@@ -46,6 +46,10 @@ int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B,
 int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,
                      const double *B, int ldb, int trans_b, double beta, double *C, int ldc,
                      int tri, int nmod, int depth, int reps, double *avg_ms);
+/* C = A B for 32 x 32 int8 A (row-major) and B (row-major in k) by one
+ * v_mfma_i32_32x32x32_i8 with the operand and accumulator lane layout the CRT
+ * product assumes (pins that layout) */
+int unc_diag_mfma_i8_32(const signed char *A, const signed char *B, int *C);
 /* X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels, plus the Newton step
  * with the error-free residual when refine != 0 */
 int unc_diag_inverse(int n, const double *A, double *X, int refine);

@@ -36,6 +36,7 @@ def dlib():
     L.unc_diag_residual.argtypes = [i, vp, vp, vp, i]
     L.unc_diag_residual_oz.argtypes = [i, vp, vp, vp, i, dp]
     L.unc_diag_chol_inverse.argtypes = [i, vp, vp, vp]
+    L.unc_diag_mfma_i8_32.argtypes = [vp, vp, vp]
     L.unc_diag_gemm_ex.argtypes = [i, i, i, ctypes.c_double, vp, i, vp, i, i, ctypes.c_double,
                                    vp, i, i, i, i, i, dp]
     L.unc_diag_gemm_bench.argtypes = [i, i, i, i, dp]
@@ -108,6 +109,17 @@ def gemm_sym_batch(A: np.ndarray, Bs, reps: int = 0):
 
 
 GEMM_A_LOWER, GEMM_B_LOWER, GEMM_BT_LOWER, GEMM_C_LOWER = 1, 2, 4, 8
+
+
+def mfma_i8_32(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """A @ B (32 x 32 int8 each) by one v_mfma_i32_32x32x32_i8 with the lane
+    layout the CRT product uses."""
+    Ai = np.ascontiguousarray(A, np.int8)
+    Bi = np.ascontiguousarray(B, np.int8)
+    C = np.zeros((32, 32), np.int32)
+    _check(dlib().unc_diag_mfma_i8_32(Ai.ctypes.data, Bi.ctypes.data, C.ctypes.data),
+           "unc_diag_mfma_i8_32")
+    return C
 
 
 def gemm_ex(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0,
