@@ -156,11 +156,9 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 constexpr int CRT_MIN_MOD = 6, CRT_MAX_MOD = 13, CRT_MAX_K = 131071;
 int crt_bits(int nmod);  // q of nmod moduli (0 when out of range)
 size_t crt_gemm_work_bytes(int M, int N, int K, int nmod);
-// depth: the product kernel's LDS buffers, 3..5 with 256-column workgroup tiles or
-// 13..14 for 3..4 buffers with 128-column tiles (0 = the library's)
 hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, bool trans_b, double beta, double *C, int ldc, int tri, int nmod,
-                    void *work, size_t work_bytes, hipStream_t st, int depth = 0);
+                    void *work, size_t work_bytes, hipStream_t st);
 // The same in two parts, so that one operand's residues can serve several
 // products: crt_split cuts the R rows of an operand (element (r, k) at
 // P[r + k ld] when gather, else at P[k + r ld]; mask 1 keeps k <= r, 2 keeps
@@ -179,7 +177,7 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
                      void *area, CrtOperand *out, hipStream_t st);
 hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
                        double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
-                       hipStream_t st, int depth = 0);
+                       hipStream_t st);
 // gauge-fixed Z^+ path (k_zplus.hip)
 // a recursion node of 64 < n <= 128 rows (halves 64 and n - 64) in one launch
 hipError_t chol_inv_node128(int n, double *A, int lda, double *X, int ldx, unsigned int *err,

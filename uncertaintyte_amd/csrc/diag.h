@@ -39,17 +39,11 @@ int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B,
 /* C = alpha A op(B) + beta C (op(B) = B^T when trans_b, B then N x K) with the
  * operand flags of the library's general product (1 A lower, 2 B lower, 4 B^T
  * lower, 8 lower C only): nmod = 0 by the FP64 MFMA product, 6..13 by the int8
- * CRT product with that many moduli, its pipeline `depth` LDS buffers deep (3..5
- * with 256-column workgroup tiles, 13..14 for 3..4 buffers with 128-column tiles,
- * 0 = the library's).  avg_ms (may be NULL): average of `reps` further calls on
+ * CRT product with that many moduli.  avg_ms (may be NULL): average of `reps` further calls on
  * the device-resident operands. */
 int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,
                      const double *B, int ldb, int trans_b, double beta, double *C, int ldc,
-                     int tri, int nmod, int depth, int reps, double *avg_ms);
-/* C = A B for 32 x 32 int8 A (row-major) and B (row-major in k) by one
- * v_mfma_i32_32x32x32_i8 with the operand and accumulator lane layout the CRT
- * product assumes (pins that layout) */
-int unc_diag_mfma_i8_32(const signed char *A, const signed char *B, int *C);
+                     int tri, int nmod, int reps, double *avg_ms);
 /* X = A^-1 (SPD A) by the blocked Gauss-Jordan kernels, plus the Newton step
  * with the error-free residual when refine != 0 */
 int unc_diag_inverse(int n, const double *A, double *X, int refine);

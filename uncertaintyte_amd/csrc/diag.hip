@@ -83,52 +83,9 @@ __global__ void k_fill_pattern(size_t n, double *A, double scale) {
     A[x] = scale * (double)((x * 2654435761ull) % 1000003ull) / 1000003.0 - 0.5 * scale;
 }
 
-// One v_mfma_i32_32x32x32_i8 on 32 x 32 int8 operands (A row-major, B row-major
-// in k): lane l holds A's row l % 32 and B's column l % 32 at k 16 (l / 32) ..
-// + 15; accumulator register r of lane l is C's row 8 (r / 4) + 4 (l / 32) +
-// r % 4, column l % 32.
-__global__ __launch_bounds__(64) void k_mfma_i8_32_probe(const signed char *A,
-                                                         const signed char *B, int *C) {
-  typedef int v4i_t __attribute__((ext_vector_type(4)));
-  typedef int v16i_t __attribute__((ext_vector_type(16)));
-  const int l = threadIdx.x;
-  unsigned int wa[4], wb[4];
-  for (int g = 0; g < 4; ++g) {
-    wa[g] = wb[g] = 0u;
-    for (int t = 0; t < 4; ++t) {
-      const int k = 16 * (l / 32) + 4 * g + t;
-      wa[g] |= (unsigned int)(unsigned char)A[(l % 32) * 32 + k] << (8 * t);
-      wb[g] |= (unsigned int)(unsigned char)B[k * 32 + (l % 32)] << (8 * t);
-    }
-  }
-  const v4i_t a = {(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
-  const v4i_t b = {(int)wb[0], (int)wb[1], (int)wb[2], (int)wb[3]};
-  v16i_t acc;
-  for (int r = 0; r < 16; ++r) acc[r] = 0;
-  acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
-  for (int r = 0; r < 16; ++r) C[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = acc[r];
-}
-
 }  // namespace
 
 extern "C" {
-
-int unc_diag_mfma_i8_32(const signed char *A, const signed char *B, int *C) {
-  if (!A || !B || !C) return UNC_ERR_BAD_INPUT;
-  DSTREAM(s);
-  DevBuf buf;
-  if (buf.alloc(2048 + 4096)) return UNC_ERR_OOM;
-  signed char *dA = (signed char *)buf.p, *dB = dA + 1024;
-  int *dC = (int *)(dB + 1024);
-  DC(hipMemcpyAsync(dA, A, 1024, hipMemcpyHostToDevice, s));
-  DC(hipMemcpyAsync(dB, B, 1024, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_mfma_i8_32_probe, dim3(1), dim3(64), 0, s, dA, dB, dC);
-  DC(hipGetLastError());
-  DC(hipMemcpyAsync(C, dC, 4096, hipMemcpyDeviceToHost, s));
-  DC(hipStreamSynchronize(s));
-  return UNC_OK;
-}
-
 
 int unc_diag_gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                       int ldb, double beta, double *C, int ldc, double *absmax_out) {
@@ -315,10 +272,9 @@ int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int
 
 int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,
                      const double *B, int ldb, int trans_b, double beta, double *C, int ldc,
-                     int tri, int nmod, int depth, int reps, double *avg_ms) {
+                     int tri, int nmod, int reps, double *avg_ms) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || lda < M || ldc < M ||
-      ldb < (trans_b ? N : K) || reps < 0 || (nmod != 0 && crt_bits(nmod) == 0) ||
-      (depth != 0 && !(depth >= 3 && depth <= 5) && depth != 13 && depth != 14))
+      ldb < (trans_b ? N : K) || reps < 0 || (nmod != 0 && crt_bits(nmod) == 0))
     return UNC_ERR_BAD_INPUT;
   DSTREAM(s);
   const size_t na = (size_t)lda * K, nb = (size_t)ldb * (trans_b ? K : N), nc = (size_t)ldc * N;
@@ -333,7 +289,7 @@ int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda
   DC(hipMemcpyAsync(dC0, dC, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
   auto call = [&]() -> hipError_t {
     return nmod ? gemm_crt(M, N, K, alpha, dA, lda, dB, ldb, trans_b != 0, beta, dC, ldc, tri,
-                           nmod, work, wb, s, depth)
+                           nmod, work, wb, s)
                 : gemm_f64_ex(M, N, K, alpha, dA, lda, dB, ldb, trans_b != 0, beta, dC, ldc, tri,
                               s);
   };

@@ -35,27 +35,25 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int CRT_T = 256;      // output tile (rows and columns), 4 waves of 128 x 128
+constexpr int CRT_T = 256;      // rows of an output tile; operand rows are padded to it
 constexpr int CRT_KB = 64;      // k per block and per pipeline step
 constexpr int CRT_BLK = 1024;   // bytes of one 16 x 64 residue block
 constexpr int CRT_NCH = 16;     // K chunks of the row-norm partial sums (gathered operands)
 constexpr int CRT_THREADS = 256;
-// product kernel variant: 128-column workgroup tiles (two workgroups per CU) and
-// three LDS buffers of 24 KB; 3.9 -> 3.4 ms against the 256-column tiles on the
-// 6784-row products of a 13500-row recursion
-constexpr int CRT_DEPTH = 13;
 
 // pairwise coprime moduli <= 256; their residues fit int8 in the symmetric range
 constexpr int kModuli[CRT_MAX_MOD] = {256, 255, 253, 251, 247, 241, 239,
                                       233, 229, 227, 223, 217, 211};
 
+constexpr int CRT_LIMBS = 7;  // 16-bit limbs of E_l < P < 2^112 (13 moduli: 2^103.3)
+
 struct CrtConst {
-  int nmod, q;
+  int nmod, q, nlimb;
   int m[CRT_MAX_MOD];
   double minv[CRT_MAX_MOD];
-  unsigned long long elo[CRT_MAX_MOD], ehi[CRT_MAX_MOD];  // E_l
-  unsigned long long plo, phi;                            // P
-  double pinv;                                            // 1 / P (rounded)
+  unsigned int e16[CRT_MAX_MOD][CRT_LIMBS];  // E_l in 16-bit limbs, least significant first
+  unsigned long long plo, phi;                // P
+  double pinv;                                // 1 / P (rounded)
 };
 
 typedef unsigned __int128 u128;
@@ -78,9 +76,10 @@ CrtConst make_const(int nmod) {
     const u128 E = (Pl * y) % P;
     c.m[l] = (int)m;
     c.minv[l] = 1.0 / (double)m;
-    c.elo[l] = (unsigned long long)E;
-    c.ehi[l] = (unsigned long long)(E >> 64);
+    for (int k = 0; k < CRT_LIMBS; ++k) c.e16[l][k] = (unsigned int)(E >> (16 * k)) & 0xffffu;
   }
+  c.nlimb = 1;
+  while (c.nlimb < CRT_LIMBS && (P >> (16 * c.nlimb)) != 0) ++c.nlimb;
   c.plo = (unsigned long long)P;
   c.phi = (unsigned long long)(P >> 64);
   const double pd = (double)c.phi * 18446744073709551616.0 + (double)c.plo;
@@ -255,36 +254,41 @@ __device__ __forceinline__ unsigned int crt_reduce(int c, int m, double minv) {
   return (unsigned int)r;
 }
 
-// One 256 x 256 tile of one modulus' product per workgroup: 4 waves (one per
-// SIMD), wave (wm, wn) = (w & 1, w >> 1) owning rows 128 wm .. + 127 and columns
-// 128 wn .. + 127 as 8 x 8 blocks of 16 x 16 (the int32 sums in the accumulation
-// registers).  A k step (64) is 32 residue blocks of 1 KB (16 of A, 16 of B),
-// copied straight from global memory into one of NBUF LDS buffers (each wave
-// moves 8 blocks, one global_load_lds_dwordx4 each).  Each step reads its 16
-// fragments per wave from LDS once, while the 64 MFMAs of the step before run.  The residues of the int32 sums go to the C plane of
-// the modulus (column-major, leading dimension Mp), four rows per 32-bit store.
+// One 256 x 128 tile of one modulus' product per workgroup, two workgroups per
+// CU (one covers the other's barrier and load waits): 4 waves (one per SIMD),
+// wave (wm, wn) = (w & 1, w >> 1) owning rows 128 wm .. + 127 and columns
+// 64 wn .. + 63 as 8 x 4 blocks of 16 x 16 (the int32 sums in the accumulation
+// registers).  A k step (64) is 24 residue blocks of 1 KB (16 of A, 8 of B),
+// copied straight from global memory into one of CRT_NBUF LDS buffers (each
+// wave moves 6 blocks, one global_load_lds_dwordx4 each).  Each step reads its
+// 12 fragments per wave from LDS once, while the 32 MFMAs run.  The residues of
+// the int32 sums go to the C plane of the modulus in 16 x 16 blocks.
 // Grid: nmod x the tiles (the lower ones only when GEMM_C_LOWER), modulus-major,
 // each XCD taking a contiguous range so that the workgroups sharing a modulus'
 // panels share its L2.
+// Measured on the 6784-row products of a 13500-row recursion (12 moduli): 256 x
+// 256 tiles, one workgroup per CU, 3.9 ms against 3.5 for these; the same tiles
+// on v_mfma_i32_32x32x32_i8 (32 x 32 blocks, half the instructions) 3.7 ms: the
+// kernel waits on its operand loads (L2 hit rate ~65 %), not on MFMA issue.
 typedef __attribute__((address_space(3))) void lds_void;
 
 // s_waitcnt immediate for vmcnt(n), everything else unconstrained (gfx9:
 // vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
 constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | (((n >> 4) & 3) << 14); }
 
-// BN: columns of the workgroup tile, 256 (one workgroup per CU, 128 x 128 per
-// wave) or 128 (two per CU, 128 x 64 per wave: half the accumulators, so a
-// second workgroup covers one's barrier and load waits).
-template <int NBUF, int BN>
-__global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
+constexpr int CRT_BN = 128;                // columns of the workgroup tile
+constexpr int CRT_NBUF = 3;                // LDS buffers (k steps in flight + 1)
+constexpr int CRT_JN = CRT_BN / 32;        // 16-column blocks per wave
+constexpr int CRT_NBL = 16 + CRT_BN / 16;  // residue blocks per k step (A then B)
+constexpr int CRT_PER = CRT_NBL / 4;       // of them copied by each wave
+constexpr int CRT_TPR = CRT_T / CRT_BN;    // tile columns per 256 columns
+
+__global__ __launch_bounds__(CRT_THREADS, 2) void k_crt_gemm(
     int Mp, int Np, int Kp, int tiles_m, int tiles_n, int ntiles, int tri, CrtConst cc,
     const signed char *__restrict__ PA, const signed char *__restrict__ PB, size_t plane_a,
     size_t plane_b, unsigned char *__restrict__ PC, size_t plane_c) {
-  constexpr int JN = BN / 32;          // 16-column blocks per wave
-  constexpr int NBL = 16 + BN / 16;    // residue blocks per k step (A then B)
-  constexpr int PER = NBL / 4;         // of them copied by each wave
-  constexpr int R = CRT_T / BN;        // tile columns per tile row
-  __shared__ __attribute__((aligned(16))) signed char L[NBUF][NBL * CRT_BLK];
+  constexpr int JN = CRT_JN, PER = CRT_PER, R = CRT_TPR;
+  __shared__ __attribute__((aligned(16))) signed char L[CRT_NBUF][CRT_NBL * CRT_BLK];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int w = xcd_contiguous((int)blockIdx.x, cc.nmod * ntiles);
@@ -301,10 +305,10 @@ __global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
     tm = g0 + (t % per) % gs;
     tn = (t % per) / gs;
   }
-  const int m0 = tm * CRT_T, n0 = tn * BN;
+  const int m0 = tm * CRT_T, n0 = tn * CRT_BN;
   int kbeg = 0, kend = Kp;
   if (tri & GEMM_A_LOWER) kend = min(kend, m0 + CRT_T);
-  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + BN);
+  if (tri & GEMM_BT_LOWER) kend = min(kend, n0 + CRT_BN);
   if (tri & GEMM_B_LOWER) kbeg = n0;
   const int kblocks = Kp / CRT_KB;
   const int s0 = kbeg / CRT_KB, nsteps = max(0, (kend - kbeg + CRT_KB - 1) / CRT_KB);
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
                        : pb + (size_t)(blk - 16) * kblocks * CRT_BLK) + 16 * lane;
   }
   auto issue = [&](int q) {
-    signed char *dst = L[q % NBUF] + PER * wave * CRT_BLK;
+    signed char *dst = L[q % CRT_NBUF] + PER * wave * CRT_BLK;
 #pragma unroll
     for (int b = 0; b < PER; ++b)
       __builtin_amdgcn_global_load_lds((const void *)(src[b] + (size_t)(s0 + q) * CRT_BLK),
@@ -330,29 +334,26 @@ __global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < JN; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
-  // Steps run through NBUF LDS buffers: while step q is multiplied, steps up to
-  // q + NBUF - 1 are being copied in; step r's copy is issued at the barrier
-  // before step r - NBUF + 1 is read, into the buffer of step r - NBUF, whose
-  // fragment reads every wave finished before that barrier.
+  // Steps run through the three LDS buffers: while step q is multiplied, step
+  // q + 1 is being copied in; step r + 2's copy is issued at the barrier before
+  // step r is read, into the buffer of step r - 1, whose fragment reads every
+  // wave finished before that barrier.
   auto sync_step = [&](int r) {
-    const int later = min(nsteps - 1 - r, NBUF - 2);
     // this wave's fragment reads are done and its copies of step r have landed
-    // (vector memory operations complete in order; the PER of each later step
-    // issued may be in flight); the barrier then makes every wave's visible
+    // (vector memory operations complete in order; the PER of step r + 1 may
+    // be in flight); the barrier then makes every wave's visible
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-    if (NBUF > 4 && later >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_imm(3 * PER));
-    else if (NBUF > 3 && later >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * PER));
-    else if (later >= 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));
+    if (r + 1 < nsteps) __builtin_amdgcn_s_waitcnt(vmcnt_imm(PER));
     else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
     __builtin_amdgcn_s_barrier();
-    if (r + NBUF - 1 < nsteps) issue(r + NBUF - 1);
+    if (r + CRT_NBUF - 1 < nsteps) issue(r + CRT_NBUF - 1);
   };
 #pragma unroll
-  for (int q = 0; q < NBUF - 1; ++q)
+  for (int q = 0; q < CRT_NBUF - 1; ++q)
     if (q < nsteps) issue(q);
   for (int q = 0; q < nsteps; ++q) {
     sync_step(q);
-    const signed char *Lb = L[q % NBUF];
+    const signed char *Lb = L[q % CRT_NBUF];
     // the A fragments first, then each B fragment read while the eight MFMAs
     // of the one before it run
     v4i af[8], bq[2];
@@ -376,14 +377,15 @@ __global__ __launch_bounds__(CRT_THREADS, 256 / BN) void k_crt_gemm(
   const double mi = cc.minv[mod];
   unsigned char *pc = PC + (size_t)mod * plane_c;
   // C plane in 16 x 16 blocks (column-block major), each the 256 bytes of one
-  // MFMA output block in lane order: one contiguous store per block and wave
+  // MFMA output block in lane order (the word at 4 p holds rows 4 (p / 16) ..
+  // + 3 of column p % 16): one contiguous store per block and wave
   const size_t rblocks = (size_t)Mp / 16;
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
       const size_t rb = (size_t)(m0 + wm * 128 + i * 16) / 16;
-      const size_t cb = (size_t)(n0 + wn * (BN / 2) + j * 16) / 16;
+      const size_t cb = (size_t)(n0 + wn * (CRT_BN / 2) + j * 16) / 16;
       const unsigned int u = crt_reduce(acc[i][j][0], m, mi) |
                              crt_reduce(acc[i][j][1], m, mi) << 8 |
                              crt_reduce(acc[i][j][2], m, mi) << 16 |
@@ -405,13 +407,25 @@ __device__ __forceinline__ void u128_sub(unsigned long long &alo, unsigned long 
   ahi -= bhi + borrow;
 }
 
+// v |= d << sh on a 128-bit value held as two 64-bit limbs (sh a constant)
+__device__ __forceinline__ void u128_put(unsigned long long &lo, unsigned long long &hi,
+                                         unsigned long long d, int sh) {
+  if (sh >= 64) {
+    hi |= d << (sh - 64);
+  } else {
+    lo |= d << sh;
+    if (sh > 0) hi |= d >> (64 - sh);
+  }
+}
+
 // C_ij = alpha 2^-(ea_i + eb_j) crt(r_1 .. r_s) + beta C_ij.  One thread per
 // four rows of one column, read from the blocked C planes as one 32-bit word
 // per plane (the lane's word of a 16 x 16 block); 256 threads take four row
-// blocks of one column block.  The CRT sum sum_l r_l E_l is formed on 32-bit
-// limbs of E_l (s_k = sum_l r_l e_lk < 13 * 2^40, one 32 x 32 -> 64-bit
-// multiply-add each), then carried into 128 bits.  With `lower` only i >= j is
-// written.
+// blocks of one column block.  The CRT sum sum_l r_l E_l is formed on NL 16-bit
+// limbs of E_l: s_k = sum_l r_l e_lk < 13 * 255 * 65535 < 2^28 with one 24-bit
+// integer multiply-add per term (full rate), then carried into 128 bits.  With
+// `lower` only i >= j is written.
+template <int NL>
 __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int lower, double alpha,
                                                      double beta, CrtConst cc,
                                                      const unsigned char *__restrict__ PC,
@@ -424,18 +438,14 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int l
   if (i0 >= M || j >= N) return;
   if (lower && i0 + 3 < j) return;
   const size_t at = ((size_t)cb * (Mp / 16) + rb) * 256 + 4 * lane;
-  unsigned long long sk[4][4] = {};  // [row][limb]
+  unsigned int sk[4][NL] = {};  // [row][limb]
   for (int l = 0; l < cc.nmod; ++l) {
     const unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + at);
-    const unsigned int e0 = (unsigned int)cc.elo[l], e1 = (unsigned int)(cc.elo[l] >> 32);
-    const unsigned int e2 = (unsigned int)cc.ehi[l], e3 = (unsigned int)(cc.ehi[l] >> 32);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const unsigned long long r = (u >> (8 * b)) & 255u;
-      sk[b][0] += r * e0;
-      sk[b][1] += r * e1;
-      sk[b][2] += r * e2;
-      sk[b][3] += r * e3;
+      const unsigned int r = __builtin_amdgcn_ubfe(u, 8 * b, 8);
+#pragma unroll
+      for (int k = 0; k < NL; ++k) sk[b][k] += __umul24(r, cc.e16[l][k]);
     }
   }
   const int ej = eb[j];
@@ -445,14 +455,15 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int l
     const int i = i0 + b;
     if (i >= M || (lower && i < j)) continue;
     // carry the limb sums into x = lo + 2^64 hi
-    unsigned long long c = sk[b][0];
-    unsigned long long lo = c & 0xffffffffull;
-    c = (c >> 32) + sk[b][1];
-    lo |= c << 32;
-    c = (c >> 32) + sk[b][2];
-    unsigned long long hi = c & 0xffffffffull;
-    c = (c >> 32) + sk[b][3];
-    hi |= c << 32;
+    unsigned long long lo = 0, hi = 0;
+    unsigned int c = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      c += sk[b][k];
+      u128_put(lo, hi, c & 0xffffu, 16 * k);
+      c >>= 16;
+    }
+    u128_put(lo, hi, c, 16 * NL);
     // x mod P: the quotient (< 2^12) from FP64, one below to stay non-negative
     const double xd = (double)hi * 18446744073709551616.0 + (double)lo;
     const double qd = floor(xd * cc.pinv) - 1.0;
@@ -564,7 +575,7 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
 
 hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
                        double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
-                       hipStream_t st, int depth) {
+                       hipStream_t st) {
   const int M = A.R, N = B.R, nmod = A.nmod;
   if (M <= 0 || N <= 0) return hipSuccess;
   if (A.K != B.K || A.Kp != B.Kp || A.nmod != B.nmod || nmod < CRT_MIN_MOD ||
@@ -575,26 +586,23 @@ hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, d
   unsigned char *pc = (unsigned char *)cplanes;
   const size_t plane_a = (size_t)A.Rp * A.Kp, plane_b = (size_t)B.Rp * B.Kp;
   const size_t plane_c = (size_t)A.Rp * B.Rp;
-  if (depth == 0) depth = CRT_DEPTH;
-  // variant: pipeline depth 3..5 with 256-column tiles, or 13..14: depth 3..4 with
-  // 128-column tiles (two workgroups per CU)
-  const int bn = depth >= 10 ? 128 : 256, nbuf = depth % 10;
-  const int tm = A.Rp / CRT_T, tn = B.Rp / bn;
-  const int ntiles = (tri & GEMM_C_LOWER) ? (CRT_T / bn) * tm * (tm + 1) / 2 : tm * tn;
-#define UNC_CRT_GEMM(NB, BN)                                                                  \
-  hipLaunchKernelGGL((k_crt_gemm<NB, BN>), dim3(nmod * ntiles), dim3(CRT_THREADS), 0, st, A.Rp, \
-                     B.Rp, A.Kp, tm, tn, ntiles, tri, cc, A.planes, B.planes, plane_a, plane_b, \
-                     pc, plane_c)
-  if (bn == 256 && nbuf == 3) UNC_CRT_GEMM(3, 256);
-  else if (bn == 256 && nbuf == 4) UNC_CRT_GEMM(4, 256);
-  else if (bn == 256 && nbuf == 5) UNC_CRT_GEMM(5, 256);
-  else if (bn == 128 && nbuf == 3) UNC_CRT_GEMM(3, 128);
-  else if (bn == 128 && nbuf == 4) UNC_CRT_GEMM(4, 128);
-  else return hipErrorInvalidValue;
-#undef UNC_CRT_GEMM
-  hipLaunchKernelGGL(k_crt_rebuild, dim3(A.Rp / 64, B.Rp / 16), dim3(256), 0, st, M, N, A.Rp,
-                     (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, cc, pc, plane_c, A.expo, B.expo,
-                     C, ldc);
+  const int tm = A.Rp / CRT_T, tn = B.Rp / CRT_BN;
+  const int ntiles = (tri & GEMM_C_LOWER) ? CRT_TPR * tm * (tm + 1) / 2 : tm * tn;
+  hipLaunchKernelGGL(k_crt_gemm, dim3(nmod * ntiles), dim3(CRT_THREADS), 0, st, A.Rp, B.Rp, A.Kp,
+                     tm, tn, ntiles, tri, cc, A.planes, B.planes, plane_a, plane_b, pc, plane_c);
+#define UNC_CRT_REBUILD(NL)                                                                \
+  hipLaunchKernelGGL(k_crt_rebuild<NL>, dim3(A.Rp / 64, B.Rp / 16), dim3(256), 0, st, M, N, A.Rp, \
+                     (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, cc, pc, plane_c, A.expo, B.expo, \
+                     C, ldc)
+  switch (cc.nlimb) {
+    case 3: UNC_CRT_REBUILD(3); break;
+    case 4: UNC_CRT_REBUILD(4); break;
+    case 5: UNC_CRT_REBUILD(5); break;
+    case 6: UNC_CRT_REBUILD(6); break;
+    case 7: UNC_CRT_REBUILD(7); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef UNC_CRT_REBUILD
   return hipGetLastError();
 }
 
@@ -606,7 +614,7 @@ size_t crt_gemm_work_bytes(int M, int N, int K, int nmod) {
 
 hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
                     int ldb, bool trans_b, double beta, double *C, int ldc, int tri, int nmod,
-                    void *work, size_t work_bytes, hipStream_t st, int depth) {
+                    void *work, size_t work_bytes, hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
   if (nmod < CRT_MIN_MOD || nmod > CRT_MAX_MOD || K <= 0 || K > CRT_MAX_K)
     return hipErrorInvalidValue;
@@ -625,7 +633,7 @@ hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda,
   else
     e = crt_split(N, K, B, ldb, false, (tri & GEMM_B_LOWER) ? 2 : 0, nmod, w + ba, &ob, st);
   if (e != hipSuccess) return e;
-  return crt_product(alpha, oa, ob, beta, C, ldc, tri, w + ba + bb, bc, st, depth);
+  return crt_product(alpha, oa, ob, beta, C, ldc, tri, w + ba + bb, bc, st);
 }
 
 }  // namespace unc

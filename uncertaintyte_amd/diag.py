@@ -36,9 +36,8 @@ def dlib():
     L.unc_diag_residual.argtypes = [i, vp, vp, vp, i]
     L.unc_diag_residual_oz.argtypes = [i, vp, vp, vp, i, dp]
     L.unc_diag_chol_inverse.argtypes = [i, vp, vp, vp]
-    L.unc_diag_mfma_i8_32.argtypes = [vp, vp, vp]
     L.unc_diag_gemm_ex.argtypes = [i, i, i, ctypes.c_double, vp, i, vp, i, i, ctypes.c_double,
-                                   vp, i, i, i, i, i, dp]
+                                   vp, i, i, i, i, dp]
     L.unc_diag_gemm_bench.argtypes = [i, i, i, i, dp]
     L.unc_diag_residual_bench.argtypes = [i, i, i, dp]
     L.unc_diag_leaf_bench.argtypes = [i, i, dp]
@@ -111,23 +110,12 @@ def gemm_sym_batch(A: np.ndarray, Bs, reps: int = 0):
 GEMM_A_LOWER, GEMM_B_LOWER, GEMM_BT_LOWER, GEMM_C_LOWER = 1, 2, 4, 8
 
 
-def mfma_i8_32(A: np.ndarray, B: np.ndarray) -> np.ndarray:
-    """A @ B (32 x 32 int8 each) by one v_mfma_i32_32x32x32_i8 with the lane
-    layout the CRT product uses."""
-    Ai = np.ascontiguousarray(A, np.int8)
-    Bi = np.ascontiguousarray(B, np.int8)
-    C = np.zeros((32, 32), np.int32)
-    _check(dlib().unc_diag_mfma_i8_32(Ai.ctypes.data, Bi.ctypes.data, C.ctypes.data),
-           "unc_diag_mfma_i8_32")
-    return C
-
-
 def gemm_ex(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0,
-            trans_b: bool = False, tri: int = 0, nmod: int = 0, reps: int = 0, depth: int = 0):
+            trans_b: bool = False, tri: int = 0, nmod: int = 0, reps: int = 0):
     """(C, avg_ms) with C = alpha A op(B) + beta C (op(B) = B.T when trans_b) under the
     operand flags `tri` (GEMM_*): nmod = 0 takes the FP64 MFMA product, 6..13 the int8
-    CRT product with that many moduli (its pipeline `depth` LDS buffers deep, 3..5, 0 =
-    the library's); avg_ms of `reps` more calls (None if reps = 0)."""
+    CRT product with that many moduli; avg_ms of `reps` more calls (None if
+    reps = 0)."""
     M, K = A.shape
     N = B.shape[0] if trans_b else B.shape[1]
     assert (B.shape[1] if trans_b else B.shape[0]) == K
@@ -137,7 +125,7 @@ def gemm_ex(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=
     ms = ctypes.c_double(0.0)
     _check(dlib().unc_diag_gemm_ex(M, N, K, alpha, Af.ctypes.data, M, Bf.ctypes.data,
                                    Bf.shape[0], int(trans_b), beta, Cf.ctypes.data, M, tri, nmod,
-                                   depth, reps, ctypes.byref(ms) if reps else None),
+                                   reps, ctypes.byref(ms) if reps else None),
            "unc_diag_gemm_ex")
     return np.ascontiguousarray(Cf), (ms.value if reps else None)
 
