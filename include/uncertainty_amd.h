@@ -86,9 +86,9 @@ typedef struct unc_options {
                          basis N; 0 disables refinement                           */
   int device;         /* HIP device ordinal                                           */
   int flags;          /* reserved, 0                                                  */
-  int crt_moduli;     /* UNC_PREC_MIXED: moduli of the int8 CRT products, 6..13 (more
-                         moduli = more bits kept: q = 23, 27, 31, 35, 39, 43, 46, 50);
-                         0 = automatic (12)                                         */
+  int crt_moduli;     /* UNC_PREC_MIXED: moduli of the int8 CRT products, 6..20 (more
+                         moduli = more bits kept: q = 23, 27, 31, 35, 39, 43, 46, 50,
+                         54, 58, 62, 65, 69, 73, 77); 0 = automatic (12)             */
 } unc_options;
 
 typedef struct unc_stats {

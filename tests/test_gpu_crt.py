@@ -10,9 +10,10 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211]
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191,
+          181, 179, 173]
 Q_BITS = {s: int(np.floor((np.log2(float(np.prod(np.array(MODULI[:s], dtype=object)))) - 1) / 2))
-          for s in range(6, 14)}
+          for s in range(6, 21)}
 
 
 def _diag():
@@ -29,11 +30,12 @@ def _bound(A, B, q, extra=None):
     return b + 4 * np.finfo(float).eps * np.abs(exact) + (0 if extra is None else extra)
 
 
-@pytest.mark.parametrize("nmod", [6, 8, 10, 12, 13])
+@pytest.mark.parametrize("nmod", [6, 8, 10, 12, 13, 16, 20])
 def test_crt_integer_operands_exact(nmod):
     """Integer operands whose row / column norms stay below 2^(q-3) are not cut:
-    the product is the exact integer product, bit for bit (6, 8, 10, 12 and 13
-    moduli rebuild on 3, 4, 5, 6 and 7 16-bit limbs)."""
+    the product is the exact integer product, bit for bit (6, 8, 10, 12, 13, 16
+    and 20 moduli rebuild on 3, 4, 5, 6, 7, 8 and 10 16-bit limbs; from 16
+    moduli the cut integers exceed 2^52 and the residues take two steps)."""
     d = _diag()
     rng = np.random.default_rng(nmod)
     M, N, K = 300, 260, 700
@@ -120,6 +122,6 @@ def test_crt_rejects_bad_moduli_count():
     d = _diag()
     from uncertaintyte_amd import UncError
     A = np.ones((4, 4))
-    for nmod in (5, 14):
+    for nmod in (5, 21):
         with pytest.raises(UncError):
             d.gemm_ex(A, A, nmod=nmod)

@@ -156,26 +156,29 @@ def test_residual_kernel_extraction(n, tile_cols):
     assert np.abs(R - Ro).max() <= 1e-10 * np.abs(Ro).max() + 1e-300
 
 
-@pytest.mark.parametrize("n", [1, 100, 300, 1100])
-def test_residual_int8_ozaki(n):
-    """The Newton residual on the int8 matrix cores (k_ozaki.hip): ten signed
-    7-bit digits per column-scaled operand, the 55 int8 products with s + t <= 11
-    summed exactly per level, the levels subtracted in FP64.  Proven bound
-    against the compensated residual: 12 n 2^(e_i + f_j - 70) (2^e_i < 2 rowmax|A|_i,
-    2^f_j < 2 colmax|X|_j) + 2 eps |R_ij|, plus the oracle's own 17 n^3 eps^2.
-    A must be symmetric (its rows are sliced as columns), X need not be."""
-    say(f"residual (int8 ozaki) n={n}")
+@pytest.mark.parametrize("n,blocks", [(1, 1), (100, 1), (300, 1), (300, 3), (1100, 1),
+                                      (1100, 4)])
+def test_residual_int8_crt(n, blocks):
+    """The Newton residual on the int8 matrix cores (residual_crt): rows of A and
+    columns of X scaled to q = 77-bit integers, their exact product formed modulo
+    20 moduli and rebuilt by the CRT, the identity subtracted in the residues.
+    Bound against the compensated residual: 2^(3 - q) sqrt(n) |A_i.| |X_.j| (the
+    cut) + 4 eps |R_ij| (the final roundings), plus the oracle's own
+    17 n^3 eps^2 rowmax colmax.  A must be symmetric (its rows are cut as
+    columns), X need not be.  The row blocks change no value."""
+    say(f"residual (int8 crt) n={n} blocks={blocks}")
     A, X = _residual_case(n, seed=11)
-    R, _ = diag.residual_oz(A, X)
+    R, _ = diag.residual_crt(A, X, blocks=blocks)
     Ro = oracle.residual_dd(A, X)
     eps = 2.0 ** -53
+    ra, cx = np.linalg.norm(A, axis=1), np.linalg.norm(X, axis=0)
     rmax, cmax = np.abs(A).max(axis=1), np.abs(X).max(axis=0)
-    bound = ((12.0 * n * 4.0 * 2.0 ** -70 + 17.0 * n ** 3 * eps ** 2) * np.outer(rmax, cmax)
-             + 4 * eps * np.abs(Ro))
+    bound = (2.0 ** (3 - 77) * np.sqrt(n) * np.outer(ra, cx)
+             + 17.0 * n ** 3 * eps ** 2 * np.outer(rmax, cmax) + 4 * eps * np.abs(Ro))
     assert np.all(np.abs(R - Ro) <= bound)
-    # in practice an order of magnitude inside it (a numpy model of the same
-    # digits reaches 0.013 and 0.009 of the bound at n = 300 and 1100)
-    assert np.all(np.abs(R - Ro) <= 0.1 * bound)
+    if blocks > 1:
+        R1, _ = diag.residual_crt(A, X, blocks=1)
+        assert np.array_equal(R, R1)
 
 
 def test_inverse_kernel():

@@ -247,7 +247,7 @@ def compute_covariances(jac, precision: int = PREC_FP64, refine: bool = True, de
     """Run the pipeline on a host-resident Jacobian (jacob.Jacobian).  algorithm
     SVD_QR_ITERATION / SVD_DEVIDE_AND_CONQUER selects the dense pseudo-inverse
     modes (small problems only) with svd_remove_n / epsilon as in cov::Options.
-    moduli: PREC_MIXED's int8 CRT moduli (6..13, 0 = the library's default)."""
+    moduli: PREC_MIXED's int8 CRT moduli (6..20, 0 = the library's default)."""
     L = lib()
     o = make_options(jac, precision, refine, device, fix, lam, algorithm, svd_remove_n, epsilon,
                      moduli)

@@ -29,7 +29,7 @@ HIP_SOURCES = ["k_assemble.hip", "k_sort.hip", "k_schur.hip", "k_dense.hip", "k_
 PER_FILE_FLAGS = {"k_dense.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
                   "k_zplus.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CPP_SOURCES = ["uncertainty_api.cpp", "uncertainty_te.cpp", "scene_io.cpp", "server.cpp"]
-DIAG_SOURCES = ["diag.hip", "k_ozaki.hip"]
+DIAG_SOURCES = ["diag.hip"]
 DIAG_LIB = os.path.join(OUT, "libuncertainty_amd_diag.so")
 
 ARCH = os.environ.get("UNC_OFFLOAD_ARCH", "gfx950")

@@ -153,7 +153,7 @@ hipError_t gemm_f64_ex(int M, int N, int K, double alpha, const double *A, int l
 // int8 matrix cores and rebuilt by the Chinese remainder theorem
 // (k_crt_gemm.hip); same operand flags as gemm_f64_ex.  work: 256-byte aligned,
 // crt_gemm_work_bytes(M, N, K, nmod) bytes.
-constexpr int CRT_MIN_MOD = 6, CRT_MAX_MOD = 13, CRT_MAX_K = 131071;
+constexpr int CRT_MIN_MOD = 6, CRT_MAX_MOD = 20, CRT_MAX_K = 131071;
 int crt_bits(int nmod);  // q of nmod moduli (0 when out of range)
 size_t crt_gemm_work_bytes(int M, int N, int K, int nmod);
 hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda, const double *B,
@@ -165,7 +165,9 @@ hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda,
 // k >= r) into residue planes in `area` (crt_operand_bytes, 256-byte aligned);
 // crt_product forms C = alpha A B^T + beta C of two such operands (rows of A
 // are C's rows, rows of B its columns; tri as for gemm_f64_ex, consistent with
-// the operands' masks) with its residue planes in `cplanes` (crt_product_bytes).
+// the operands' masks) with its residue planes in `cplanes` (crt_product_bytes);
+// doff >= 0 subtracts the identity exactly, C = alpha (A B^T - D) + beta C with
+// D_ij = 1 where i + doff == j (I - A B^T for alpha = -1).
 struct CrtOperand {
   signed char *planes = nullptr;
   int *expo = nullptr;
@@ -177,7 +179,15 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
                      void *area, CrtOperand *out, hipStream_t st);
 hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
                        double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
-                       hipStream_t st);
+                       hipStream_t st, int doff = -1);
+// R = I - A X for symmetric A (the Newton step's residual) on the int8 matrix
+// cores: CRT_RESID_MOD moduli (q = 77 bits per cut row / column), A in row
+// blocks sized to `work` (256-byte aligned; residual_crt_work_bytes(n, blocks)
+// for `blocks` row blocks); invalid argument when even 64 blocks do not fit.
+constexpr int CRT_RESID_MOD = 20;
+size_t residual_crt_work_bytes(int n, int blocks);
+hipError_t residual_crt(int n, const double *A, const double *X, double *R, void *work,
+                        size_t work_bytes, hipStream_t st);
 // gauge-fixed Z^+ path (k_zplus.hip)
 // a recursion node of 64 < n <= 128 rows (halves 64 and n - 64) in one launch
 hipError_t chol_inv_node128(int n, double *A, int lda, double *X, int ldx, unsigned int *err,

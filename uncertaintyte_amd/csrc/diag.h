@@ -50,13 +50,11 @@ int unc_diag_inverse(int n, const double *A, double *X, int refine);
 /* R = I - A X by the refinement's residual kernel; tile_cols 16 or 32 forces the
  * 64 x 16 or 64 x 32 tiles, 0 takes the library's rule */
 int unc_diag_residual(int n, const double *A, const double *X, double *R, int tile_cols);
-/* R = I - A X for symmetric A and X by the int8 Ozaki residual (k_ozaki.hip, ten
- * digits per operand, 55 int8 products; built into this library only: at N = 1800
- * it takes 1.6 ms against the product's 1.3 ms error-free FP64 form, its 5.9 GB
- * of digit-panel re-reads being bound near 4 TB/s); avg_ms (may be NULL):
- * average of `reps` more calls */
-int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int reps,
-                         double *avg_ms);
+/* R = I - A X for symmetric A (X any) by the int8 CRT residual of the Newton
+ * step (residual_crt, 20 moduli), A cut in `blocks` row blocks (1..64);
+ * avg_ms (may be NULL): average of `reps` more calls */
+int unc_diag_residual_crt(int n, const double *A, const double *X, double *R, int blocks,
+                          int reps, double *avg_ms);
 /* Cholesky A = L L^T (only L's lower triangle is the factor; above the 64 x 64
  * diagonal blocks L keeps A's values) and X = L^-1 (upper triangle 0) by the
  * recursive kernels of the Z^+ path */

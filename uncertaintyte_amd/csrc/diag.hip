@@ -9,10 +9,6 @@
 #include "diag.h"
 
 namespace unc {
-// int8 Ozaki residual (k_ozaki.hip, this library only)
-size_t residual_oz_work_bytes(int n);
-hipError_t residual_oz(int n, const double *A, const double *X, double *R, void *work,
-                       hipStream_t st);
 }  // namespace unc
 
 using namespace unc;
@@ -241,18 +237,18 @@ int unc_diag_residual(int n, const double *A, const double *X, double *R, int ti
   return UNC_OK;
 }
 
-int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int reps,
-                         double *avg_ms) {
-  if (n <= 0 || !A || !X || !R || reps < 0) return UNC_ERR_BAD_INPUT;
+int unc_diag_residual_crt(int n, const double *A, const double *X, double *R, int blocks,
+                          int reps, double *avg_ms) {
+  if (n <= 0 || !A || !X || !R || reps < 0 || blocks < 1 || blocks > 64) return UNC_ERR_BAD_INPUT;
   DSTREAM(s);
-  const size_t nn = (size_t)n * n, wb = residual_oz_work_bytes(n);
+  const size_t nn = (size_t)n * n, wb = residual_crt_work_bytes(n, blocks);
   DevBuf buf;
   if (buf.alloc(sizeof(double) * 3 * nn + wb + 256)) return UNC_ERR_OOM;
   double *b = (double *)buf.p;
   void *work = (void *)(((uintptr_t)(b + 3 * nn) + 255) & ~(uintptr_t)255);
   DC(hipMemcpyAsync(b, A, sizeof(double) * nn, hipMemcpyHostToDevice, s));
   DC(hipMemcpyAsync(b + nn, X, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-  DC(residual_oz(n, b, b + nn, b + 2 * nn, work, s));
+  DC(residual_crt(n, b, b + nn, b + 2 * nn, work, wb, s));
   DC(hipMemcpyAsync(R, b + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
   DC(hipStreamSynchronize(s));
   if (avg_ms && reps > 0) {
@@ -260,7 +256,7 @@ int unc_diag_residual_oz(int n, const double *A, const double *X, double *R, int
     DC(hipEventCreate(&ev.a));
     DC(hipEventCreate(&ev.b));
     DC(hipEventRecord(ev.a, s));
-    for (int r = 0; r < reps; ++r) DC(residual_oz(n, b, b + nn, b + 2 * nn, work, s));
+    for (int r = 0; r < reps; ++r) DC(residual_crt(n, b, b + nn, b + 2 * nn, work, wb, s));
     DC(hipEventRecord(ev.b, s));
     DC(hipEventSynchronize(ev.b));
     float ms = 0.0f;

@@ -24,6 +24,7 @@
 // them: a plane holds blocks of 16 rows x 64 k, each a contiguous 1 KB in which
 // lane l's 16 bytes (row l % 16, k 16 (l / 16) .. + 15) sit at 16 l, so every
 // global load, LDS store and LDS read of a wave is one contiguous 1 KB.
+#include <algorithm>
 #include <cmath>
 
 #include "common.h"
@@ -40,26 +41,59 @@ constexpr int CRT_KB = 64;      // k per block and per pipeline step
 constexpr int CRT_BLK = 1024;   // bytes of one 16 x 64 residue block
 constexpr int CRT_NCH = 16;     // K chunks of the row-norm partial sums (gathered operands)
 constexpr int CRT_THREADS = 256;
+constexpr int CRT_NARROW_Q = 51;  // up to q bits the cut integers stay below 2^52
 
 // pairwise coprime moduli <= 256; their residues fit int8 in the symmetric range
-constexpr int kModuli[CRT_MAX_MOD] = {256, 255, 253, 251, 247, 241, 239,
-                                      233, 229, 227, 223, 217, 211};
+constexpr int kModuli[CRT_MAX_MOD] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                      223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
 
-constexpr int CRT_LIMBS = 7;  // 16-bit limbs of E_l < P < 2^112 (13 moduli: 2^103.3)
+constexpr int CRT_LIMBS = 10;  // 16-bit limbs of E_l < P < 2^160 (20 moduli: 2^155.4)
+constexpr int CRT_WORDS = 6;   // 32-bit words of P and of the rebuilt sums
 
 struct CrtConst {
   int nmod, q, nlimb;
   int m[CRT_MAX_MOD];
+  int c16[CRT_MAX_MOD];                      // 2^16 mod m
   double minv[CRT_MAX_MOD];
   unsigned int e16[CRT_MAX_MOD][CRT_LIMBS];  // E_l in 16-bit limbs, least significant first
-  unsigned long long plo, phi;                // P
-  double pinv;                                // 1 / P (rounded)
+  unsigned int p[CRT_WORDS], ph[CRT_WORDS];  // P and P / 2 in 32-bit words
+  double pinv;                               // 1 / P (rounded)
 };
 
-typedef unsigned __int128 u128;
+// Non-negative integers below 2^256 as eight 32-bit words (host only)
+struct Big {
+  unsigned int w[8] = {};
+  void mul(unsigned int f) {
+    unsigned long long c = 0;
+    for (unsigned int &x : w) {
+      c += (unsigned long long)x * f;
+      x = (unsigned int)c;
+      c >>= 32;
+    }
+  }
+  unsigned int divmod(unsigned int d) {  // *this /= d, returns the remainder
+    unsigned long long r = 0;
+    for (int k = 7; k >= 0; --k) {
+      const unsigned long long cur = (r << 32) | w[k];
+      w[k] = (unsigned int)(cur / d);
+      r = cur % d;
+    }
+    return (unsigned int)r;
+  }
+  int bits() const {
+    for (int k = 7; k >= 0; --k)
+      if (w[k]) return 32 * k + 32 - __builtin_clz(w[k]);
+    return 0;
+  }
+  double to_double() const {
+    double v = 0.0;
+    for (int k = 7; k >= 0; --k) v = v * 4294967296.0 + (double)w[k];
+    return v;
+  }
+};
 
-inline unsigned long long inv_mod(unsigned long long a, unsigned long long m) {
-  for (unsigned long long x = 1; x < m; ++x)
+inline unsigned int inv_mod(unsigned int a, unsigned int m) {
+  for (unsigned int x = 1; x < m; ++x)
     if ((a * x) % m == 1) return x;
   return 0;
 }
@@ -67,22 +101,27 @@ inline unsigned long long inv_mod(unsigned long long a, unsigned long long m) {
 CrtConst make_const(int nmod) {
   CrtConst c{};
   c.nmod = nmod;
-  u128 P = 1;
-  for (int l = 0; l < nmod; ++l) P *= (u128)kModuli[l];
+  Big P;
+  P.w[0] = 1;
+  for (int l = 0; l < nmod; ++l) P.mul((unsigned int)kModuli[l]);
   for (int l = 0; l < nmod; ++l) {
-    const unsigned long long m = (unsigned long long)kModuli[l];
-    const u128 Pl = P / m;
-    const unsigned long long y = inv_mod((unsigned long long)(Pl % m), m);
-    const u128 E = (Pl * y) % P;
+    const unsigned int m = (unsigned int)kModuli[l];
+    Big E = P;
+    (void)E.divmod(m);          // P_l = P / m (exact)
+    Big t = E;
+    const unsigned int y = inv_mod(t.divmod(m), m);
+    E.mul(y);                   // E_l = P_l (P_l^-1 mod m) < P
     c.m[l] = (int)m;
+    c.c16[l] = (int)(65536u % m);
     c.minv[l] = 1.0 / (double)m;
-    for (int k = 0; k < CRT_LIMBS; ++k) c.e16[l][k] = (unsigned int)(E >> (16 * k)) & 0xffffu;
+    for (int k = 0; k < CRT_LIMBS; ++k) c.e16[l][k] = (E.w[k / 2] >> (16 * (k & 1))) & 0xffffu;
   }
-  c.nlimb = 1;
-  while (c.nlimb < CRT_LIMBS && (P >> (16 * c.nlimb)) != 0) ++c.nlimb;
-  c.plo = (unsigned long long)P;
-  c.phi = (unsigned long long)(P >> 64);
-  const double pd = (double)c.phi * 18446744073709551616.0 + (double)c.plo;
+  for (int k = 0; k < CRT_WORDS; ++k) {
+    c.p[k] = P.w[k];
+    c.ph[k] = (P.w[k] >> 1) | (k + 1 < 8 ? P.w[k + 1] << 31 : 0u);
+  }
+  c.nlimb = (P.bits() + 15) / 16;
+  const double pd = P.to_double();
   c.pinv = 1.0 / pd;
   c.q = (int)std::floor((std::log2(pd) - 1.0) / 2.0);
   return c;
@@ -130,8 +169,19 @@ __global__ __launch_bounds__(256) void k_crt_norm_contig(int R, int K, const dou
   if (lane == 0) part[r] = s;
 }
 
-// Symmetric residue of the integer x (|x| < 2^52, held exactly as a double).
+// Symmetric residue of the integer x (|x| < 2^52, held exactly as a double;
+// WIDE: any |x| < 2^100, in two steps: x - m rint(x / m) is exact and below
+// 2^25 even where x / m rounds, and the second step leaves |r| <= m / 2).
+template <bool WIDE>
 __device__ __forceinline__ int crt_residue(double x, int m, double minv) {
+  if (WIDE) {
+    const double r1 = fma(-(double)m, rint(x * minv), x);
+    int r = (int)fma(-(double)m, rint(r1 * minv), r1);
+    const int hi = (m - 1) >> 1, lo = hi - m + 1;
+    if (r > hi) r -= m;
+    else if (r < lo) r += m;
+    return r;
+  }
   if (m == 256) {
     const int r = (int)((long long)x & 255);
     return r >= 128 ? r - 256 : r;
@@ -159,6 +209,7 @@ __device__ __forceinline__ int crt_row_exponent(int r, int R, const double *__re
 
 // The residues of the 16 integers x (a lane's row and 16 k of one 16 x 64 block)
 // for every modulus, 16 bytes per plane at the lane's slot of the block.
+template <bool WIDE>
 __device__ __forceinline__ void crt_emit(const double (&x)[16], const CrtConst &cc,
                                          signed char *__restrict__ planes, size_t plane_bytes,
                                          size_t at) {
@@ -171,7 +222,7 @@ __device__ __forceinline__ void crt_emit(const double (&x)[16], const CrtConst &
       unsigned int u = 0;
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        u |= (unsigned int)(unsigned char)(signed char)crt_residue(x[4 * g + b], m, mi) << (8 * b);
+        u |= (unsigned int)(unsigned char)(signed char)crt_residue<WIDE>(x[4 * g + b], m, mi) << (8 * b);
       w[g] = (int)u;
     }
     *(v4i *)(planes + (size_t)l * plane_bytes + at) = (v4i){w[0], w[1], w[2], w[3]};
@@ -182,6 +233,7 @@ __device__ __forceinline__ void crt_emit(const double (&x)[16], const CrtConst &
 // P[r + k ld] (rows gathered across k; `mask` keeps k <= r (1) or k >= r (2)):
 // one wave per 16 x 64 block, lane l taking row l % 16 and k 16 (l / 16) .. + 15,
 // so each load reads 16 consecutive rows.  expo[r] = the row's scale exponent.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_crt_split_gather(int R, int K, int Rp, int Kp,
                                                           const double *__restrict__ P, int ld,
                                                           int mask,
@@ -205,13 +257,15 @@ __global__ __launch_bounds__(256) void k_crt_split_gather(int R, int K, int Rp, 
     const double v = (r < R && k < K && crt_keep(mask, r, k)) ? P[(size_t)r + (size_t)k * ld] : 0.0;
     x[t] = rint(ldexp(v, e));
   }
-  crt_emit(x, cc, planes, plane_bytes, ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
+  crt_emit<WIDE>(x, cc, planes, plane_bytes,
+                 ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
 }
 
 // The same for an operand whose rows are contiguous in k (element (r, k) at
 // P[k + r ld]): a workgroup stages 16 rows x 256 k through LDS, each load
 // instruction reading 128 consecutive bytes of four rows, then each wave cuts one
 // 16 x 64 block from LDS.  One norm partial per row.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_crt_split_contig(int R, int K, int Rp, int Kp,
                                                           const double *__restrict__ P, int ld,
                                                           int mask,
@@ -241,7 +295,8 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(int R, int K, int Rp, 
   double x[16];
 #pragma unroll
   for (int t = 0; t < 16; ++t) x[t] = rint(ldexp(T[lane & 15][kl0 + t], e));
-  crt_emit(x, cc, planes, plane_bytes, ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
+  crt_emit<WIDE>(x, cc, planes, plane_bytes,
+                 ((size_t)rb * kblocks + kb) * CRT_BLK + 16 * (size_t)lane);
 }
 
 // Residue (in [0, m)) of an int32 product sum.
@@ -394,53 +449,55 @@ __global__ __launch_bounds__(CRT_THREADS, 2) void k_crt_gemm(
     }
 }
 
-// 128-bit helpers on two 64-bit limbs
-__device__ __forceinline__ bool u128_ge(unsigned long long alo, unsigned long long ahi,
-                                        unsigned long long blo, unsigned long long bhi) {
-  return ahi > bhi || (ahi == bhi && alo >= blo);
+// 2^s mod m (s >= 0) by 16-bit steps: c16 = 2^16 mod m
+__device__ __forceinline__ unsigned int crt_pow2(int s, int m, double minv, int c16) {
+  unsigned int v = crt_reduce(1 << (s & 15), m, minv);
+  for (int t = s >> 4; t > 0; --t) v = crt_reduce((int)v * c16, m, minv);
+  return v;
 }
 
-__device__ __forceinline__ void u128_sub(unsigned long long &alo, unsigned long long &ahi,
-                                         unsigned long long blo, unsigned long long bhi) {
-  const unsigned long long borrow = alo < blo ? 1ull : 0ull;
-  alo -= blo;
-  ahi -= bhi + borrow;
-}
-
-// v |= d << sh on a 128-bit value held as two 64-bit limbs (sh a constant)
-__device__ __forceinline__ void u128_put(unsigned long long &lo, unsigned long long &hi,
-                                         unsigned long long d, int sh) {
-  if (sh >= 64) {
-    hi |= d << (sh - 64);
-  } else {
-    lo |= d << sh;
-    if (sh > 0) hi |= d >> (64 - sh);
-  }
-}
-
-// C_ij = alpha 2^-(ea_i + eb_j) crt(r_1 .. r_s) + beta C_ij.  One thread per
-// four rows of one column, read from the blocked C planes as one 32-bit word
-// per plane (the lane's word of a 16 x 16 block); 256 threads take four row
-// blocks of one column block.  The CRT sum sum_l r_l E_l is formed on NL 16-bit
-// limbs of E_l: s_k = sum_l r_l e_lk < 13 * 255 * 65535 < 2^28 with one 24-bit
-// integer multiply-add per term (full rate), then carried into 128 bits.  With
-// `lower` only i >= j is written.
+// C_ij = alpha 2^-(ea_i + eb_j) (crt(r_1 .. r_s) - D_ij 2^(ea_i + eb_j)) + beta C_ij,
+// D_ij = 1 where dshift and i + doff == j (the identity, subtracted exactly in
+// the residues, so that I - A X keeps the digits its cancellation leaves), else 0.
+// One thread per four rows of one column, read from the blocked C planes as one
+// 32-bit word per plane (the lane's word of a 16 x 16 block); 256 threads take
+// four row blocks of one column block.  The CRT sum sum_l r_l E_l is formed on
+// NL 16-bit limbs of E_l: s_k = sum_l r_l e_lk < 20 * 255 * 65535 < 2^29 with
+// one 24-bit integer multiply-add per term (full rate), carried into NW = (NL +
+// 2) / 2 32-bit words, reduced modulo P (quotient from FP64, then exact word
+// arithmetic) and taken in the symmetric range.  With `lower` only i >= j is
+// written.
 template <int NL>
 __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int lower, double alpha,
-                                                     double beta, CrtConst cc,
+                                                     double beta, int dshift, int doff,
+                                                     CrtConst cc,
                                                      const unsigned char *__restrict__ PC,
                                                      size_t plane_c, const int *__restrict__ ea,
                                                      const int *__restrict__ eb, double *C,
                                                      int ldc) {
+  constexpr int NW = (NL + 2) / 2;
   const int lane = threadIdx.x & 63;
   const int rb = blockIdx.x * 4 + (threadIdx.x >> 6), cb = blockIdx.y;
   const int j = cb * 16 + (lane & 15), i0 = rb * 16 + 4 * (lane >> 4);
   if (i0 >= M || j >= N) return;
   if (lower && i0 + 3 < j) return;
   const size_t at = ((size_t)cb * (Mp / 16) + rb) * 256 + 4 * lane;
+  const int ej = eb[j];
+  // the diagonal element among this thread's four rows, if any, and its scale
+  const int bd = dshift ? j - doff - i0 : -1;
+  const bool has_d = bd >= 0 && bd < 4 && i0 + bd < M;
+  const int sd = has_d ? ea[i0 + bd] + ej : -1;
+  const bool shift_res = has_d && sd >= 0;
   unsigned int sk[4][NL] = {};  // [row][limb]
   for (int l = 0; l < cc.nmod; ++l) {
-    const unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + at);
+    unsigned int u = *(const unsigned int *)(PC + (size_t)l * plane_c + at);
+    if (shift_res) {  // r <- r - 2^sd mod m in the diagonal row's byte
+      const int m = cc.m[l];
+      const unsigned int r = __builtin_amdgcn_ubfe(u, 8 * bd, 8);
+      const unsigned int p2 = crt_pow2(sd, m, cc.minv[l], cc.c16[l]);
+      const unsigned int nr = r >= p2 ? r - p2 : r + (unsigned int)m - p2;
+      u = (u & ~(0xffu << (8 * bd))) | (nr << (8 * bd));
+    }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const unsigned int r = __builtin_amdgcn_ubfe(u, 8 * b, 8);
@@ -448,46 +505,68 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int l
       for (int k = 0; k < NL; ++k) sk[b][k] += __umul24(r, cc.e16[l][k]);
     }
   }
-  const int ej = eb[j];
-  const unsigned long long hlo = (cc.plo >> 1) | (cc.phi << 63), hhi = cc.phi >> 1;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int i = i0 + b;
     if (i >= M || (lower && i < j)) continue;
-    // carry the limb sums into x = lo + 2^64 hi
-    unsigned long long lo = 0, hi = 0;
+    // carry the limb sums into NW words
+    unsigned int x[NW] = {};
     unsigned int c = 0;
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
       c += sk[b][k];
-      u128_put(lo, hi, c & 0xffffu, 16 * k);
+      x[k >> 1] |= (c & 0xffffu) << (16 * (k & 1));
       c >>= 16;
     }
-    u128_put(lo, hi, c, 16 * NL);
-    // x mod P: the quotient (< 2^12) from FP64, one below to stay non-negative
-    const double xd = (double)hi * 18446744073709551616.0 + (double)lo;
+    x[NL >> 1] |= c << (16 * (NL & 1));
+    // x mod P: the quotient (< 2^14) from FP64, one below to stay non-negative
+    double xd = 0.0;
+#pragma unroll
+    for (int k = NW - 1; k >= 0; --k) xd = xd * 4294967296.0 + (double)x[k];
     const double qd = floor(xd * cc.pinv) - 1.0;
     if (qd > 0.0) {
       const unsigned long long q = (unsigned long long)qd;
-      const unsigned long long plo = cc.plo, p0 = plo & 0xffffffffull, p1 = plo >> 32;
-      unsigned long long t = q * p0;
-      unsigned long long qlo = t & 0xffffffffull;
-      t = (t >> 32) + q * p1;
-      qlo |= t << 32;
-      const unsigned long long qhi = (t >> 32) + q * cc.phi;
-      u128_sub(lo, hi, qlo, qhi);
+      long long borrow = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const long long t = (long long)x[k] - (long long)(q * cc.p[k]) - borrow;
+        x[k] = (unsigned int)t;
+        borrow = -(t >> 32);
+      }
     }
-    while (u128_ge(lo, hi, cc.plo, cc.phi)) u128_sub(lo, hi, cc.plo, cc.phi);
+    auto ge = [&](const unsigned int *y) {  // x >= y
+      for (int k = NW - 1; k >= 0; --k)
+        if (x[k] != y[k]) return x[k] > y[k];
+      return true;
+    };
+    auto sub = [&](unsigned int *a, const unsigned int *y) {  // a -= y
+      unsigned int br = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const unsigned long long t = (unsigned long long)a[k] - y[k] - br;
+        a[k] = (unsigned int)t;
+        br = (unsigned int)(t >> 63);
+      }
+    };
+    while (ge(cc.p)) sub(x, cc.p);
     // symmetric range: x >= P / 2 stands for x - P
-    double v;
-    if (u128_ge(lo, hi, hlo, hhi)) {
-      unsigned long long nlo = cc.plo, nhi = cc.phi;
-      u128_sub(nlo, nhi, lo, hi);
-      v = -((double)nhi * 18446744073709551616.0 + (double)nlo);
-    } else {
-      v = (double)hi * 18446744073709551616.0 + (double)lo;
+    const bool neg = ge(cc.ph);
+    if (neg) {
+      unsigned int y[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) y[k] = cc.p[k];
+      sub(y, x);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) x[k] = y[k];
     }
-    double out = alpha * ldexp(v, -(ea[i] + ej));
+    double v = 0.0;
+#pragma unroll
+    for (int k = NW - 1; k >= 0; --k) v = v * 4294967296.0 + (double)x[k];
+    if (neg) v = -v;
+    const int sc = ea[i] + ej;
+    double out;
+    if (has_d && b == bd && !shift_res) out = alpha * (ldexp(v, -sc) - 1.0);
+    else out = alpha * ldexp(v, -sc);
     double *cp = C + (size_t)i + (size_t)j * ldc;
     if (beta != 0.0) out = fma(beta, *cp, out);
     *cp = out;
@@ -500,7 +579,9 @@ const CrtConst &crt_const(int nmod) {
   static const CrtConst table[CRT_MAX_MOD + 1] = {
       make_const(1),  make_const(1),  make_const(2),  make_const(3),  make_const(4),
       make_const(5),  make_const(6),  make_const(7),  make_const(8),  make_const(9),
-      make_const(10), make_const(11), make_const(12), make_const(13)};
+      make_const(10), make_const(11), make_const(12), make_const(13), make_const(14),
+      make_const(15), make_const(16), make_const(17), make_const(18), make_const(19),
+      make_const(20)};
   return table[nmod];
 }
 
@@ -560,14 +641,22 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
     const unsigned sg = cdiv((long long)(Ly.Rp / 16) * (Ly.Kp / CRT_KB), 4);
     hipLaunchKernelGGL(k_crt_norm_gather, dim3(cdiv(R, 64), CRT_NCH), dim3(256), 0, st, R, K, P,
                        ld, mask, part);
-    hipLaunchKernelGGL(k_crt_split_gather, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
-                       (int)Ly.Kp, P, ld, mask, part, CRT_NCH, cc, o.planes, Ly.plane, o.expo);
+    if (cc.q > CRT_NARROW_Q)
+      hipLaunchKernelGGL(k_crt_split_gather<true>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                         (int)Ly.Kp, P, ld, mask, part, CRT_NCH, cc, o.planes, Ly.plane, o.expo);
+    else
+      hipLaunchKernelGGL(k_crt_split_gather<false>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                         (int)Ly.Kp, P, ld, mask, part, CRT_NCH, cc, o.planes, Ly.plane, o.expo);
   } else {
     const unsigned sg = (unsigned)(Ly.Rp / 16) * cdiv((long long)(Ly.Kp / CRT_KB), 4);
     hipLaunchKernelGGL(k_crt_norm_contig, dim3(cdiv(R, 4)), dim3(256), 0, st, R, K, P, ld, mask,
                        part);
-    hipLaunchKernelGGL(k_crt_split_contig, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
-                       (int)Ly.Kp, P, ld, mask, part, cc, o.planes, Ly.plane, o.expo);
+    if (cc.q > CRT_NARROW_Q)
+      hipLaunchKernelGGL(k_crt_split_contig<true>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                         (int)Ly.Kp, P, ld, mask, part, cc, o.planes, Ly.plane, o.expo);
+    else
+      hipLaunchKernelGGL(k_crt_split_contig<false>, dim3(sg), dim3(256), 0, st, R, K, (int)Ly.Rp,
+                         (int)Ly.Kp, P, ld, mask, part, cc, o.planes, Ly.plane, o.expo);
   }
   *out = o;
   return hipGetLastError();
@@ -575,12 +664,13 @@ hipError_t crt_split(int R, int K, const double *P, int ld, bool gather, int mas
 
 hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, double beta,
                        double *C, int ldc, int tri, void *cplanes, size_t cplanes_size,
-                       hipStream_t st) {
+                       hipStream_t st, int doff) {
   const int M = A.R, N = B.R, nmod = A.nmod;
   if (M <= 0 || N <= 0) return hipSuccess;
   if (A.K != B.K || A.Kp != B.Kp || A.nmod != B.nmod || nmod < CRT_MIN_MOD ||
       nmod > CRT_MAX_MOD || !cplanes || ((uintptr_t)cplanes & 255) != 0 ||
-      cplanes_size < cplanes_bytes(M, N, nmod) || ((tri & GEMM_C_LOWER) && M != N))
+      cplanes_size < cplanes_bytes(M, N, nmod) || ((tri & GEMM_C_LOWER) && M != N) ||
+      (doff >= 0 && (tri & GEMM_C_LOWER)))
     return hipErrorInvalidValue;
   const CrtConst &cc = crt_const(nmod);
   unsigned char *pc = (unsigned char *)cplanes;
@@ -592,14 +682,17 @@ hipError_t crt_product(double alpha, const CrtOperand &A, const CrtOperand &B, d
                      tm, tn, ntiles, tri, cc, A.planes, B.planes, plane_a, plane_b, pc, plane_c);
 #define UNC_CRT_REBUILD(NL)                                                                \
   hipLaunchKernelGGL(k_crt_rebuild<NL>, dim3(A.Rp / 64, B.Rp / 16), dim3(256), 0, st, M, N, A.Rp, \
-                     (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, cc, pc, plane_c, A.expo, B.expo, \
-                     C, ldc)
+                     (tri & GEMM_C_LOWER) ? 1 : 0, alpha, beta, doff >= 0 ? 1 : 0, doff, cc, pc, \
+                     plane_c, A.expo, B.expo, C, ldc)
   switch (cc.nlimb) {
     case 3: UNC_CRT_REBUILD(3); break;
     case 4: UNC_CRT_REBUILD(4); break;
     case 5: UNC_CRT_REBUILD(5); break;
     case 6: UNC_CRT_REBUILD(6); break;
     case 7: UNC_CRT_REBUILD(7); break;
+    case 8: UNC_CRT_REBUILD(8); break;
+    case 9: UNC_CRT_REBUILD(9); break;
+    case 10: UNC_CRT_REBUILD(10); break;
     default: return hipErrorInvalidValue;
   }
 #undef UNC_CRT_REBUILD
@@ -636,4 +729,40 @@ hipError_t gemm_crt(int M, int N, int K, double alpha, const double *A, int lda,
   return crt_product(alpha, oa, ob, beta, C, ldc, tri, w + ba + bb, bc, st);
 }
 
+// R = I - A X for symmetric A: rows of A are its columns, so both operands are
+// cut from contiguous columns (crt_split, gather = false); A in row blocks of h
+// rows so that the residue planes fit `work` (CRT_RESID_MOD moduli: q = 77, the
+// exact product of the cut operands differs from A X by at most
+// 2^-74 sqrt(n) |A_i.| |X_.j|, and the identity is taken off in the residues).
+size_t residual_crt_work_bytes(int n, int blocks) {
+  if (n <= 0 || blocks <= 0) return 0;
+  const int h = (int)cdiv(n, blocks);
+  return crt_operand_bytes(n, n, CRT_RESID_MOD) + crt_operand_bytes(h, n, CRT_RESID_MOD) +
+         crt_product_bytes(h, n, CRT_RESID_MOD);
+}
+
+hipError_t residual_crt(int n, const double *A, const double *X, double *R, void *work,
+                        size_t work_bytes, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (n > CRT_MAX_K || !work || ((uintptr_t)work & 255) != 0) return hipErrorInvalidValue;
+  int blocks = 1;
+  while (blocks <= 64 && residual_crt_work_bytes(n, blocks) > work_bytes) ++blocks;
+  if (blocks > 64) return hipErrorInvalidValue;
+  const int h = (int)cdiv(n, blocks);
+  char *w = (char *)work;
+  const size_t bx = crt_operand_bytes(n, n, CRT_RESID_MOD);
+  const size_t ba = crt_operand_bytes(h, n, CRT_RESID_MOD);
+  const size_t bc = crt_product_bytes(h, n, CRT_RESID_MOD);
+  CrtOperand ox, oa;
+  hipError_t e = crt_split(n, n, X, n, false, 0, CRT_RESID_MOD, w, &ox, st);
+  for (int r0 = 0; e == hipSuccess && r0 < n; r0 += h) {
+    const int hr = std::min(h, n - r0);
+    e = crt_split(hr, n, A + (size_t)r0 * n, n, false, 0, CRT_RESID_MOD, w + bx, &oa, st);
+    if (e == hipSuccess)
+      e = crt_product(-1.0, oa, ox, 0.0, R + r0, n, 0, w + bx + ba, bc, st, r0);
+  }
+  return e;
+}
+
 }  // namespace unc
+

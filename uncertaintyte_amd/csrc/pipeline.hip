@@ -77,6 +77,14 @@ constexpr int ZPLUS_CRT_MIN_N = 2048;
 constexpr int TE_BATCH_MIN_N = 1024;
 bool te_batched(int N) { return N >= TE_BATCH_MIN_N; }
 
+// Taylor chain: the first product's K chunks, and the five power slots of the
+// batched chain where it runs (the chunks alias the first three)
+size_t make_sizes_split_bytes(const Sizes &z) {
+  return z.te ? sizeof(double) * std::max(gemm_sym_split_work_doubles(z.N),
+                                          te_batched(z.N) ? 5 * z.nn : (size_t)0)
+              : 0;
+}
+
 Layout make_layout(const Sizes &z) {
   size_t sz[B_COUNT];
   sz[B_OBS_CAM] = sz[B_OBS_PT] = sz[B_PT_LIST] = sizeof(int) * (size_t)z.nobs;
@@ -98,11 +106,7 @@ Layout make_layout(const Sizes &z) {
   sz[B_TP] = sz[B_RP] = sizeof(double) * GJ_BLOCK * (size_t)z.N;
   sz[B_DA] = sz[B_DB] = sz[B_DC] = sz[B_DD] = sizeof(double) * z.nn;
   sz[B_DE] = sizeof(double) * z.nn;
-  // Taylor chain: the first product's K chunks, and the five power slots of
-  // the batched chain where it runs (the chunks alias the first three)
-  sz[B_SPLIT] = z.te ? sizeof(double) * std::max(gemm_sym_split_work_doubles(z.N),
-                                                 te_batched(z.N) ? 5 * z.nn : (size_t)0)
-                     : 0;
+  sz[B_SPLIT] = make_sizes_split_bytes(z);
   sz[B_NULL] = sizeof(double) * zplus_work_doubles(z.N);
   sz[B_CRT] = z.mixed ? zplus_crt_bytes(z.N, z.moduli) : 0;
   Layout L;
@@ -769,7 +773,14 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     // antisymmetric part of the size of X0's asymmetry, which mirroring would
     // fold into the result.)
     HC(svd_symmetrize(N, dB, s));
-    HC(residual_ex(N, dA, dB, dC, dE, s));  // dE is free until X1 is formed
+    if (te_batched(N)) {
+      // on the int8 matrix cores; its residue planes take D, E and the chain
+      // slots, all free until X1 is formed
+      const size_t span = L.off[B_SPLIT] + make_sizes_split_bytes(z) - L.off[B_DD];
+      HC(residual_crt(N, dA, dB, dC, ptr(B_DD), span, s));
+    } else {
+      HC(residual_ex(N, dA, dB, dC, dE, s));  // dE is free until X1 is formed
+    }
     HC(hipEventRecord(GE[2 * gemm_pairs], s));
     HC(gemm_f64_sym(N, 1.0, dB, dC, dD, sym_work, sym_work_doubles, nullptr, nullptr, s));
     HC(hipEventRecord(GE[2 * gemm_pairs + 1], s));

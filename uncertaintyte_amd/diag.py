@@ -34,7 +34,7 @@ def dlib():
     L.unc_diag_gemm_f64_sym_batch.argtypes = [i, i, vp, vp, vp, dp, i, dp]
     L.unc_diag_inverse.argtypes = [i, vp, vp, i]
     L.unc_diag_residual.argtypes = [i, vp, vp, vp, i]
-    L.unc_diag_residual_oz.argtypes = [i, vp, vp, vp, i, dp]
+    L.unc_diag_residual_crt.argtypes = [i, vp, vp, vp, i, i, dp]
     L.unc_diag_chol_inverse.argtypes = [i, vp, vp, vp]
     L.unc_diag_gemm_ex.argtypes = [i, i, i, ctypes.c_double, vp, i, vp, i, i, ctypes.c_double,
                                    vp, i, i, i, i, dp]
@@ -162,16 +162,17 @@ def residual(A: np.ndarray, X: np.ndarray, tile_cols: int = 0) -> np.ndarray:
     return np.asarray(R)
 
 
-def residual_oz(A: np.ndarray, X: np.ndarray, reps: int = 0):
-    """(R, avg_ms) with R = I - A X for symmetric A, X by the int8 Ozaki kernels;
-    avg_ms over `reps` further calls (None when reps == 0)."""
+def residual_crt(A: np.ndarray, X: np.ndarray, blocks: int = 1, reps: int = 0):
+    """(R, avg_ms) with R = I - A X for symmetric A by the int8 CRT residual (A in
+    `blocks` row blocks); avg_ms over `reps` further calls (None when reps == 0)."""
     n = A.shape[0]
     Af = np.asfortranarray(A, dtype=np.float64)
     Xf = np.asfortranarray(X, dtype=np.float64)
     R = np.zeros((n, n), order="F")
     ms = ctypes.c_double()
-    _check(dlib().unc_diag_residual_oz(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data, reps,
-                                       ctypes.byref(ms) if reps else None), "unc_diag_residual_oz")
+    _check(dlib().unc_diag_residual_crt(n, Af.ctypes.data, Xf.ctypes.data, R.ctypes.data,
+                                        blocks, reps, ctypes.byref(ms) if reps else None),
+           "unc_diag_residual_crt")
     return np.asarray(R), (ms.value if reps else None)
 
 
