@@ -175,14 +175,15 @@ class _PinnedBlock:
 
     def __init__(self, nbytes: int):
         self.ptr = ctypes.c_void_p()
-        self._free = lib().unc_host_free  # held: module globals may be gone at exit
+        # held: module globals (lib, ctypes) may be gone when this runs at exit
+        self._free = lib().unc_host_free
         _check(lib().unc_host_alloc(max(nbytes, 1), ctypes.byref(self.ptr)), "unc_host_alloc")
         self.nbytes = nbytes
 
     def __del__(self):
-        if self.ptr:
-            self._free(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        p, self.ptr = self.ptr, None
+        if p:
+            self._free(p)
 
 
 def pinned_empty(shape, dtype) -> np.ndarray:

@@ -55,9 +55,9 @@ int unc_diag_residual(int n, const double *A, const double *X, double *R, int ti
  * avg_ms (may be NULL): average of `reps` more calls */
 int unc_diag_residual_crt(int n, const double *A, const double *X, double *R, int blocks,
                           int reps, double *avg_ms);
-/* Cholesky A = L L^T (only L's lower triangle is the factor; above the 64 x 64
- * diagonal blocks L keeps A's values) and X = L^-1 (upper triangle 0) by the
- * recursive kernels of the Z^+ path */
+/* X = L^-1 (upper triangle 0) for the Cholesky factor A = L L^T by the recursive
+ * kernels of the Z^+ path; L holds the factor in the diagonal blocks of the
+ * recursion's leaves and 128-row nodes only (elsewhere the values A had then) */
 int unc_diag_chol_inverse(int n, const double *A, double *L, double *X);
 /* average ms of one n x n x n product over reps launches on device-resident
  * operands: form 0 plain, 1 split-K, 2 symmetric lower-tile form with `parts`

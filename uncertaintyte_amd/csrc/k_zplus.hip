@@ -99,6 +99,66 @@ __device__ __forceinline__ void tri_panel8(double (*S)[CL_B + 1], double (*Xs)[C
   for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
 }
 
+// Waves 1-3 (widx 0..2) of the panelled leaf: the bulk rank-8 updates on the
+// FP64 matrix cores, over 16 x 16 tiles (v_mfma_f64_16x16x4: lane l holds A's
+// row l % 16 and B's column l % 16 at k = l / 16; accumulator r is the tile's
+// row l / 16 + 4 r, column l % 16), the tile's old values as the accumulator
+// and only the elements of the update's region written back.
+//   Cholesky: S[i][k] -= sum_jj S[i][p0 + jj] S[k][p0 + jj] for q0 + 8 <= k <= i
+//   (panel p0 = q0 - 8 on the trailing triangle beyond the panel being factored);
+__device__ __forceinline__ void chol_bulk_mfma(double (*S)[CL_B + 1], int p0, int q0, int widx,
+                                               int lane) {
+  const int st = q0 + CP_W;
+  if (st >= CL_B) return;
+  const int t0 = st >> 4, nt = 4 - t0, ntiles = nt * (nt + 1) / 2;
+  const int kq = lane >> 4, lc = lane & 15;
+  for (int t = widx; t < ntiles; t += 3) {
+    int ti = t0, rem = t;  // lower tiles row by row from (t0, t0)
+    while (rem > ti - t0) {
+      rem -= ti - t0 + 1;
+      ++ti;
+    }
+    const int r0 = 16 * ti, bc = 16 * (t0 + rem) + lc;
+    double4_t acc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = S[r0 + kq + 4 * r][bc];
+#pragma unroll
+    for (int ks = 0; ks < CP_W; ks += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-S[r0 + lc][p0 + ks + kq], S[bc][p0 + ks + kq],
+                                                 acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + kq + 4 * r;
+      if (bc >= st && bc <= i) S[i][bc] = acc[r];
+    }
+  }
+}
+
+//   inverse: Xs[r][c] -= sum_kk S[r][p0 + kk] Xs[p0 + kk][c] for rows r >= q0 + 8
+//   (only columns c < q0 change: X's rows of panel p0 are zero beyond p0 + 7).
+__device__ __forceinline__ void inv_bulk_mfma(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1],
+                                              int p0, int q0, int widx, int lane) {
+  const int st = q0 + CP_W;
+  if (st >= CL_B || q0 <= 0) return;
+  const int ti0 = st >> 4, ntj = ((q0 - 1) >> 4) + 1, ntiles = (4 - ti0) * ntj;
+  const int kq = lane >> 4, lc = lane & 15;
+  for (int t = widx; t < ntiles; t += 3) {
+    const int r0 = 16 * (ti0 + t / ntj), bc = 16 * (t % ntj) + lc;
+    double4_t acc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = Xs[r0 + kq + 4 * r][bc];
+#pragma unroll
+    for (int ks = 0; ks < CP_W; ks += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-S[r0 + lc][p0 + ks + kq], Xs[p0 + ks + kq][bc],
+                                                 acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + kq + 4 * r;
+      if (i >= st) Xs[i][bc] = acc[r];
+    }
+  }
+}
+
 // S: the SPD block (lower triangle valid, upper triangle zero) -> L with zeros
 // above the diagonal; Xs -> L^-1 with zeros above.  Returns true in wave 0 when
 // a pivot was not positive.  The first 256 threads work (act); in a larger
@@ -134,16 +194,8 @@ __device__ __forceinline__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)
           const int r = x >> 6, c = x & 63;
           Xs[r][c] = r == c ? 1.0 : 0.0;
         }
-      } else if (i < CL_B) {  // panel p0's update of columns k in [q0 + 8, i]
-        double li[CP_W];
-#pragma unroll
-        for (int jj = 0; jj < CP_W; ++jj) li[jj] = S[i][p0 + jj];
-        for (int k = q0 + CP_W + wave - 1; k <= i; k += 3) {
-          double acc = S[i][k];
-#pragma unroll
-          for (int jj = 0; jj < CP_W; ++jj) acc = fma(-li[jj], S[k][p0 + jj], acc);
-          S[i][k] = acc;
-        }
+      } else {  // panel p0's update of columns k in [q0 + 8, i]
+        chol_bulk_mfma(S, p0, q0, wave - 1, lane);
       }
     }
     __syncthreads();
@@ -167,15 +219,7 @@ __device__ __forceinline__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)
     if (wave == 0) {
       tri_panel8(S, Xs, q0, lane);
     } else if (act && p0 >= 0) {  // panel p0's update of rows r >= q0 + 8
-      double xk[CP_W];
-#pragma unroll
-      for (int kk = 0; kk < CP_W; ++kk) xk[kk] = Xs[p0 + kk][c];
-      for (int r = q0 + CP_W + wave - 1; r < CL_B; r += 3) {
-        double acc = Xs[r][c];
-#pragma unroll
-        for (int kk = 0; kk < CP_W; ++kk) acc = fma(-S[r][p0 + kk], xk[kk], acc);
-        Xs[r][c] = acc;
-      }
+      inv_bulk_mfma(S, Xs, p0, q0, wave - 1, lane);
     }
     __syncthreads();
   }

@@ -254,8 +254,10 @@ struct RecProducts {
   size_t side_scratch_bytes = 0;
 };
 
-// Nodes of at least this many rows fork their L21 X11 onto the second stream.
-constexpr int REC_FORK_MIN_N = 1024;
+// Nodes of at least this many rows fork their L21 X11 onto the second stream
+// (walk1500, 13500 rows, mixed: 38.8 ms for the recursion with 1024, 37.7 with
+// 128; the FP64 path is level within noise).
+constexpr int REC_FORK_MIN_N = 128;
 
 // The first half of a recursion node of n rows: the multiple of 64 at or just
 // above n / 2 (the second half, n - n1, can be one row larger when n is odd).
@@ -345,12 +347,14 @@ bool rec_fork_layout(int n, double *W, RecProducts *P) {
 size_t zplus_crt_bytes(int n, int nmod) { return zplus_crt_layout(n, nmod, nullptr, nullptr); }
 
 // Recursive Cholesky of the SPD block A[off.., off..] (n x n, ld N) with the
-// inverse of its factor: L overwrites A's lower triangle, X = L^-1 goes to the
-// same block of X.  With A = [A11 .; A21 A22]:
+// inverse of its factor: X = L^-1 goes to the same block of X (its upper
+// triangle untouched).  With A = [A11 .; A21 A22]:
 //   (L11, X11) = chol_inv(A11);  L21 = A21 X11^T;  A22 -= L21 L21^T;
 //   (L22, X22) = chol_inv(A22);  X21 = -X22 (L21 X11).
-// W is scratch of (n/2)^2 doubles at least.  Leaves are 64 x 64; a node of at
-// most 128 rows runs in one launch (chol_inv_node128).
+// L21 is held in X21's block, which stays free until the last product fills it
+// (so A keeps L only in the diagonal blocks of the leaves and 128-row nodes;
+// A21 keeps its input).  W is scratch of (n/2)^2 doubles at least.  Leaves are
+// 64 x 64; a node of at most 128 rows runs in one launch (chol_inv_node128).
 int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned int *err,
              const RecProducts &P, int depth, hipStream_t s) {
   if (n <= 64) {
@@ -364,29 +368,30 @@ int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned in
   const int n1 = rec_split(n), n2 = n - n1;
   int rc = chol_inv(N, A, X, W, off, n1, err, P, depth + 1, s);
   if (rc) return rc;
-  double *A21 = A + (off + n1) + (size_t)off * N;
+  const double *A21 = A + (off + n1) + (size_t)off * N;
   double *A22 = A + (off + n1) + (size_t)(off + n1) * N;
   const double *X11 = X + off + (size_t)off * N;
   const double *X22 = X + (off + n1) + (size_t)(off + n1) * N;
   double *X21 = X + (off + n1) + (size_t)off * N;
+  double *L21p = X21;  // L21 until X21 is formed
   const bool crt = P.nmod > 0 && n >= P.min_n && depth < REC_MAX_DEPTH && P.level[depth];
   const bool fork = P.side && n >= P.fork_min_n && depth < REC_MAX_DEPTH && P.wside[depth] &&
                     (!crt || P.side_scratch);
   const int nm = P.nmod;
   const size_t sb = P.scratch_bytes;
-  // L21 = A21 X11^T (into W, then over A21)
+  // L21 = A21 X11^T (into X21's block)
   if (crt)
-    HC(gemm_crt(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, nm, P.scratch,
+    HC(gemm_crt(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, L21p, N, GEMM_BT_LOWER, nm, P.scratch,
                 sb, s));
   else
-    HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, W, n2, GEMM_BT_LOWER, s));
-  HC(copy_block(n2, n1, W, n2, A21, N, 1.0, s));
+    HC(gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, true, 0.0, L21p, N, GEMM_BT_LOWER, s));
   CrtOperand L21;
-  if (crt) HC(crt_split(n2, n1, A21, N, true, 0, nm, P.level[depth], &L21, s));
+  if (crt) HC(crt_split(n2, n1, L21p, N, true, 0, nm, P.level[depth], &L21, s));
   // W3 = L21 X11, here or on the second stream
   double *W3 = fork ? P.wside[depth] : W;
   auto l21_x11 = [&](hipStream_t st, char *cs, size_t csb) -> hipError_t {
-    if (!crt) return gemm_f64_ex(n2, n1, n1, 1.0, A21, N, X11, N, false, 0.0, W3, n2, GEMM_B_LOWER, st);
+    if (!crt)
+      return gemm_f64_ex(n2, n1, n1, 1.0, L21p, N, X11, N, false, 0.0, W3, n2, GEMM_B_LOWER, st);
     CrtOperand Xc;  // the columns of X11 (lower) as the second operand
     const size_t bx = crt_operand_bytes(n1, n1, nm);
     hipError_t e = crt_split(n1, n1, X11, N, false, 2, nm, cs, &Xc, st);
@@ -403,7 +408,7 @@ int chol_inv(int N, double *A, double *X, double *W, int off, int n, unsigned in
   if (crt)
     HC(crt_product(-1.0, L21, L21, 1.0, A22, N, GEMM_C_LOWER, P.scratch, sb, s));
   else
-    HC(gemm_f64_ex(n2, n2, n1, -1.0, A21, N, A21, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
+    HC(gemm_f64_ex(n2, n2, n1, -1.0, L21p, N, L21p, N, true, 1.0, A22, N, GEMM_C_LOWER, s));
   rc = chol_inv(N, A, X, W, off + n1, n2, err, P, depth + 1, s);
   if (rc) return rc;
   if (fork)

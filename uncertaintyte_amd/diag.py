@@ -140,7 +140,8 @@ def inverse(A: np.ndarray, refine: bool = True) -> np.ndarray:
 
 
 def chol_inverse(A: np.ndarray):
-    """(L, X): Cholesky factor of SPD A (lower triangle) and X = L^-1."""
+    """(L, X): X = L^-1 for the Cholesky factor L of SPD A; L holds the factor in
+    its diagonal leaf blocks only (the recursion keeps L21 in X's block)."""
     n = A.shape[0]
     Af = np.asfortranarray(A, dtype=np.float64)
     L = np.zeros((n, n), order="F")
