@@ -636,42 +636,50 @@ hipError_t gj_update(int n, int k0, int kb, double *W, const double *P, const do
 // ---------------------------------------------------------------------------
 // skinny products with KC right-hand columns (n x KC, column-major, ld n)
 // ---------------------------------------------------------------------------
-// C = A^T B.  A workgroup owns 16 output rows (columns of A), a wave 4 of
-// them; rows of B are staged through LDS 64 at a time.  lower: A is lower
-// triangular, so column i only has rows r >= i.
+// C = A^T B.  A wave owns 4 output rows (columns of A, contiguous in memory)
+// and streams them U x 64 rows at a time, every load of A and of B (the rows
+// of B from L2, shared by the workgroup's waves) issued before the first is
+// used; no LDS and no barriers.  The 64 lanes' partial sums are then folded
+// by a fixed butterfly.  lower: A is lower triangular, so column i only has
+// rows r >= i.
 constexpr int TN_COLS = 16;
 
 template <int KC>
 __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restrict__ A, int lda,
                                                    const double *__restrict__ B,
                                                    double *__restrict__ C, int lower) {
-  __shared__ double Bs[64][KC + 1];
+  constexpr int U = KC > 8 ? 2 : 4;  // 64-row groups in flight
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int i0 = blockIdx.x * TN_COLS;
-  const int ibase = i0 + 4 * wave;
+  const int ibase = blockIdx.x * TN_COLS + 4 * wave;
+  if (ibase >= n) return;
   double acc[4][KC];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
     for (int j = 0; j < KC; ++j) acc[c][j] = 0.0;
-  const int rstart = lower ? (i0 & ~63) : 0;
-  for (int r0 = rstart; r0 < n; r0 += 64) {
-    __syncthreads();
-    for (int x = tid; x < 64 * KC; x += 256) {
-      const int rr = x & 63, j = x >> 6;
-      Bs[rr][j] = (r0 + rr < n) ? B[(size_t)(r0 + rr) + (size_t)j * n] : 0.0;
+  const double *col[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) col[c] = A + (size_t)min(ibase + c, n - 1) * lda;
+  const int rstart = lower ? (ibase & ~63) : 0;
+  for (int r0 = rstart; r0 < n; r0 += 64 * U) {
+    double a[U][4], b[U][KC];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = min(r0 + 64 * u + lane, n - 1);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a[u][c] = col[c][r];
+#pragma unroll
+      for (int j = 0; j < KC; ++j) b[u][j] = B[(size_t)r + (size_t)j * n];
     }
-    __syncthreads();
-    const int r = r0 + lane;
-    if (r < n) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + 64 * u + lane;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int i = ibase + c;
-        if (i < n && (!lower || r >= i)) {
-          const double a = A[(size_t)r + (size_t)i * lda];
+        const double av = (r < n && i < n && (!lower || r >= i)) ? a[u][c] : 0.0;
 #pragma unroll
-          for (int j = 0; j < KC; ++j) acc[c][j] = fma(a, Bs[lane][j], acc[c][j]);
-        }
+        for (int j = 0; j < KC; ++j) acc[c][j] = fma(av, b[u][j], acc[c][j]);
       }
     }
   }
@@ -695,17 +703,19 @@ __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restri
 }
 
 // C = A B, split over the columns of A into NN_SPLIT partial sums (reduced by
-// k_skinny_reduce).  Thread = output row; B's rows for 64 columns of A at a
-// time are staged through LDS.  lower: row r only has columns c <= r.
+// k_skinny_reduce).  Thread = output row; the column loop is unrolled by 8 so
+// that 8 loads of A (coalesced over the rows) are in flight, and B's values,
+// the same for the whole workgroup, come through the scalar cache.  lower: row
+// r only has columns c <= r.
 constexpr int NN_SPLIT = 16;
 
 template <int KC>
 __global__ __launch_bounds__(256) void k_skinny_nn(int n, const double *__restrict__ A, int lda,
                                                    const double *__restrict__ B,
                                                    double *__restrict__ part, int lower) {
-  __shared__ double Bs[64][KC];
+  constexpr int UC = 8;
   const int tid = threadIdx.x;
-  const int r0 = blockIdx.x * 256, r = r0 + tid;
+  const int r0 = blockIdx.x * 256, r = r0 + tid, rc = min(r, n - 1);
   const int per = (n + NN_SPLIT - 1) / NN_SPLIT;
   const int cb = blockIdx.y * per;
   int ce = min(n, cb + per);
@@ -713,21 +723,16 @@ __global__ __launch_bounds__(256) void k_skinny_nn(int n, const double *__restri
   double acc[KC];
 #pragma unroll
   for (int j = 0; j < KC; ++j) acc[j] = 0.0;
-  for (int c0 = cb; c0 < ce; c0 += 64) {
-    const int cnt = min(64, ce - c0);
-    __syncthreads();
-    for (int x = tid; x < 64 * KC; x += 256) {
-      const int cc = x & 63, j = x >> 6;
-      Bs[cc][j] = (cc < cnt) ? B[(size_t)(c0 + cc) + (size_t)j * n] : 0.0;
-    }
-    __syncthreads();
-    if (r < n) {
-      for (int cc = 0; cc < cnt; ++cc) {
-        const int c = c0 + cc;
-        const double a = (!lower || c <= r) ? A[(size_t)r + (size_t)c * lda] : 0.0;
+  for (int c0 = cb; c0 < ce; c0 += UC) {
+    double a[UC];
 #pragma unroll
-        for (int j = 0; j < KC; ++j) acc[j] = fma(a, Bs[cc][j], acc[j]);
-      }
+    for (int u = 0; u < UC; ++u) a[u] = A[(size_t)rc + (size_t)min(c0 + u, ce - 1) * lda];
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {  // branch-free, so that the loads above stay ahead
+      const int c = c0 + u, cc = min(c, ce - 1);
+      const double av = (c < ce && (!lower || c <= r)) ? a[u] : 0.0;
+#pragma unroll
+      for (int j = 0; j < KC; ++j) acc[j] = fma(av, B[(size_t)cc + (size_t)j * n], acc[j]);
     }
   }
   if (r < n) {
@@ -973,14 +978,30 @@ hipError_t fill_hash(size_t cnt, double *V, unsigned long long seed, hipStream_t
 
 // out = A + V V^T (n x n; V n x 7 column-major); block (x, c) covers 256 rows
 // of column c
-__global__ void k_rank_copy(int n, const double *A, const double *V, double *out) {
-  const int c = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  double s = 0.0;
+// out = A + V V^T on and below the diagonal (the Cholesky recursion reads
+// nothing above it): a thread per row r over RC_COLS columns c <= r, V's row r
+// in registers and V's rows c, the same for the whole workgroup, through the
+// scalar cache.
+constexpr int RC_COLS = 8;
+__global__ __launch_bounds__(256) void k_rank_copy(int n, const double *__restrict__ A,
+                                                   const double *__restrict__ V,
+                                                   double *__restrict__ out) {
+  const int c0 = blockIdx.y * RC_COLS, r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n || blockIdx.x * 256 + 255 < c0) return;
+  double vr[7];
 #pragma unroll
-  for (int q = 0; q < 7; ++q) s = fma(V[r + (size_t)q * n], V[c + (size_t)q * n], s);
-  const size_t x = (size_t)r + (size_t)c * n;
-  out[x] = A[x] + s;
+  for (int q = 0; q < 7; ++q) vr[q] = V[r + (size_t)q * n];
+#pragma unroll
+  for (int cc = 0; cc < RC_COLS; ++cc) {
+    const int c = c0 + cc;
+    if (c >= n) break;  // uniform
+    if (c > r) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) s = fma(vr[q], V[c + (size_t)q * n], s);
+    const size_t x = (size_t)r + (size_t)c * n;
+    out[x] = A[x] + s;
+  }
 }
 
 // max_i X_ii^2 in one workgroup: a strided pass, then a fixed tree in LDS
@@ -1007,7 +1028,8 @@ hipError_t diag_max_sq(int n, const double *X, int ldx, double *out, hipStream_t
 
 hipError_t rank_copy(int n, int k, const double *A, const double *V, double *out, hipStream_t st) {
   if (k != 7 || n > 65535) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_rank_copy, dim3(cdiv(n, 256), n), dim3(256), 0, st, n, A, V, out);
+  hipLaunchKernelGGL(k_rank_copy, dim3(cdiv(n, 256), cdiv(n, RC_COLS)), dim3(256), 0, st, n, A, V,
+                     out);
   return hipGetLastError();
 }
 
