@@ -703,17 +703,19 @@ __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restri
 }
 
 // C = A B, split over the columns of A into NN_SPLIT partial sums (reduced by
-// k_skinny_reduce).  Thread = output row; the column loop is unrolled by 8 so
-// that 8 loads of A (coalesced over the rows) are in flight, and B's values,
-// the same for the whole workgroup, come through the scalar cache.  lower: row
-// r only has columns c <= r.
+// k_skinny_reduce).  Thread = output row; B's rows for 256 columns of A at a
+// time are staged through LDS (two barriers per 256 columns), and the column
+// loop is unrolled by 8 so that 8 loads of A (coalesced over the rows) are in
+// flight.  lower: row r only has columns c <= r.
 constexpr int NN_SPLIT = 16;
+constexpr int NN_CHUNK = 256;
 
 template <int KC>
 __global__ __launch_bounds__(256) void k_skinny_nn(int n, const double *__restrict__ A, int lda,
                                                    const double *__restrict__ B,
                                                    double *__restrict__ part, int lower) {
   constexpr int UC = 8;
+  __shared__ double Bs[NN_CHUNK][KC];
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x * 256, r = r0 + tid, rc = min(r, n - 1);
   const int per = (n + NN_SPLIT - 1) / NN_SPLIT;
@@ -723,16 +725,26 @@ __global__ __launch_bounds__(256) void k_skinny_nn(int n, const double *__restri
   double acc[KC];
 #pragma unroll
   for (int j = 0; j < KC; ++j) acc[j] = 0.0;
-  for (int c0 = cb; c0 < ce; c0 += UC) {
-    double a[UC];
+  for (int k0 = cb; k0 < ce; k0 += NN_CHUNK) {
+    const int cnt = min(NN_CHUNK, ce - k0);
+    __syncthreads();
+    for (int x = tid; x < NN_CHUNK * KC; x += 256) {
+      const int cc = x % NN_CHUNK, j = x / NN_CHUNK;
+      Bs[cc][j] = cc < cnt ? B[(size_t)(k0 + cc) + (size_t)j * n] : 0.0;
+    }
+    __syncthreads();
+    for (int u0 = 0; u0 < cnt; u0 += UC) {
+      double a[UC];
 #pragma unroll
-    for (int u = 0; u < UC; ++u) a[u] = A[(size_t)rc + (size_t)min(c0 + u, ce - 1) * lda];
+      for (int u = 0; u < UC; ++u)
+        a[u] = A[(size_t)rc + (size_t)min(k0 + u0 + u, ce - 1) * lda];
 #pragma unroll
-    for (int u = 0; u < UC; ++u) {  // branch-free, so that the loads above stay ahead
-      const int c = c0 + u, cc = min(c, ce - 1);
-      const double av = (c < ce && (!lower || c <= r)) ? a[u] : 0.0;
+      for (int u = 0; u < UC; ++u) {  // branch-free, so that the loads above stay ahead
+        const int c = k0 + u0 + u;
+        const double av = (c < ce && (!lower || c <= r)) ? a[u] : 0.0;
 #pragma unroll
-      for (int j = 0; j < KC; ++j) acc[j] = fma(av, B[(size_t)cc + (size_t)j * n], acc[j]);
+        for (int j = 0; j < KC; ++j) acc[j] = fma(av, Bs[min(u0 + u, NN_CHUNK - 1)][j], acc[j]);
+      }
     }
   }
   if (r < n) {
