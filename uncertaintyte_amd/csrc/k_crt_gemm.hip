@@ -523,46 +523,48 @@ __global__ __launch_bounds__(256) void k_crt_rebuild(int M, int N, int Mp, int l
     double xd = 0.0;
 #pragma unroll
     for (int k = NW - 1; k >= 0; --k) xd = xd * 4294967296.0 + (double)x[k];
+    // (the estimate of floor(x / P) is off by at most one, so 0 <= x - q P < 3 P)
     const double qd = floor(xd * cc.pinv) - 1.0;
-    if (qd > 0.0) {
-      const unsigned long long q = (unsigned long long)qd;
-      long long borrow = 0;
+    const unsigned long long q = qd > 0.0 ? (unsigned long long)qd : 0ull;
+    long long borrow = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const long long t = (long long)x[k] - (long long)(q * cc.p[k]) - borrow;
+      x[k] = (unsigned int)t;
+      borrow = -(t >> 32);
+    }
+    // at most two more subtractions of P, branch-free: the difference replaces
+    // x where it did not borrow
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      unsigned int t[NW], br = 0;
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
-        const long long t = (long long)x[k] - (long long)(q * cc.p[k]) - borrow;
-        x[k] = (unsigned int)t;
-        borrow = -(t >> 32);
+        const unsigned long long d = (unsigned long long)x[k] - cc.p[k] - br;
+        t[k] = (unsigned int)d;
+        br = (unsigned int)(d >> 63);
       }
+#pragma unroll
+      for (int k = 0; k < NW; ++k) x[k] = br ? x[k] : t[k];
     }
-    auto ge = [&](const unsigned int *y) {  // x >= y
-      for (int k = NW - 1; k >= 0; --k)
-        if (x[k] != y[k]) return x[k] > y[k];
-      return true;
-    };
-    auto sub = [&](unsigned int *a, const unsigned int *y) {  // a -= y
-      unsigned int br = 0;
+    // symmetric range: x >= P / 2 stands for x - P = -(P - x)
+    unsigned int y[NW], by = 0, bh = 0;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        const unsigned long long t = (unsigned long long)a[k] - y[k] - br;
-        a[k] = (unsigned int)t;
-        br = (unsigned int)(t >> 63);
-      }
-    };
-    while (ge(cc.p)) sub(x, cc.p);
-    // symmetric range: x >= P / 2 stands for x - P
-    const bool neg = ge(cc.ph);
-    if (neg) {
-      unsigned int y[NW];
-#pragma unroll
-      for (int k = 0; k < NW; ++k) y[k] = cc.p[k];
-      sub(y, x);
-#pragma unroll
-      for (int k = 0; k < NW; ++k) x[k] = y[k];
+    for (int k = 0; k < NW; ++k) {
+      const unsigned long long d = (unsigned long long)cc.p[k] - x[k] - by;
+      y[k] = (unsigned int)d;
+      by = (unsigned int)(d >> 63);
+      const unsigned long long e = (unsigned long long)x[k] - cc.ph[k] - bh;
+      bh = (unsigned int)(e >> 63);
     }
-    double v = 0.0;
+    const bool neg = bh == 0;
+    double vx = 0.0, vy = 0.0;
 #pragma unroll
-    for (int k = NW - 1; k >= 0; --k) v = v * 4294967296.0 + (double)x[k];
-    if (neg) v = -v;
+    for (int k = NW - 1; k >= 0; --k) {
+      vx = vx * 4294967296.0 + (double)x[k];
+      vy = vy * 4294967296.0 + (double)y[k];
+    }
+    const double v = neg ? -vy : vx;
     const int sc = ea[i] + ej;
     double out;
     if (has_d && b == bd && !shift_res) out = alpha * (ldexp(v, -sc) - 1.0);
