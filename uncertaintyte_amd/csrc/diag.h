@@ -38,7 +38,7 @@ int unc_diag_gemm_f64_sym_batch(int n, int nb, const double *A, const double *B,
                                 double *absmax_out, int reps, double *avg_ms);
 /* C = alpha A op(B) + beta C (op(B) = B^T when trans_b, B then N x K) with the
  * operand flags of the library's general product (1 A lower, 2 B lower, 4 B^T
- * lower, 8 lower C only): nmod = 0 by the FP64 MFMA product, 6..13 by the int8
+ * lower, 8 lower C only): nmod = 0 by the FP64 MFMA product, 6..20 by the int8
  * CRT product with that many moduli.  avg_ms (may be NULL): average of `reps` further calls on
  * the device-resident operands. */
 int unc_diag_gemm_ex(int M, int N, int K, double alpha, const double *A, int lda,

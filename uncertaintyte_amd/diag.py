@@ -113,7 +113,7 @@ GEMM_A_LOWER, GEMM_B_LOWER, GEMM_BT_LOWER, GEMM_C_LOWER = 1, 2, 4, 8
 def gemm_ex(A: np.ndarray, B: np.ndarray, C: Optional[np.ndarray] = None, alpha=1.0, beta=0.0,
             trans_b: bool = False, tri: int = 0, nmod: int = 0, reps: int = 0):
     """(C, avg_ms) with C = alpha A op(B) + beta C (op(B) = B.T when trans_b) under the
-    operand flags `tri` (GEMM_*): nmod = 0 takes the FP64 MFMA product, 6..13 the int8
+    operand flags `tri` (GEMM_*): nmod = 0 takes the FP64 MFMA product, 6..20 the int8
     CRT product with that many moduli; avg_ms of `reps` more calls (None if
     reps = 0)."""
     M, K = A.shape
