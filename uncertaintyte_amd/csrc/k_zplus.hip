@@ -21,6 +21,8 @@
 //
 // Skinny products with 7 or 14 right-hand columns stream X or Z once from HBM.
 #include "common.h"
+
+#include <utility>
 #include "gemm_tile.h"
 
 namespace unc {
@@ -29,16 +31,19 @@ namespace unc {
 constexpr int CL_B = 64;
 
 // ---------------------------------------------------------------------------
-// Panelled leaf: 256 threads, the block in LDS, 8-column panels.
-//   Cholesky, right-looking by panels with one panel of look-ahead: all four
-//   waves first apply panel p's rank-8 update to the next panel's 8 columns;
-//   then wave 0 factors that panel (lane = row; the panel rows stay in
-//   registers and the pivot row's values are broadcast with readlane) while
-//   waves 1-3 apply panel p's update to the rest of the trailing lower
-//   triangle, so the serial column chain of wave 0 hides the bulk update.
-//   X = L^-1 the same way: all waves update the next panel's 8 rows, then wave
-//   0 solves them for all 64 columns (lane = column) while waves 1-3 update
-//   the rows below.
+// Panelled leaf: 256 threads, the block in LDS, 8-column panels, one
+// barrier per step.  Step s:
+//   wave 0 (lane = row) applies panel s - 1, still in its registers, to panel
+//   s's 8 columns and factors them, right-looking inside the panel (each
+//   finished column updates the later ones at once, so the dependent chain of
+//   a column is one FMA, one broadcast and the reciprocal square root; the
+//   pivot row's values are broadcast with readlane);
+//   wave 1 (lane = column) does the same for row panel s - 1 of X = L^-1,
+//   one panel behind: it needs L's column panels up to s - 1 only;
+//   waves 2-3 apply panel s - 1's rank-8 update to the trailing columns beyond
+//   panel s and X's row panel s - 2's update to X's rows beyond panel s - 1.
+// Nine steps; every element receives the same updates in the same order as
+// the plain column-by-column factorisation and forward substitution.
 // The loops over panels stay rolled, so the code is small (no instruction-fetch
 // stalls); every register array is indexed by unrolled constants.
 // ---------------------------------------------------------------------------
@@ -52,111 +57,149 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Wave 0 (lane = row i): factor the 8 columns [p0, p0 + 8) of S in place,
-// their rows above p0 being zero.  True when a pivot was not positive.
-__device__ __forceinline__ bool chol_panel8(double (*S)[CL_B + 1], int p0, int lane) {
+// Wave 0 (lane = row i): factor the 8 columns [p0, p0 + 8) held in r (all
+// earlier panels applied) and store them to S; rows above the diagonal become
+// zero.  True when a pivot was not positive.
+__device__ __forceinline__ bool chol_panel8(double (*S)[CL_B + 1], int p0, int lane,
+                                            double (&r)[CP_W]) {
   const int i = lane;
   bool bad = false;
-  double r[CP_W];
-#pragma unroll
-  for (int jj = 0; jj < CP_W; ++jj) r[jj] = S[i][p0 + jj];
 #pragma unroll
   for (int jj = 0; jj < CP_W; ++jj) {
     const int j = p0 + jj;
-    double s = r[jj];
-#pragma unroll
-    for (int kk = 0; kk < jj; ++kk) s = fma(-r[kk], readlane_d(r[kk], j), s);
-    const double piv = readlane_d(s, j);
+    const double piv = readlane_d(r[jj], j);
     if (!(piv > 0.0)) bad = true;
     // 1 / L_jj from the refined reciprocal square root; it is kept in the
     // padding column for the triangular solve
     const double rd = rsqrt_normal(piv);
     const double d = piv * rd;
-    r[jj] = i > j ? s * rd : (i == j ? d : 0.0);
+    r[jj] = i > j ? r[jj] * rd : (i == j ? d : 0.0);
     if (i == j) S[j][CL_B] = rd;
+#pragma unroll
+    for (int mm = jj + 1; mm < CP_W; ++mm) r[mm] = fma(-r[jj], readlane_d(r[jj], p0 + mm), r[mm]);
   }
 #pragma unroll
   for (int jj = 0; jj < CP_W; ++jj) S[i][p0 + jj] = r[jj];
   return bad;
 }
 
-// Wave 0 (lane = column c): rows [p0, p0 + 8) of X = L^-1 from their updated
-// right-hand sides in Xs and L's diagonal block.
+// Wave 1 (lane = column c): rows [p0, p0 + 8) of X = L^-1 from their updated
+// right-hand sides in x and L's diagonal block; stored to Xs.
 __device__ __forceinline__ void tri_panel8(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int p0,
-                                           int lane) {
-  const int c = lane;
-  double x[CP_W];
-#pragma unroll
-  for (int rr = 0; rr < CP_W; ++rr) x[rr] = Xs[p0 + rr][c];
+                                           int lane, double (&x)[CP_W]) {
 #pragma unroll
   for (int rr = 0; rr < CP_W; ++rr) {
-    double s = x[rr];
+    double v = x[rr];
 #pragma unroll
-    for (int kk = 0; kk < rr; ++kk) s = fma(-S[p0 + rr][p0 + kk], x[kk], s);
-    x[rr] = s * S[p0 + rr][CL_B];
+    for (int kk = 0; kk < rr; ++kk) v = fma(-S[p0 + rr][p0 + kk], x[kk], v);
+    x[rr] = v * S[p0 + rr][CL_B];
   }
 #pragma unroll
-  for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][c] = x[rr];
+  for (int rr = 0; rr < CP_W; ++rr) Xs[p0 + rr][lane] = x[rr];
 }
 
-// Waves 1-3 (widx 0..2) of the panelled leaf: the bulk rank-8 updates on the
-// FP64 matrix cores, over 16 x 16 tiles (v_mfma_f64_16x16x4: lane l holds A's
-// row l % 16 and B's column l % 16 at k = l / 16; accumulator r is the tile's
-// row l / 16 + 4 r, column l % 16), the tile's old values as the accumulator
-// and only the elements of the update's region written back.
-//   Cholesky: S[i][k] -= sum_jj S[i][p0 + jj] S[k][p0 + jj] for q0 + 8 <= k <= i
-//   (panel p0 = q0 - 8 on the trailing triangle beyond the panel being factored);
-__device__ __forceinline__ void chol_bulk_mfma(double (*S)[CL_B + 1], int p0, int q0, int widx,
-                                               int lane) {
-  const int st = q0 + CP_W;
-  if (st >= CL_B) return;
-  const int t0 = st >> 4, nt = 4 - t0, ntiles = nt * (nt + 1) / 2;
-  const int kq = lane >> 4, lc = lane & 15;
-  for (int t = widx; t < ntiles; t += 3) {
-    int ti = t0, rem = t;  // lower tiles row by row from (t0, t0)
-    while (rem > ti - t0) {
-      rem -= ti - t0 + 1;
-      ++ti;
-    }
-    const int r0 = 16 * ti, bc = 16 * (t0 + rem) + lc;
-    double4_t acc;
+// Waves 2-3 (W = 0, 1) of the panelled leaf at step SS: the bulk rank-8 updates
+// on the FP64 matrix cores, over 16 x 16 tiles (v_mfma_f64_16x16x4: lane l holds
+// A's row l % 16 and B's column l % 16 at k = l / 16; accumulator r is the
+// tile's row l / 16 + 4 r, column l % 16), the tile's old values as the
+// accumulator and only the elements of the update's region written back:
+//   Cholesky (SS < 8, panel p0 = 8 (SS - 1)): S[i][k] -= sum_jj S[i][p0 + jj]
+//   S[k][p0 + jj] for p0 + 16 <= k <= i (the trailing triangle beyond panel SS);
+//   inverse (SS >= 2, X's row panel xp = 8 (SS - 2)): Xs[r][c] -= sum_kk
+//   S[r][xp + kk] Xs[xp + kk][c] for rows r >= xp + 16 (only columns c < xp + 8
+//   change: X's rows of panel xp are zero beyond xp + 7).
+// The tiles of both form one list dealt alternately to the two waves.  The
+// list is fixed by SS, so each (SS, W) is its own code: tile positions are
+// constants and a wave loads every operand of its tiles before its first MFMA.
+struct BulkPlan {
+  int nc, ni, ct0, cst, xt0, xnj, xst, p0, xp;
+  constexpr BulkPlan(int ss)
+      : nc(0), ni(0), ct0(0), cst(8 * ss + 8), xt0(0), xnj(0), xst(8 * ss), p0(8 * (ss - 1)),
+        xp(8 * (ss - 2)) {
+    ct0 = cst >> 4;
+    const int cnt = (ss < CL_B / CP_W && cst < CL_B) ? 4 - ct0 : 0;
+    nc = cnt * (cnt + 1) / 2;
+    xt0 = xst >> 4;
+    xnj = (xp >= 0 && xst < CL_B) ? ((xst - CP_W - 1) >> 4) + 1 : 0;
+    ni = (4 - xt0) * xnj;
+  }
+  // tile t of the list: Cholesky tiles (lower, row by row from (ct0, ct0))
+  // first, then the inverse's (rows from xt0, columns 0 .. xnj - 1)
+  constexpr int row(int t) const {
+    if (t >= nc) return 16 * (xt0 + (t - nc) / xnj);
+    int ti = ct0, rem = t;
+    while (rem > ti - ct0) rem -= ti++ - ct0 + 1;
+    return 16 * ti;
+  }
+  constexpr int col(int t) const {
+    if (t >= nc) return 16 * ((t - nc) % xnj);
+    int ti = ct0, rem = t;
+    while (rem > ti - ct0) rem -= ti++ - ct0 + 1;
+    return 16 * (ct0 + rem);
+  }
+};
+
+template <int SS, int W, int U>
+struct BulkTile {
+  static constexpr BulkPlan P{SS};
+  static constexpr int t = W + 2 * U;
+  static constexpr bool chol = t < P.nc;
+  static constexpr int r0 = P.row(t), c0 = P.col(t), k0 = chol ? P.p0 : P.xp;
+};
+
+template <int SS, int W, int U>
+__device__ __forceinline__ void bulk_load(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int kq,
+                                          int lc, double4_t &acc, double (&av)[2],
+                                          double (&bv)[2]) {
+  using T = BulkTile<SS, W, U>;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] = S[r0 + kq + 4 * r][bc];
+  for (int r = 0; r < 4; ++r)
+    acc[r] = T::chol ? S[T::r0 + kq + 4 * r][T::c0 + lc] : Xs[T::r0 + kq + 4 * r][T::c0 + lc];
 #pragma unroll
-    for (int ks = 0; ks < CP_W; ks += 4)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-S[r0 + lc][p0 + ks + kq], S[bc][p0 + ks + kq],
-                                                 acc, 0, 0, 0);
+  for (int h = 0; h < 2; ++h) {
+    const int k = T::k0 + 4 * h + kq;
+    av[h] = -S[T::r0 + lc][k];
+    bv[h] = T::chol ? S[T::c0 + lc][k] : Xs[k][T::c0 + lc];
+  }
+}
+
+template <int SS, int W, int U>
+__device__ __forceinline__ void bulk_store(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int kq,
+                                           int lc, const double4_t &acc) {
+  using T = BulkTile<SS, W, U>;
+  const int c = T::c0 + lc;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = r0 + kq + 4 * r;
-      if (bc >= st && bc <= i) S[i][bc] = acc[r];
+  for (int r = 0; r < 4; ++r) {
+    const int i = T::r0 + kq + 4 * r;
+    if (T::chol) {
+      if (c >= T::P.cst && c <= i) S[i][c] = acc[r];
+    } else if (i >= T::P.xst) {
+      Xs[i][c] = acc[r];
     }
   }
 }
 
-//   inverse: Xs[r][c] -= sum_kk S[r][p0 + kk] Xs[p0 + kk][c] for rows r >= q0 + 8
-//   (only columns c < q0 change: X's rows of panel p0 are zero beyond p0 + 7).
-__device__ __forceinline__ void inv_bulk_mfma(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1],
-                                              int p0, int q0, int widx, int lane) {
-  const int st = q0 + CP_W;
-  if (st >= CL_B || q0 <= 0) return;
-  const int ti0 = st >> 4, ntj = ((q0 - 1) >> 4) + 1, ntiles = (4 - ti0) * ntj;
+template <int SS, int W, int... U>
+__device__ __forceinline__ void bulk_tiles(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int lane,
+                                           std::integer_sequence<int, U...>) {
   const int kq = lane >> 4, lc = lane & 15;
-  for (int t = widx; t < ntiles; t += 3) {
-    const int r0 = 16 * (ti0 + t / ntj), bc = 16 * (t % ntj) + lc;
-    double4_t acc;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] = Xs[r0 + kq + 4 * r][bc];
-#pragma unroll
-    for (int ks = 0; ks < CP_W; ks += 4)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-S[r0 + lc][p0 + ks + kq], Xs[p0 + ks + kq][bc],
-                                                 acc, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = r0 + kq + 4 * r;
-      if (i >= st) Xs[i][bc] = acc[r];
-    }
-  }
+  double4_t acc[sizeof...(U) + 1];
+  double av[sizeof...(U) + 1][2], bv[sizeof...(U) + 1][2];
+  (bulk_load<SS, W, U>(S, Xs, kq, lc, acc[U], av[U], bv[U]), ...);
+  ((acc[U] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[U][0], bv[U][0], acc[U], 0, 0, 0)), ...);
+  ((acc[U] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[U][1], bv[U][1], acc[U], 0, 0, 0)), ...);
+  (bulk_store<SS, W, U>(S, Xs, kq, lc, acc[U]), ...);
+}
+
+template <int SS>
+__device__ __forceinline__ void bulk_step(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1], int widx,
+                                          int lane) {
+  constexpr BulkPlan P{SS};
+  constexpr int NT = P.nc + P.ni;
+  if (widx == 0)
+    bulk_tiles<SS, 0>(S, Xs, lane, std::make_integer_sequence<int, (NT + 1) / 2>{});
+  else
+    bulk_tiles<SS, 1>(S, Xs, lane, std::make_integer_sequence<int, NT / 2>{});
 }
 
 // S: the SPD block (lower triangle valid, upper triangle zero) -> L with zeros
@@ -165,61 +208,61 @@ __device__ __forceinline__ void inv_bulk_mfma(double (*S)[CL_B + 1], double (*Xs
 // workgroup the others pass only the barriers.
 __device__ __forceinline__ bool chol_inv_lds(double (*S)[CL_B + 1], double (*Xs)[CL_B + 1],
                                              bool act = true) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // uniform: scalar tile logic
+  constexpr int NP = CL_B / CP_W;  // panels
   bool bad = false;
-  // Cholesky; step q0 factors panel q0 after applying panel p0 = q0 - 8
-  for (int q0 = 0; q0 < CL_B; q0 += CP_W) {
-    const int p0 = q0 - CP_W, i = q0 + lane;
-    if (p0 >= 0) {
-      // panel q0's columns k in [q0, q0 + 8), k <= i: wave w takes q0 + w, q0 + w + 4
-      if (act && i < CL_B) {
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const int k = q0 + wave + 4 * c;
-          if (k <= i) {
-            double acc = S[i][k];
-#pragma unroll
-            for (int jj = 0; jj < CP_W; ++jj) acc = fma(-S[i][p0 + jj], S[k][p0 + jj], acc);
-            S[i][k] = acc;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  double lp[CP_W] = {};  // wave 0: L's panel s - 1 in this lane's row
+  double xr[CP_W] = {};  // wave 1: X's row panel s - 2 in this lane's column
+  for (int s = 0; s <= NP; ++s) {
+    const int q0 = s * CP_W, p0 = q0 - CP_W;  // Cholesky: panel and its predecessor
+    const int xq = p0, xp = p0 - CP_W;        // X: row panel and its predecessor
     if (wave == 0) {
-      bad = chol_panel8(S, q0, lane) || bad;
+      if (s < NP) {
+        double r[CP_W];
+#pragma unroll
+        for (int jj = 0; jj < CP_W; ++jj) r[jj] = S[lane][q0 + jj];
+        if (s >= 1) {
+#pragma unroll
+          for (int jj = 0; jj < CP_W; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < CP_W; ++kk) r[jj] = fma(-lp[kk], S[q0 + jj][p0 + kk], r[jj]);
+        }
+        bad = chol_panel8(S, q0, lane, r) || bad;
+#pragma unroll
+        for (int jj = 0; jj < CP_W; ++jj) lp[jj] = r[jj];
+      }
     } else if (act) {
-      if (p0 < 0) {  // first step: Xs = I meanwhile
+      if (s == 0) {  // Xs = I meanwhile
         for (int x = t - 64; x < CL_B * CL_B; x += CP_THREADS - 64) {
           const int r = x >> 6, c = x & 63;
           Xs[r][c] = r == c ? 1.0 : 0.0;
         }
-      } else {  // panel p0's update of columns k in [q0 + 8, i]
-        chol_bulk_mfma(S, p0, q0, wave - 1, lane);
-      }
-    }
-    __syncthreads();
-  }
-  // X = L^-1; step q0 solves panel q0's rows after applying panel p0 = q0 - 8
-  for (int q0 = 0; q0 < CL_B; q0 += CP_W) {
-    const int p0 = q0 - CP_W, c = lane;
-    if (p0 >= 0) {
-      if (act) {  // panel q0's rows r in [q0, q0 + 8): wave w takes q0 + w, q0 + w + 4
+      } else if (wave == 1) {
+        double x[CP_W];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = q0 + wave + 4 * h;
-          double acc = Xs[r][c];
+        for (int rr = 0; rr < CP_W; ++rr) x[rr] = Xs[xq + rr][lane];
+        if (xp >= 0) {
 #pragma unroll
-          for (int kk = 0; kk < CP_W; ++kk) acc = fma(-S[r][p0 + kk], Xs[p0 + kk][c], acc);
-          Xs[r][c] = acc;
+          for (int rr = 0; rr < CP_W; ++rr)
+#pragma unroll
+            for (int kk = 0; kk < CP_W; ++kk) x[rr] = fma(-S[xq + rr][xp + kk], xr[kk], x[rr]);
+        }
+        tri_panel8(S, Xs, xq, lane, x);
+#pragma unroll
+        for (int rr = 0; rr < CP_W; ++rr) xr[rr] = x[rr];
+      } else {
+        switch (s) {
+          case 1: bulk_step<1>(S, Xs, wave - 2, lane); break;
+          case 2: bulk_step<2>(S, Xs, wave - 2, lane); break;
+          case 3: bulk_step<3>(S, Xs, wave - 2, lane); break;
+          case 4: bulk_step<4>(S, Xs, wave - 2, lane); break;
+          case 5: bulk_step<5>(S, Xs, wave - 2, lane); break;
+          case 6: bulk_step<6>(S, Xs, wave - 2, lane); break;
+          case 7: bulk_step<7>(S, Xs, wave - 2, lane); break;
+          default: break;  // step 8: nothing left beyond the last panels
         }
       }
-      __syncthreads();
-    }
-    if (wave == 0) {
-      tri_panel8(S, Xs, q0, lane);
-    } else if (act && p0 >= 0) {  // panel p0's update of rows r >= q0 + 8
-      inv_bulk_mfma(S, Xs, p0, q0, wave - 1, lane);
     }
     __syncthreads();
   }
