@@ -282,7 +282,7 @@ void unc_host_free(void *ptr);
  * No counterpart in the reference (which handles one problem per call).
  * unc_server_submit queues a request and returns a ticket at once; the host
  * buffers must stay valid until unc_server_wait returns for that ticket.  Slots
- * (each a context with its own HIP stream) run admitted requests concurrently; a
+ * (each a context with its own HIP streams) run admitted requests concurrently; a
  * request is admitted when a slot is idle and the device bytes all slots then
  * hold stay within hbm_budget (see unc_workspace_bytes).  FIFO, with a later
  * request allowed to pass a head that does not fit yet for at most bypass_ms.
@@ -300,7 +300,9 @@ typedef struct unc_server_stats {
   size_t held_bytes, peak_held_bytes;
   int max_concurrent;
 } unc_server_stats;
-/* defaults: device 0, 4 slots, 64 GiB, 50 ms */
+/* defaults: device 0, 2 slots, 64 GiB, 50 ms.  With more than 2 slots each slot
+ * keeps the Z+ recursion on its own single stream (no second-stream fork), so
+ * that the slots' streams stay within the device's four hardware queues. */
 void unc_server_default_config(unc_server_config *cfg);
 int unc_server_create(const unc_server_config *cfg, unc_server **out);
 /* UNC_ERR_OOM when the request alone exceeds hbm_budget */

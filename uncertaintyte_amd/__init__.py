@@ -515,7 +515,7 @@ class Server:
     returns that request's Result plus its latency and queueing time (ms).
     The Jacobian arrays are kept referenced until the ticket is waited for."""
 
-    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 64 << 30,
+    def __init__(self, device: int = 0, slots: int = 2, hbm_budget: int = 64 << 30,
                  bypass_ms: float = 50.0):
         L = lib()
         cfg = ServerConfig()

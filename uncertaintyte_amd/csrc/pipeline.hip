@@ -29,9 +29,12 @@ struct unc_context {
   hipEvent_t stage_ev[16];
   hipEvent_t gemm_ev[2 * 24];
   // the Z^+ recursion's second stream (default priority) and its fork / join
-  // events, one pair per recursion depth
+  // events, one pair per recursion depth; fork = false keeps the recursion on
+  // the context's own stream (set by a request server whose slots would
+  // otherwise drive more streams than the device has hardware queues)
   hipStream_t side = nullptr;
   hipEvent_t fork_ev[40], join_ev[40];
+  bool fork = true;
   std::mutex mu;
 };
 
@@ -726,7 +729,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
     double *Xd = (double *)ptr(B_XD);
     RecProducts P;
     if (z.mixed) zplus_crt_layout(N, z.moduli, (char *)ptr(B_CRT), &P);
-    if (rec_fork_layout(N, dD, &P)) {
+    if (ctx->fork && rec_fork_layout(N, dD, &P)) {
       P.side = ctx->side;
       P.fork = ctx->fork_ev;
       P.join = ctx->join_ev;
@@ -991,6 +994,12 @@ unc_context *default_context(int device, int *rc) {
 size_t unc_ctx_held_bytes(unc_context *ctx) { return ctx ? ctx->ws_bytes + ctx->io_bytes : 0; }
 size_t unc_ctx_ws_bytes(unc_context *ctx) { return ctx ? ctx->ws_bytes : 0; }
 size_t unc_ctx_io_bytes(unc_context *ctx) { return ctx ? ctx->io_bytes : 0; }
+
+void unc_ctx_set_fork(unc_context *ctx, bool on) {
+  if (!ctx) return;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->fork = on;
+}
 
 void unc_ctx_release(unc_context *ctx) {
   if (!ctx) return;

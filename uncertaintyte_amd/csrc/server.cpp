@@ -148,13 +148,15 @@ struct unc_server {
   }
 };
 
+constexpr int SERVER_FORK_MAX_SLOTS = 2;  // two streams per slot, four hardware queues
+
 extern "C" {
 
 void unc_server_default_config(unc_server_config *c) {
   if (!c) return;
   std::memset(c, 0, sizeof(*c));
   c->device = 0;
-  c->slots = 4;
+  c->slots = 2;
   c->hbm_budget = (size_t)64 << 30;
   c->bypass_ms = 50.0;
 }
@@ -173,6 +175,11 @@ int unc_server_create(const unc_server_config *cfg, unc_server **out) {
       delete s;
       return rc;
     }
+    // each slot's recursion forks onto a second stream only while the slots'
+    // streams do not outnumber the device's four hardware queues (HIP's
+    // default): shared queues serialise unrelated streams behind the
+    // cross-stream waits of the fork
+    if (cfg->slots > SERVER_FORK_MAX_SLOTS) unc_ctx_set_fork(s->slots[i].ctx, false);
   }
   for (int i = 0; i < cfg->slots; ++i) s->slots[i].th = std::thread(&unc_server::worker, s, i);
   *out = s;
