@@ -97,6 +97,8 @@ hipError_t launch_cam_output(const Structure &s, const Blocks &b, const double *
 hipError_t launch_extract_diag(const Structure &s, const double *X, double *Xd, hipStream_t st);
 hipError_t launch_point_output(const Structure &s, const Blocks &b, const double *Xd,
                                double *pt_out, int npts_out, hipStream_t st);
+// *flag |= 1 when any of a[0 .. n) is not finite (flag cleared by the caller)
+hipError_t flag_nonfinite(size_t n, const double *a, unsigned int *flag, hipStream_t st);
 
 // dense
 hipError_t gemm_f64(int M, int N, int K, double alpha, const double *A, int lda, const double *B,

@@ -14,6 +14,8 @@
 // ~300 FMA per observation and no pseudo-inverse.  Same fma order as the
 // oracle.  Output slot: compact id mapped back past the partially sorted
 // fixed triple (compute.cpp:443-447); fixed points keep the zero written first.
+#include <algorithm>
+
 #include "common.h"
 
 namespace unc {
@@ -157,6 +159,23 @@ hipError_t launch_point_output(const Structure &s, const Blocks &b, const double
     case 9: launch_point_cov_cp<9>(s, b, Xd, pt_out, npts_out, st); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// Output check of the host API: one pass over the packed outputs on the
+// device instead of a scan on the host after the copy.
+__global__ __launch_bounds__(256) void k_flag_nonfinite(size_t n, const double *__restrict__ a,
+                                                        unsigned int *flag) {
+  bool bad = false;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    bad |= !isfinite(a[i]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+hipError_t flag_nonfinite(size_t n, const double *a, unsigned int *flag, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int grid = (int)std::min<size_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_flag_nonfinite, dim3(grid), dim3(256), 0, st, n, a, flag);
   return hipGetLastError();
 }
 

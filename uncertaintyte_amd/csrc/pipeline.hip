@@ -9,9 +9,11 @@
 // trips are the error-word reads after decode and assembly and the final
 // read of the Taylor control block.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -35,6 +37,11 @@ struct unc_context {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev[40], join_ev[40];
   bool fork = true;
+  // page-locked staging of outputs bound for pageable host memory, and the
+  // events that mark each staged chunk's arrival
+  char *stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t copy_ev[64];
   std::mutex mu;
 };
 
@@ -966,6 +973,7 @@ int context_init(unc_context *c) {
   HC(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   for (auto &e : c->fork_ev) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto &e : c->join_ev) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto &e : c->copy_ev) HC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return scratch_setup(c->device);
 }
 
@@ -1018,10 +1026,106 @@ size_t unc_io_bytes_for(const unc_options *o, int num_rows, size_t nnz) {
   auto r256 = [](size_t b) { return ((b + 255) / 256) * 256; };
   return r256(sizeof(int) * ((size_t)num_rows + 1)) + r256(sizeof(int) * nnz) +
          r256(sizeof(double) * nnz) + r256(sizeof(double) * (size_t)o->num_cams * per) +
-         r256(sizeof(double) * (size_t)o->num_points * 6);
+         r256(sizeof(double) * (size_t)o->num_points * 6) + 256;
 }
 
 namespace {
+
+bool page_locked(const void *p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+// Device -> host copies of the outputs, ordered after the work on s.  Page-locked
+// destinations take one DMA each.  Pageable ones go through the context's
+// page-locked staging buffer in chunks: every chunk's DMA is queued with an
+// event behind it, and host threads copy chunks out as they land, so the host
+// copy overlaps the transfer (the runtime's own path for pageable memory moves
+// small pieces through a blit kernel, one host copy at a time).
+struct Seg {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+constexpr size_t STAGE_MIN_CHUNK = 256u << 10, STAGE_MAX_CHUNK = 8u << 20;
+constexpr size_t STAGE_THREAD_BYTES = 2u << 20;  // host threads only beyond this
+constexpr int STAGE_THREADS = 4;
+
+int download(unc_context *ctx, const Seg *segs, int nseg, hipStream_t s) {
+  size_t pageable = 0;
+  int npage = 0;
+  bool locked[4] = {};
+  if (nseg > 4) return UNC_ERR_BAD_INPUT;
+  for (int i = 0; i < nseg; ++i) {
+    if (segs[i].bytes == 0) continue;
+    locked[i] = page_locked(segs[i].dst);
+    if (locked[i])
+      HC(hipMemcpyAsync(segs[i].dst, segs[i].src, segs[i].bytes, hipMemcpyDeviceToHost, s));
+    else {
+      pageable += segs[i].bytes;
+      ++npage;
+    }
+  }
+  if (pageable == 0) {
+    HC(hipStreamSynchronize(s));
+    return UNC_OK;
+  }
+  if (ctx->stage_bytes < pageable) {
+    HC(hipStreamSynchronize(s));  // the old buffer may still be read by queued copies
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
+    if (hipHostMalloc((void **)&ctx->stage, pageable, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->stage = nullptr;
+      return UNC_ERR_OOM;
+    }
+    ctx->stage_bytes = pageable;
+  }
+  const int max_chunks = (int)(sizeof(ctx->copy_ev) / sizeof(ctx->copy_ev[0])) - npage;
+  size_t chunk = (pageable + max_chunks - 1) / max_chunks;
+  chunk = std::max(chunk, std::min(STAGE_MAX_CHUNK, std::max(STAGE_MIN_CHUNK, pageable / 16)));
+  chunk = (chunk + 4095) / 4096 * 4096;
+  struct Piece {
+    char *dst;
+    const char *from;
+    size_t bytes;
+  };
+  std::vector<Piece> pieces;
+  size_t off = 0;
+  for (int i = 0; i < nseg; ++i) {
+    if (segs[i].bytes == 0 || locked[i]) continue;
+    for (size_t o = 0; o < segs[i].bytes; o += chunk) {
+      const size_t len = std::min(chunk, segs[i].bytes - o);
+      HC(hipMemcpyAsync(ctx->stage + off, (const char *)segs[i].src + o, len,
+                        hipMemcpyDeviceToHost, s));
+      HC(hipEventRecord(ctx->copy_ev[pieces.size()], s));
+      pieces.push_back({(char *)segs[i].dst + o, ctx->stage + off, len});
+      off += len;
+    }
+  }
+  std::atomic<int> next{0}, failed{0};
+  auto worker = [&]() {
+    for (int k; (k = next.fetch_add(1)) < (int)pieces.size();) {
+      if (hipEventSynchronize(ctx->copy_ev[k]) != hipSuccess) {
+        failed = 1;
+        continue;
+      }
+      std::memcpy(pieces[k].dst, pieces[k].from, pieces[k].bytes);
+    }
+  };
+  std::vector<std::thread> helpers;
+  if (pageable > STAGE_THREAD_BYTES)
+    for (int t = 1; t < STAGE_THREADS && t < (int)pieces.size(); ++t) helpers.emplace_back(worker);
+  worker();
+  for (auto &h : helpers) h.join();
+  HC(hipStreamSynchronize(s));
+  return failed ? UNC_ERR_DEVICE : UNC_OK;
+}
 
 int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
                  const int *rows, const int *cols, const double *values, double *cam_unc,
@@ -1044,7 +1148,8 @@ int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int nu
   const size_t b_vals = ((sizeof(double) * nnz + 255) / 256) * 256;
   const size_t b_cam = ((sizeof(double) * cam_n + 255) / 256) * 256;
   const size_t b_pt = ((sizeof(double) * pt_n + 255) / 256) * 256;
-  rc = ensure(&ctx->io, &ctx->io_bytes, b_rows + b_cols + b_vals + b_cam + b_pt, ctx->stream);
+  rc = ensure(&ctx->io, &ctx->io_bytes, b_rows + b_cols + b_vals + b_cam + b_pt + 256,
+              ctx->stream);
   if (rc) return rc;
   int *d_rows = (int *)ctx->io;
   int *d_cols = (int *)(ctx->io + b_rows);
@@ -1059,14 +1164,18 @@ int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int nu
   rc = run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_vals, d_cam, d_pt, stats, dbg, s, true,
            nullptr);
   if (rc) return rc;
-  HC(hipMemcpyAsync(cam_unc, d_cam, sizeof(double) * cam_n, hipMemcpyDeviceToHost, s));
-  HC(hipMemcpyAsync(pt_unc, d_pt, sizeof(double) * pt_n, hipMemcpyDeviceToHost, s));
-  HC(hipStreamSynchronize(s));
-  for (size_t i = 0; i < cam_n; ++i)
-    if (!std::isfinite(cam_unc[i])) return UNC_ERR_NUMERIC;
-  for (size_t i = 0; i < pt_n; ++i)
-    if (!std::isfinite(pt_unc[i])) return UNC_ERR_NUMERIC;
-  return UNC_OK;
+  // non-finite outputs are flagged on the device (one pass over both arrays)
+  unsigned int *d_flag = (unsigned int *)(ctx->io + b_rows + b_cols + b_vals + b_cam + b_pt);
+  HC(hipMemsetAsync(d_flag, 0, sizeof(unsigned int), s));
+  HC(flag_nonfinite(cam_n, d_cam, d_flag, s));
+  HC(flag_nonfinite(pt_n, d_pt, d_flag, s));
+  unsigned int hflag = 0;
+  const Seg segs[3] = {{cam_unc, d_cam, sizeof(double) * cam_n},
+                       {pt_unc, d_pt, sizeof(double) * pt_n},
+                       {&hflag, d_flag, sizeof(unsigned int)}};
+  rc = download(ctx, segs, 3, s);
+  if (rc) return rc;
+  return hflag ? UNC_ERR_NUMERIC : UNC_OK;
 }
 
 }  // namespace
@@ -1126,6 +1235,9 @@ void unc_context_destroy(unc_context *c) {
   for (auto &e : c->join_ev)
     if (e) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamDestroy(c->side);
+  for (auto &e : c->copy_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stage) (void)hipHostFree(c->stage);
   if (c->ws) (void)hipFreeAsync(c->ws, c->stream);
   if (c->io) (void)hipFreeAsync(c->io, c->stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
