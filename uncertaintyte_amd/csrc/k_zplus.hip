@@ -378,19 +378,19 @@ __device__ __forceinline__ void store_xtx(int b, double (*Xs)[CL_B + 1], double 
 }
 
 // A recursion node of n rows, 64 < n <= 128, in one workgroup: halves of 64
-// and n2 = n - 64 rows, every block in LDS (S, Xs: the diagonal blocks and their
-// inverse factors in turn; T: A21 -> L21; U: W = L21 X11), the four products on
-// the FP64 matrix cores:
+// and n2 = n - 64 rows, the four products on the FP64 matrix cores:
 //   (L11, X11) = leaf(A11);  L21 = A21 X11^T;  W = L21 X11;
 //   (L22, X22) = leaf(A22 - L21 L21^T);  X21 = -X22 W.
+// Two 64 x 64 blocks of LDS only (S: A11 -> L11, then A21 -> L21, then
+// A22 - P -> L22, then W; Xs: X11, then X22); W and P wait in registers.  At
+// 66.5 KB the node fits on a CU beside one workgroup of the large products that
+// may run on the other stream, so it does not wait for a CU to drain.
 // The same results as the recursion's two leaves and four products (to
 // rounding), in one launch instead of seven.
 __global__ __launch_bounds__(CP_THREADS) void k_chol_inv128(int n, double *A, int lda, double *Xo,
                                                             int ldx, unsigned int *err) {
   __shared__ double S[CL_B][CL_B + 1];
   __shared__ double Xs[CL_B][CL_B + 1];
-  __shared__ double T[CL_B][CL_B + 1];
-  __shared__ double U[CL_B][CL_B + 1];
   const int n2 = n - CL_B;
   double *A21 = A + CL_B, *A22 = A + CL_B + (size_t)CL_B * lda;
   double *X21 = Xo + CL_B, *X22 = Xo + CL_B + (size_t)CL_B * ldx;
@@ -402,27 +402,28 @@ __global__ __launch_bounds__(CP_THREADS) void k_chol_inv128(int n, double *A, in
     A[(size_t)i + (size_t)k * lda] = S[i][k];
     Xo[(size_t)i + (size_t)k * ldx] = Xs[i][k];
   }
-  stage64(T, [&](int i, int k) { return i < n2 ? A21[(size_t)i + (size_t)k * lda] : 0.0; });
+  __syncthreads();  // S (L11) read by the stores above
+  stage64(S, [&](int i, int k) { return i < n2 ? A21[(size_t)i + (size_t)k * lda] : 0.0; });
   __syncthreads();
-  // L21 = A21 X11^T (X11 lower), over A21 in T
+  // L21 = A21 X11^T (X11 lower), over A21 in S
   double4_t acc[2][2];
-  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
+  mfma_quadrant64(CL_B, [&](int row, int k) { return S[row][k]; },
                   [&](int k, int col) { return Xs[col][k]; }, acc);
   __syncthreads();
-  quadrant64_for_each(acc, [&](int i, int j, double v, int) { T[i][j] = v; });
+  quadrant64_for_each(acc, [&](int i, int j, double v, int) { S[i][j] = v; });
   __syncthreads();
   for (int x = threadIdx.x; x < CL_B * CL_B; x += CP_THREADS) {
     const int i = x & 63, k = x >> 6;
-    if (i < n2) A21[(size_t)i + (size_t)k * lda] = T[i][k];
+    if (i < n2) A21[(size_t)i + (size_t)k * lda] = S[i][k];
   }
-  // W = L21 X11 into U; P = L21 L21^T subtracted from A22 staged in S
-  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
-                  [&](int k, int col) { return Xs[k][col]; }, acc);
-  quadrant64_for_each(acc, [&](int i, int j, double v, int) { U[i][j] = v; });
-  double4_t pacc[2][2];
-  mfma_quadrant64(CL_B, [&](int row, int k) { return T[row][k]; },
-                  [&](int k, int col) { return T[col][k]; }, pacc);
-  load_spd64(n2, A22, lda, S);  // S held L11, already stored
+  // W = L21 X11 and P = L21 L21^T, both kept in registers
+  double4_t wacc[2][2], pacc[2][2];
+  mfma_quadrant64(CL_B, [&](int row, int k) { return S[row][k]; },
+                  [&](int k, int col) { return Xs[k][col]; }, wacc);
+  mfma_quadrant64(CL_B, [&](int row, int k) { return S[row][k]; },
+                  [&](int k, int col) { return S[col][k]; }, pacc);
+  __syncthreads();  // L21 in S read by the products and the stores
+  load_spd64(n2, A22, lda, S);
   quadrant64_for_each(pacc, [&](int i, int j, double v, int) {
     if (i < n2 && j <= i) S[i][j] -= v;
   });
@@ -436,9 +437,12 @@ __global__ __launch_bounds__(CP_THREADS) void k_chol_inv128(int n, double *A, in
       X22[(size_t)i + (size_t)k * ldx] = Xs[i][k];
     }
   }
-  // X21 = -X22 W (X22 lower)
+  __syncthreads();  // S (L22) read by the stores above
+  quadrant64_for_each(wacc, [&](int i, int j, double v, int) { S[i][j] = v; });
+  __syncthreads();
+  // X21 = -X22 W (X22 lower), W in S
   mfma_quadrant64(CL_B, [&](int row, int k) { return Xs[row][k]; },
-                  [&](int k, int col) { return U[k][col]; }, acc);
+                  [&](int k, int col) { return S[k][col]; }, acc);
   quadrant64_for_each(acc, [&](int i, int j, double v, int) {
     if (i < n2) X21[(size_t)i + (size_t)j * ldx] = -v;
   });
