@@ -1119,8 +1119,13 @@ int download(unc_context *ctx, const Seg *segs, int nseg, hipStream_t s) {
     }
   };
   std::vector<std::thread> helpers;
-  if (pageable > STAGE_THREAD_BYTES)
-    for (int t = 1; t < STAGE_THREADS && t < (int)pieces.size(); ++t) helpers.emplace_back(worker);
+  if (pageable > STAGE_THREAD_BYTES) {
+    try {
+      for (int t = 1; t < STAGE_THREADS && t < (int)pieces.size(); ++t)
+        helpers.emplace_back(worker);
+    } catch (...) {  // no helper threads to be had: this thread copies the rest
+    }
+  }
   worker();
   for (auto &h : helpers) h.join();
   HC(hipStreamSynchronize(s));
