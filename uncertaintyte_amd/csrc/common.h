@@ -52,6 +52,9 @@ struct Structure {
   int fix[3];                       // partially sorted triple
   const int *rows, *cols;           // input CRS (device)
   const double *vals;
+  // per observation (first camera column, first point column - N) when the host
+  // has checked the CRS layout; rows and cols are then not read
+  const int *ids = nullptr;
   // outputs of decode
   int *obs_cam;                     // camera per observation, -1 when dropped
   int *obs_pt;                      // compact point per observation, -1 when dropped

@@ -19,31 +19,9 @@ namespace unc {
 // ---------------------------------------------------------------------------
 // decode: one thread per observation (two CRS rows)
 // ---------------------------------------------------------------------------
-__global__ void k_decode(Structure s) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= s.nobs) return;
-  const int cp = s.cp, rl = cp + 3, N = s.N;
-  const int r0 = 2 * o;
-  unsigned int e = 0;
-  if (s.rows[r0] != r0 * rl || s.rows[r0 + 1] != (r0 + 1) * rl || s.rows[r0 + 2] != (r0 + 2) * rl)
-    e |= ERRBIT_LAYOUT;
-  int a = -1, p = -1;
-  if (!e) {
-    const int *c0 = s.cols + (size_t)r0 * rl;
-    const int *c1 = c0 + rl;
-    int cam0 = c0[0];
-    a = cam0 / cp;
-    if (cam0 < 0 || cam0 % cp != 0 || a >= s.nc) e |= ERRBIT_RANGE;
-    int pc = c0[cp] - N;
-    p = pc / 3;
-    if (pc < 0 || pc % 3 != 0 || p >= s.np) e |= ERRBIT_RANGE;
-    if (!e) {
-      for (int j = 0; j < cp; ++j)
-        if (c0[j] != cam0 + j || c1[j] != cam0 + j) e |= ERRBIT_LAYOUT;
-      for (int j = 0; j < 3; ++j)
-        if (c0[cp + j] != N + pc + j || c1[cp + j] != N + pc + j) e |= ERRBIT_LAYOUT;
-    }
-  }
+// error bits, fixed-point drop and compact point id of observation o
+__device__ __forceinline__ void decode_finish(const Structure &s, int o, unsigned int e, int a,
+                                              int p) {
   if (e) {
     atomicOr(s.err, e);
     s.obs_cam[o] = -1;
@@ -63,6 +41,51 @@ __global__ void k_decode(Structure s) {
   atomicAdd(&s.pt_cnt[pcmp], 1);
 }
 
+// the first camera column cam0 and first point column pc (minus N) of an
+// observation must be block starts of an existing camera and point
+__device__ __forceinline__ unsigned int decode_range(const Structure &s, int cam0, int pc, int &a,
+                                                     int &p) {
+  unsigned int e = 0;
+  a = cam0 / s.cp;
+  if (cam0 < 0 || cam0 % s.cp != 0 || a >= s.nc) e |= ERRBIT_RANGE;
+  p = pc / 3;
+  if (pc < 0 || pc % 3 != 0 || p >= s.np) e |= ERRBIT_RANGE;
+  return e;
+}
+
+__global__ void k_decode(Structure s) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= s.nobs) return;
+  const int cp = s.cp, rl = cp + 3, N = s.N;
+  const int r0 = 2 * o;
+  unsigned int e = 0;
+  if (s.rows[r0] != r0 * rl || s.rows[r0 + 1] != (r0 + 1) * rl || s.rows[r0 + 2] != (r0 + 2) * rl)
+    e |= ERRBIT_LAYOUT;
+  int a = -1, p = -1;
+  if (!e) {
+    const int *c0 = s.cols + (size_t)r0 * rl;
+    const int *c1 = c0 + rl;
+    const int cam0 = c0[0], pc = c0[cp] - N;
+    e |= decode_range(s, cam0, pc, a, p);
+    if (!e) {
+      for (int j = 0; j < cp; ++j)
+        if (c0[j] != cam0 + j || c1[j] != cam0 + j) e |= ERRBIT_LAYOUT;
+      for (int j = 0; j < 3; ++j)
+        if (c0[cp + j] != N + pc + j || c1[cp + j] != N + pc + j) e |= ERRBIT_LAYOUT;
+    }
+  }
+  decode_finish(s, o, e, a, p);
+}
+
+// the same from the host-checked ids (the layout already holds)
+__global__ void k_decode_ids(Structure s) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= s.nobs) return;
+  int a = -1, p = -1;
+  const unsigned int e = decode_range(s, s.ids[2 * o], s.ids[2 * o + 1], a, p);
+  decode_finish(s, o, e, a, p);
+}
+
 __global__ void k_init_lists(Structure s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < s.npc) { s.pt_cnt[i] = 0; s.pt_cursor[i] = 0; }
@@ -72,7 +95,12 @@ __global__ void k_init_lists(Structure s) {
 hipError_t launch_decode(const Structure &s, hipStream_t st) {
   int n = s.npc > s.nc ? s.npc : s.nc;
   hipLaunchKernelGGL(k_init_lists, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, s);
-  if (s.nobs > 0) hipLaunchKernelGGL(k_decode, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
+  if (s.nobs > 0) {
+    if (s.ids)
+      hipLaunchKernelGGL(k_decode_ids, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
+    else
+      hipLaunchKernelGGL(k_decode, dim3(cdiv(s.nobs, 256)), dim3(256), 0, st, s);
+  }
   return hipGetLastError();
 }
 

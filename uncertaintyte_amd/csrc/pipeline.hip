@@ -620,7 +620,8 @@ struct Workspace {
 
 int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int *d_rows,
         const int *d_cols, const double *d_vals, double *d_cam, double *d_pt, unc_stats *st,
-        unc_debug *dbg, hipStream_t s, bool host_api_upload_timed, const Workspace *fixed_ws) {
+        unc_debug *dbg, hipStream_t s, bool host_api_upload_timed, const Workspace *fixed_ws,
+        const int *d_ids = nullptr) {
   if (o && (o->algorithm == UNC_ALG_SVD_QR_ITERATION ||
             o->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER)) {
     if (fixed_ws) return UNC_ERR_UNSUPPORTED;
@@ -646,7 +647,7 @@ int run(unc_context *ctx, const unc_options *o, int nrows, int ncols, const int 
   Structure S;
   S.cp = z.cp; S.nc = z.nc; S.np = z.np; S.npc = z.npc; S.nobs = z.nobs; S.N = z.N;
   S.fix[0] = fix[0]; S.fix[1] = fix[1]; S.fix[2] = fix[2];
-  S.rows = d_rows; S.cols = d_cols; S.vals = d_vals;
+  S.rows = d_rows; S.cols = d_cols; S.vals = d_vals; S.ids = d_ids;
   S.obs_cam = (int *)ptr(B_OBS_CAM); S.obs_pt = (int *)ptr(B_OBS_PT);
   S.cam_off = (int *)ptr(B_CAM_OFF);
   S.cam_list = (int *)ptr(B_CAM_LIST);
@@ -1132,6 +1133,49 @@ int download(unc_context *ctx, const Seg *segs, int nseg, hipStream_t s) {
   return failed ? UNC_ERR_DEVICE : UNC_OK;
 }
 
+// The CRS index arrays of the standard layout (every observation two rows of
+// cp camera columns then 3 point columns, rows equally long) reduced on the host
+// to (first camera column, first point column - n_cam) per observation, in
+// parallel chunks; false at the first row that departs from the layout (the
+// device decode then reads the full arrays and reports it).
+constexpr int IDS_THREADS = 8;
+constexpr int IDS_THREAD_OBS = 1 << 16;  // observations per thread at least
+
+bool host_ids(const int *rows, const int *cols, int nobs, int cp, int ncam_cols, int *ids) {
+  const int rl = cp + 3;
+  std::atomic<bool> ok{true};
+  auto part = [&](int o0, int o1) {
+    for (int o = o0; o < o1; ++o) {
+      if ((o & 1023) == 0 && !ok.load(std::memory_order_relaxed)) return;
+      const int r0 = 2 * o;
+      const int *c0 = cols + (size_t)r0 * rl, *c1 = c0 + rl;
+      bool good = rows[r0 + 1] == (r0 + 1) * rl && rows[r0 + 2] == (r0 + 2) * rl;
+      const int cam0 = c0[0], pc = c0[cp] - ncam_cols;
+      for (int j = 0; j < cp; ++j) good &= c0[j] == cam0 + j && c1[j] == cam0 + j;
+      for (int j = 0; j < 3; ++j)
+        good &= c0[cp + j] == ncam_cols + pc + j && c1[cp + j] == ncam_cols + pc + j;
+      if (!good) {
+        ok = false;
+        return;
+      }
+      ids[2 * o] = cam0;
+      ids[2 * o + 1] = pc;
+    }
+  };
+  const int nt = std::max(1, std::min(IDS_THREADS, nobs / IDS_THREAD_OBS));
+  const int per = (nobs + nt - 1) / nt;
+  std::vector<std::thread> helpers;
+  int t0 = 1;
+  try {
+    for (; t0 < nt; ++t0) helpers.emplace_back(part, t0 * per, std::min(nobs, (t0 + 1) * per));
+  } catch (...) {  // fewer threads: this one takes the remaining chunks
+  }
+  part(0, std::min(nobs, per));
+  for (int t = t0; t < nt; ++t) part(t * per, std::min(nobs, (t + 1) * per));
+  for (auto &h : helpers) h.join();
+  return ok.load();
+}
+
 int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int num_cols,
                  const int *rows, const int *cols, const double *values, double *cam_unc,
                  double *pt_unc, unc_stats *stats, unc_debug *dbg) {
@@ -1163,11 +1207,37 @@ int compute_host(unc_context *ctx, const unc_options *opts, int num_rows, int nu
   double *d_pt = (double *)(ctx->io + b_rows + b_cols + b_vals + b_cam);
   hipStream_t s = ctx->stream;
   HC(hipEventRecord(ctx->stage_ev[0], s));
-  HC(hipMemcpyAsync(d_rows, rows, sizeof(int) * (num_rows + 1), hipMemcpyHostToDevice, s));
-  HC(hipMemcpyAsync(d_cols, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+  // the values first: the largest transfer is under way while the host checks
+  // the index arrays' layout; where it holds, 8 bytes of ids per observation
+  // cross the link instead of the index arrays (2 (cp + 3) + 2 ints)
   HC(hipMemcpyAsync(d_vals, values, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  const int nobs = num_rows / 2;
+  const int *d_ids = nullptr;
+  const bool svd = opts->algorithm == UNC_ALG_SVD_QR_ITERATION ||
+                   opts->algorithm == UNC_ALG_SVD_DEVIDE_AND_CONQUER;
+  if (!svd && nobs > 0) {
+    const size_t ids_bytes = sizeof(int) * 2 * (size_t)nobs;
+    if (ctx->stage_bytes < ids_bytes) {
+      HC(hipStreamSynchronize(s));  // a previous call's copies may still read the old buffer
+      if (ctx->stage) (void)hipHostFree(ctx->stage);
+      ctx->stage = nullptr;
+      ctx->stage_bytes = 0;
+      if (hipHostMalloc((void **)&ctx->stage, ids_bytes, hipHostMallocDefault) == hipSuccess)
+        ctx->stage_bytes = ids_bytes;
+      else
+        (void)hipGetLastError();
+    }
+    if (ctx->stage && host_ids(rows, cols, nobs, cp, opts->num_cams * cp, (int *)ctx->stage)) {
+      HC(hipMemcpyAsync(d_cols, ctx->stage, ids_bytes, hipMemcpyHostToDevice, s));
+      d_ids = d_cols;
+    }
+  }
+  if (!d_ids) {
+    HC(hipMemcpyAsync(d_rows, rows, sizeof(int) * (num_rows + 1), hipMemcpyHostToDevice, s));
+    HC(hipMemcpyAsync(d_cols, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+  }
   rc = run(ctx, opts, num_rows, num_cols, d_rows, d_cols, d_vals, d_cam, d_pt, stats, dbg, s, true,
-           nullptr);
+           nullptr, d_ids);
   if (rc) return rc;
   // non-finite outputs are flagged on the device (one pass over both arrays)
   unsigned int *d_flag = (unsigned int *)(ctx->io + b_rows + b_cols + b_vals + b_cam + b_pt);
