@@ -342,6 +342,22 @@ def test_error_bad_layout(cube):
     assert e.value.code == u.UNC_ERR_BAD_INPUT
 
 
+def test_error_camera_out_of_range_regular_layout(cube):
+    """An observation whose two rows keep the regular layout (cp consecutive camera
+    columns, then 3 point columns) but name a camera past the last: the host-side
+    layout check passes it on as ids, and the device's range check refuses it."""
+    say("error: camera out of range, regular layout")
+    j = copy.deepcopy(cube)
+    cp, rl = j.cam_params, j.cam_params + 3
+    cols = j.cols.reshape(-1, rl).copy()
+    # the camera block of the first observation's two rows one camera past the end
+    cols[0:2, :cp] = j.num_cams * cp + np.arange(cp)
+    j.cols = cols.reshape(-1)
+    with pytest.raises(u.UncError) as e:
+        u.compute_covariances(j)
+    assert e.value.code == u.UNC_ERR_BAD_INPUT
+
+
 # ----------------------------------------------------------------- device API
 
 def test_device_api_matches_host_api(daliborka):
