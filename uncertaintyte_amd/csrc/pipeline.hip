@@ -5,9 +5,10 @@
 //   decode + fixPts -> point lists -> J^T J blocks, scales, 3x3 inverses ->
 //   Schur complement -> (Z + lambda I)^-1 (+ compensated Newton step) ->
 //   Taylor chain -> camera output -> point back-propagation.
-// Everything after the input upload runs on one stream; the only host round
-// trips are the error-word reads after decode and assembly and the final
-// read of the Taylor control block.
+// Everything after the input upload runs on one stream (the Z^+ recursion forks
+// one product per node onto a second); the one host round trip is at the end,
+// reading the error word, the kept-observation count and the Taylor control
+// block (TE path) or the conditioning bound (Z^+ path).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
