@@ -683,38 +683,40 @@ hipError_t gj_update(int n, int k0, int kb, double *W, const double *P, const do
 // ---------------------------------------------------------------------------
 // skinny products with KC right-hand columns (n x KC, column-major, ld n)
 // ---------------------------------------------------------------------------
-// C = A^T B.  A wave owns 4 output rows (columns of A, contiguous in memory)
-// and streams them U x 64 rows at a time, every load of A and of B (the rows
-// of B from L2, shared by the workgroup's waves) issued before the first is
-// used; no LDS and no barriers.  The 64 lanes' partial sums are then folded
-// by a fixed butterfly.  lower: A is lower triangular, so column i only has
-// rows r >= i.
-constexpr int TN_COLS = 16;
+// C = A^T B.  A wave owns tn_wave_cols(KC) output rows (columns of A, contiguous
+// in memory) and streams them 64 rows per group, U groups at a time, every load
+// of A and of B (the rows of B from L2, shared by the workgroup's waves) issued
+// before the first is used; no LDS and no barriers.  Eight columns per wave for
+// KC = 7 halve the B rows fetched per element of A.  The 64 lanes' partial sums
+// are then folded by a fixed butterfly.  lower: A is lower triangular, so
+// column i only has rows r >= i.
+__host__ __device__ constexpr int tn_wave_cols(int kc) { return kc > 8 ? 4 : 8; }
+__host__ __device__ constexpr int tn_cols(int kc) { return 4 * tn_wave_cols(kc); }
 
 template <int KC>
 __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restrict__ A, int lda,
                                                    const double *__restrict__ B,
                                                    double *__restrict__ C, int lower) {
-  constexpr int U = KC > 8 ? 2 : 4;  // 64-row groups in flight
+  constexpr int CW = tn_wave_cols(KC), U = 2;  // columns per wave, 64-row groups in flight
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ibase = blockIdx.x * TN_COLS + 4 * wave;
+  const int ibase = blockIdx.x * tn_cols(KC) + CW * wave;
   if (ibase >= n) return;
-  double acc[4][KC];
+  double acc[CW][KC];
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < CW; ++c)
 #pragma unroll
     for (int j = 0; j < KC; ++j) acc[c][j] = 0.0;
-  const double *col[4];
+  const double *col[CW];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) col[c] = A + (size_t)min(ibase + c, n - 1) * lda;
+  for (int c = 0; c < CW; ++c) col[c] = A + (size_t)min(ibase + c, n - 1) * lda;
   const int rstart = lower ? (ibase & ~63) : 0;
   for (int r0 = rstart; r0 < n; r0 += 64 * U) {
-    double a[U][4], b[U][KC];
+    double a[U][CW], b[U][KC];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int r = min(r0 + 64 * u + lane, n - 1);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) a[u][c] = col[c][r];
+      for (int c = 0; c < CW; ++c) a[u][c] = col[c][r];
 #pragma unroll
       for (int j = 0; j < KC; ++j) b[u][j] = B[(size_t)r + (size_t)j * n];
     }
@@ -722,7 +724,7 @@ __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restri
     for (int u = 0; u < U; ++u) {
       const int r = r0 + 64 * u + lane;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < CW; ++c) {
         const int i = ibase + c;
         const double av = (r < n && i < n && (!lower || r >= i)) ? a[u][c] : 0.0;
 #pragma unroll
@@ -731,7 +733,7 @@ __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restri
     }
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < CW; ++c)
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
       double v = acc[c][j];
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(256) void k_skinny_tn(int n, const double *__restri
     }
   if (lane == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < CW; ++c) {
       const int i = ibase + c;
       if (i < n)
 #pragma unroll
@@ -815,7 +817,7 @@ __global__ void k_skinny_reduce(int n, const double *part, double *C) {
 template <int KC>
 static hipError_t skinny_tn_k(int n, const double *A, int lda, const double *B, double *C,
                               bool lower, hipStream_t st) {
-  hipLaunchKernelGGL(k_skinny_tn<KC>, dim3(cdiv(n, TN_COLS)), dim3(256), 0, st, n, A, lda, B, C,
+  hipLaunchKernelGGL(k_skinny_tn<KC>, dim3(cdiv(n, tn_cols(KC))), dim3(256), 0, st, n, A, lda, B, C,
                      lower ? 1 : 0);
   return hipGetLastError();
 }
