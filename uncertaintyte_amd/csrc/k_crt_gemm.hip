@@ -133,40 +133,68 @@ __device__ __forceinline__ bool crt_keep(int mask, int r, int k) {
 
 // part[ch * R + r] = sum over the ch-th of CRT_NCH chunks of k of P(r, k)^2 for an
 // operand whose element (r, k) sits at P[r + k ld] (rows gathered across k);
-// 64 rows x 4 interleaved k phases per workgroup, phases added in order.
+// 64 rows x 4 interleaved k phases per workgroup, phases added in order.  Each
+// thread keeps four loads in flight (four partial sums); masked elements are
+// loaded and dropped by a select, never multiplied.
 __global__ __launch_bounds__(256) void k_crt_norm_gather(int R, int K, const double *__restrict__ P,
                                                           int ld, int mask, double *part) {
   __shared__ double s4[4][64];
   const int rr = threadIdx.x & 63, ph = threadIdx.x >> 6, r = blockIdx.x * 64 + rr;
   const int per = (K + CRT_NCH - 1) / CRT_NCH;
   const int kb = blockIdx.y * per, ke = min(K, kb + per);
-  double s = 0.0;
-  if (r < R)
-    for (int k = kb + ph; k < ke; k += 4)
-      if (crt_keep(mask, r, k)) {
-        const double v = P[(size_t)r + (size_t)k * ld];
-        s = fma(v, v, s);
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (r < R) {
+    int k = kb + ph;
+    for (; k + 12 < ke; k += 16) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = P[(size_t)r + (size_t)(k + 4 * u) * ld];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double x = crt_keep(mask, r, k + 4 * u) ? v[u] : 0.0;
+        s[u] = fma(x, x, s[u]);
       }
-  s4[ph][rr] = s;
+    }
+    for (; k < ke; k += 4) {
+      const double v = P[(size_t)r + (size_t)k * ld];
+      const double x = crt_keep(mask, r, k) ? v : 0.0;
+      s[0] = fma(x, x, s[0]);
+    }
+  }
+  s4[ph][rr] = (s[0] + s[1]) + (s[2] + s[3]);
   __syncthreads();
   if (ph == 0 && r < R)
     part[(size_t)blockIdx.y * R + r] = ((s4[0][rr] + s4[1][rr]) + s4[2][rr]) + s4[3][rr];
 }
 
 // part[r] = sum_k P(r, k)^2 for an operand whose element (r, k) sits at
-// P[k + r ld] (rows contiguous in k): one wave per row, a fixed butterfly.
+// P[k + r ld] (rows contiguous in k): one wave per row, four loads in flight
+// per lane (four partial sums), then a fixed butterfly.
 __global__ __launch_bounds__(256) void k_crt_norm_contig(int R, int K, const double *__restrict__ P,
                                                           int ld, int mask, double *part) {
   const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= R) return;
-  double s = 0.0;
-  for (int k = lane; k < K; k += 64)
-    if (crt_keep(mask, r, k)) {
-      const double v = P[(size_t)k + (size_t)r * ld];
-      s = fma(v, v, s);
+  const double *row = P + (size_t)r * ld;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int k = lane;
+  for (; k + 192 < K; k += 256) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = row[k + 64 * u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double x = crt_keep(mask, r, k + 64 * u) ? v[u] : 0.0;
+      s[u] = fma(x, x, s[u]);
     }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) part[r] = s;
+  }
+  for (; k < K; k += 64) {
+    const double v = row[k];
+    const double x = crt_keep(mask, r, k) ? v : 0.0;
+    s[0] = fma(x, x, s[0]);
+  }
+  double t = (s[0] + s[1]) + (s[2] + s[3]);
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+  if (lane == 0) part[r] = t;
 }
 
 // Symmetric residue of the integer x (|x| < 2^52, held exactly as a double;
