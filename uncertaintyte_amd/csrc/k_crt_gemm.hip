@@ -278,11 +278,16 @@ __global__ __launch_bounds__(256) void k_crt_split_gather(int R, int K, int Rp, 
   const int r = rb * 16 + (lane & 15), k0 = kb * CRT_KB + 16 * (lane >> 4);
   const int e = crt_row_exponent(r, R, part, nch, cc.q);
   if (kb == 0 && lane < 16) expo[r] = e;
+  // every load issued before the first is used: clamped into the operand,
+  // the elements outside it or masked dropped by a select
+  const int rc = min(r, R - 1);
   double x[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) x[t] = P[(size_t)rc + (size_t)min(k0 + t, K - 1) * ld];
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int k = k0 + t;
-    const double v = (r < R && k < K && crt_keep(mask, r, k)) ? P[(size_t)r + (size_t)k * ld] : 0.0;
+    const double v = (r < R && k < K && crt_keep(mask, r, k)) ? x[t] : 0.0;
     x[t] = rint(ldexp(v, e));
   }
   crt_emit<WIDE>(x, cc, planes, plane_bytes,
@@ -307,11 +312,15 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(int R, int K, int Rp, 
   const int kblocks = Kp / CRT_KB, kgroups = (kblocks + 3) / 4;
   const int rb = blockIdx.x / kgroups, kg = blockIdx.x - rb * kgroups;
   {
-    const int lr = tid >> 4, c = tid & 15, r = rb * 16 + lr;
+    const int lr = tid >> 4, c = tid & 15, r = rb * 16 + lr, rc = min(r, R - 1);
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      v[i] = P[(size_t)min(kg * 4 * CRT_KB + c + 16 * i, K - 1) + (size_t)rc * ld];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int kl = c + 16 * i, k = kg * 4 * CRT_KB + kl;
-      T[lr][kl] = (r < R && k < K && crt_keep(mask, r, k)) ? P[(size_t)k + (size_t)r * ld] : 0.0;
+      T[lr][kl] = (r < R && k < K && crt_keep(mask, r, k)) ? v[i] : 0.0;
     }
   }
   __syncthreads();
