@@ -1,6 +1,8 @@
 """GPU: the request server -- results in a concurrent mix equal the results of
 the same requests alone (bit for bit: same kernels, deterministic reductions),
 HBM-footprint admission stays within its budget, oversize requests are refused."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -191,3 +193,28 @@ def test_pinned_payloads_same_results(small_scenes):
     for r in outs:
         assert np.array_equal(r.cam, a.cam) and np.array_equal(r.pt, a.pt)
     del jp  # the pinned block is freed with its last view
+
+
+def test_page_locked_outputs_same_results():
+    """Output arrays in page-locked memory take the direct copy; pageable ones the
+    staged, chunked copy-out (beyond 2 MB with helper threads): the same bits
+    either way, also when one context serves sizes that grow its staging buffer."""
+    from uncertaintyte_amd import Stats, _ptr, lib, make_options
+    ctx = u.Context()
+    for cams, ppc, seed in ((40, 100, 401), (150, 400, 402)):
+        j = serve.request_scene(cams, ppc, seed)
+        ref = u.compute_covariances(j, precision=u.PREC_MIXED, ctx=ctx.handle)
+        o = make_options(j, u.PREC_MIXED, True, 0, None, None)
+        cp = j.cam_params
+        cam = u.pinned_empty(j.num_cams * (cp * (cp + 1) // 2), np.float64)
+        pt = u.pinned_empty(j.num_points * 6, np.float64)
+        cam[:] = np.nan
+        pt[:] = np.nan
+        rows = np.ascontiguousarray(j.rows, np.int32)
+        cols = np.ascontiguousarray(j.cols, np.int32)
+        vals = np.ascontiguousarray(j.vals, np.float64)
+        st = Stats()
+        rc = lib().unc_compute(ctx.handle, ctypes.byref(o), j.num_rows, j.num_cols, _ptr(rows),
+                               _ptr(cols), _ptr(vals), _ptr(cam), _ptr(pt), ctypes.byref(st))
+        assert rc == 0
+        assert np.array_equal(cam, ref.cam.ravel()) and np.array_equal(pt, ref.pt.ravel())
