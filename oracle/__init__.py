@@ -1,7 +1,7 @@
 """ctypes bindings for the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by `__graft_entry__.smoke()` and
-by the cpu_baseline leg of bench.py, as the checker.  The product package
+by the cpu_baseline leg of bench.py, to compare results against.  The product package
 (uncertaintyte_amd) never imports this module.
 """
 from __future__ import annotations

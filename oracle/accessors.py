@@ -1,8 +1,8 @@
 """CPU restatement of the cov::Uncertainty accessors
 (src/uncertaintyTE/uncertainty.cpp:48-185).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
-eigen kernels (unc_eig_cameras / unc_eig_points); the product package never
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference results for the
+device eigen kernels (unc_eig_cameras / unc_eig_points); the product package never
 imports it.
 
 Pinning:

@@ -2,7 +2,8 @@
  * oracle.h -- CPU restatement of uncertaintyTE's TAYLOR_EXPANSION covariance
  * pipeline (src/uncertaintyTE/compute.cpp:458-540 and the containers it calls).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the MI355X path:
+ * TEST INFRASTRUCTURE ONLY.  This library gives the reference results the MI355X
+ * path is compared with:
  * it may be linked or loaded by tests/, by __graft_entry__.smoke() and by the
  * cpu_baseline leg of bench.py, and by nothing else.  The product library
  * (uncertaintyte_amd/) never calls into it.

@@ -3,8 +3,8 @@ builds with Ceres (src/uncertaintyTE/snavely_reprojection_error.h:110-302,
 composed by JacobianComposer::scene2Jacobian, JacobianComposer.cpp:84-194),
 and their Jacobian by central finite differences.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
-Jacobian kernel (unc_scene_jacobian); the product package never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference results for the
+device Jacobian kernel (unc_scene_jacobian); the product package never imports it.
 
 Pinning: the residual is written the way the reference's functors are (the
 angle-axis rotation with its small-angle branch, :110-140); the derivatives

@@ -2,8 +2,8 @@
 (src/uncertaintyTE/compute.cpp:483-495 -> svdInverse :185-249, camera output
 removeScaleJ4Z :168-183 with a NULL scale).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device SVD
-path; the product package never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference results for the
+device SVD path; the product package never imports it.
 
 It follows the reference literally: the dense, unscaled M = J^T J over every
 camera and point, LAPACK's dgesvd (SVD_QR_ITERATION) or dgesdd
